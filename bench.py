@@ -1,0 +1,209 @@
+"""Benchmark of the D3Q15 collide-stream step on one MI355X.
+
+Default workload: 256^3 periodic domain decomposed into 64 boxes of 64^3,
+FP64, tau 0.6, rho = 1 + 1e-3 N(0,1), u_a ~ U(-0.01, 0.01) (seed 2024).
+A step is one lattice-Boltzmann time step over the whole domain.  The timed
+region is one Iterate(K) call (K steps) with the state resident in HBM,
+bracketed by stream synchronisations.
+
+Prints one JSON line.  `roofline` is computed for the fused stream-collide
+kernel from its average duration, measured with HIP events on the library's
+own stream; algorithmic bytes are 240 B per lattice update (15 x 8 B read +
+15 x 8 B written) in FP64 and 120 B in FP32-shifted storage.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from lambrex_amd import lbx, workloads  # noqa: E402
+
+METRIC = "MLUPS & % HBM roofline (FP64 vs FP32-shifted); p50/p99 latency of batched runs"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LU = {lbx.LBX_FP64: 240, lbx.LBX_FP32_SHIFTED: 120}
+TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup):
+    boxes = workloads.chop(dims, mgs)
+    sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
+    try:
+        sim.define_level(0, boxes)
+        sim.set_tau(0, tau)
+        sim.init_from_fields(rho, u)
+        c0 = sim.conservation()
+        if warmup:
+            sim.iterate(warmup)
+        ctx.sync()
+        ctx.profile_enable(True)
+        t0 = time.perf_counter()
+        sim.iterate(steps)
+        ctx.sync()
+        t1 = time.perf_counter()
+        kern_ms, launches = ctx.profile_read()
+        ctx.profile_enable(False)
+        c1 = sim.conservation()
+    finally:
+        sim.close()
+    cells = dims[0] * dims[1] * dims[2]
+    wall = t1 - t0
+    avg_ms = kern_ms / launches if launches else float("nan")
+    achieved = BYTES_PER_LU[precision] * cells / (avg_ms * 1e-3) / 1e9
+    return {
+        "boxes": len(boxes),
+        "mlups": cells * steps / wall / 1e6,
+        "ms_per_step": wall * 1e3 / steps,
+        "kernel_avg_ms": avg_ms,
+        "kernel_launches": launches,
+        "kernel_gbs": achieved,
+        "mass_drift_rel": abs(c1[0] - c0[0]) / abs(c0[0]),
+    }
+
+
+def traffic_for(kernel, workload):
+    """HBM bytes per dispatch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    try:
+        data = json.loads(TRAFFIC_FILE.read_text())
+        return data[kernel][workload]["bytes_per_dispatch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(dims, mgs, tau, rho, u, steps):
+    """The CPU oracle (the reference's algorithm restated in C, serial, same
+    FillPatch / collide / FillBoundary / stream structure) on the same
+    workload, timed for a bounded number of steps."""
+    from oracle import oracle as orc
+    boxes = workloads.chop(dims, mgs)
+    s = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in boxes])
+    s.init(np.ascontiguousarray(rho.transpose(2, 1, 0)).reshape(-1),
+           np.ascontiguousarray(u.transpose(3, 2, 1, 0)).reshape(-1))
+    t0 = time.perf_counter()
+    s.iterate(steps)
+    t1 = time.perf_counter()
+    s.close()
+    cells = dims[0] * dims[1] * dims[2]
+    return cells * steps / (t1 - t0) / 1e6, t1 - t0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="cubic domain edge")
+    ap.add_argument("--max-grid-size", type=int, default=64)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-shifted measurement")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the domain would shard by boxes with one halo exchange per step; see DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    n = args.n
+    dims = (n, n, n)
+    tau = 0.6
+    seed = 2024
+    log(f"[bench] generating {n}^3 state (seed {seed})")
+    rho, u = workloads.random_state(dims, seed=seed, rho_sigma=1e-3, u_amp=0.01)
+
+    with lbx.Context(0) as ctx:
+        log(f"[bench] FP64 {n}^3 boxes of {args.max_grid_size}^3: warmup {args.warmup}, steps {args.steps}")
+        r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup)
+        log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
+            f"({r64['kernel_gbs']:.0f} GB/s)")
+        r32 = None
+        if not args.no_fp32:
+            log("[bench] FP32-shifted, same workload")
+            r32 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP32_SHIFTED,
+                          args.steps, args.warmup)
+            log(f"[bench] FP32-shifted: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_avg_ms']:.4f} ms "
+                f"({r32['kernel_gbs']:.0f} GB/s)")
+
+    workload = f"{n}^3 periodic, {r64['boxes']} boxes of {args.max_grid_size}^3, tau {tau}"
+    kernel = "k_step_f64<STEP_FUSED>"
+    traffic = traffic_for(kernel, f"{n}^3")
+    cells = n ** 3
+    out = {
+        "metric": METRIC,
+        "value": round(r64["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 / Box-Muller initial state, equilibrium populations)",
+        "config": {"workload": workload, "cells": cells, "boxes": r64["boxes"], "tau": tau,
+                   "seed": seed, "precision": "FP64"},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(r64["kernel_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(r64["kernel_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": kernel,
+            "kernel_avg_ms": round(r64["kernel_avg_ms"], 5),
+            "algorithmic_bytes_per_launch": BYTES_PER_LU[lbx.LBX_FP64] * cells,
+        },
+        "mass_drift_rel": r64["mass_drift_rel"],
+    }
+    if r32:
+        out["fp32_shifted"] = {
+            "value": round(r32["mlups"], 1),
+            "unit": "MLUPS",
+            "ms_per_step": round(r32["ms_per_step"], 5),
+            "speedup_vs_fp64": round(r32["mlups"] / r64["mlups"], 3),
+            "roofline": {"bound": "hbm", "achieved": round(r32["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(r32["kernel_gbs"] / HBM_PEAK_GBS, 4),
+                         "kernel": "k_step_f32s<STEP_FUSED>",
+                         "kernel_avg_ms": round(r32["kernel_avg_ms"], 5)},
+        }
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline: oracle, 1 thread, {n}^3 as {r64['boxes']} boxes, {args.cpu_steps} step(s)")
+        mlups, secs = cpu_baseline(dims, args.max_grid_size, tau, rho, u, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": round(mlups, 3),
+            "unit": "MLUPS",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} step(s) of the same {workload} FP64 workload "
+                      f"({secs:.1f} s), serial C restatement of the reference; host CPU {cpu_model()}, "
+                      f"{os.cpu_count()} logical cores present",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+/*
+ * lbx.h -- C ABI of the MI355X-native D3Q15 collide-stream library
+ * (liblbx.so, built from lambrex_amd/csrc).
+ *
+ * The reference (otbrown/LAMBReX) keeps this path inside the C++ class
+ * AmrSim on top of AMReX MultiFabs (include/AmrSim.h:23-155).  This ABI is
+ * what that class's protected machinery becomes: every distribution array
+ * lives on one GPU, owned by a context; the host passes plain integers,
+ * doubles and host buffers.  The C++ AmrSim facade
+ * (lambrex_amd/host/include/AmrSim.h) sits on top of it with the
+ * reference's public API.
+ *
+ * Conventions
+ *   - Every call returns LBX_OK (0) or a negative LBX_E* code and never
+ *     aborts or throws; lbx_last_error() describes the last failure.
+ *   - One host thread per context; each context owns one HIP stream
+ *     (default priority).  Calls are stream-ordered; calls that hand data
+ *     back to the host synchronise before returning.
+ *   - Boxes are inclusive integer boxes, 6 int32 per box: lo x,y,z then
+ *     hi x,y,z, in the level's own index space (fine = 2 x coarse).
+ *   - Host field buffers are x fastest: rho[nz][ny][nx],
+ *     u[3][nz][ny][nx], f[15][nz][ny][nx] over the box's valid cells, with
+ *     populations in the reference order (include/d3q15_bgk.h:14-22).
+ */
+#ifndef LBX_H
+#define LBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBX_OK 0
+#define LBX_E_ARG (-1)      /* invalid argument */
+#define LBX_E_STATE (-2)    /* call not valid in the current state */
+#define LBX_E_HIP (-3)      /* HIP runtime error */
+#define LBX_E_NOMEM (-4)    /* device or host allocation failed */
+#define LBX_E_UNSUPPORTED (-5)
+
+#define LBX_FP64 0          /* populations stored and collided in FP64 */
+#define LBX_FP32_SHIFTED 1  /* stored as f_i - w_i*rho0 in FP32, collided in FP32 */
+
+typedef struct lbx_ctx lbx_ctx;
+typedef struct lbx_sim lbx_sim;
+
+/* Process / device set-up.  Replaces lambrexInit()/lambrexFinalise()
+ * (include/lambrex.h:6-7, src/lambrex.cpp:4-14): creates the device
+ * context (stream, scratch) that all simulations on that GPU share. */
+int lbx_ctx_create(int device, lbx_ctx** out);
+int lbx_ctx_destroy(lbx_ctx* ctx);
+
+/* Last error message of this context (ctx may be NULL for errors raised
+ * before a context existed).  Replaces amrex::Abort messages
+ * (src/AmrSim.cpp:211, 291, 399, 690, 795). */
+const char* lbx_last_error(const lbx_ctx* ctx);
+
+/* Wait for all work queued on the context's stream. */
+int lbx_sync(lbx_ctx* ctx);
+
+/* Free and total device memory (bytes) as seen by the context's device. */
+int lbx_mem_info(lbx_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+/* A simulation on the periodic domain [0,dims).  Replaces the AmrSim
+ * constructor's geometry and boundary set-up (src/AmrSim.cpp:753-802);
+ * like the reference, any non-periodic direction is rejected
+ * (LBX_E_UNSUPPORTED, src/AmrSim.cpp:794-796).  rho0 is the reference
+ * density of LBX_FP32_SHIFTED storage (ignored for LBX_FP64). */
+int lbx_sim_create(lbx_ctx* ctx, const int32_t dims[3], const int32_t periodic[3],
+                   int precision, double rho0, lbx_sim** out);
+int lbx_sim_destroy(lbx_sim* sim);
+
+/* Define the boxes of a level and allocate its device storage.  Level 0's
+ * boxes must tile the domain exactly.  Replaces
+ * MakeNewLevelFromScratch/MakeNewLevelFromCoarse's MultiFab definitions
+ * (src/AmrSim.cpp:665-713, include/multilevel.h:62-67). */
+int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
+
+/* Release a level's device storage (ClearLevel, src/AmrSim.cpp:746-751). */
+int lbx_level_clear(lbx_sim* sim, int level);
+
+/* Relaxation rates of a level, omega = 1/(tau + 1/2) computed by the caller
+ * exactly as src/AmrSim.cpp:125-126. */
+int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b);
+
+/* Equilibrium initialisation of one box from host density and velocity
+ * (CalcEquilibriumDist, src/AmrSim.cpp:845-936, same operation order). */
+int lbx_init_equilibrium(lbx_sim* sim, int level, int box, const double* rho, const double* u);
+
+/* Copy one box's populations in or out (FP64 on the host side whatever the
+ * storage precision).  There is no public reference counterpart; the
+ * reference's tests reach the MultiFabs through AmrTest
+ * (tests/AmrTest.h:13-39). */
+int lbx_upload_f(lbx_sim* sim, int level, int box, const double* f);
+int lbx_download_f(lbx_sim* sim, int level, int box, double* f);
+
+/* Advance nsteps level-0 steps (Iterate, src/AmrSim.cpp:981-993): the
+ * single-level collide-stream step (src/AmrSim.cpp:124-135, 109-122,
+ * 324-333) on every box.  Stream-ordered: may return before the GPU
+ * finishes; lbx_sync() waits. */
+int lbx_iterate(lbx_sim* sim, int nsteps);
+
+/* Density and velocity of one box's valid cells (CalcHydroVars,
+ * src/AmrSim.cpp:938-979).  Synchronous. */
+int lbx_hydro(lbx_sim* sim, int level, int box, double* rho, double* u);
+
+/* FP64 global sums over a level's valid cells: out[0] = sum rho,
+ * out[1..3] = sum rho*u.  No reference counterpart (the per-level mass at
+ * src/AmrSim.cpp:312,318 is never used); deterministic order. */
+int lbx_conservation(lbx_sim* sim, int level, double out[4]);
+
+/* Timing of the stream-collide kernel with HIP events recorded on the
+ * context's stream around every launch while enabled.  *ms is the summed
+ * kernel time, *launches the number of timed launches since the last
+ * reset. */
+int lbx_profile_enable(lbx_ctx* ctx, int enable);
+int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+// lbx_internal.h -- declarations shared by the C-ABI implementation
+// (lbx_abi.cpp) and the HIP kernels (lbx_lattice.hip).  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lbx {
+
+constexpr int NV_TOTAL = 15;  // populations per cell (D3Q15)
+
+// A periodic lattice: one level stored as a single SoA array over the whole
+// domain, population-major, x fastest.  Population q of cell (x,y,z) lives at
+// q*qs + (z*ny + y)*nx + x.  Boxes of the level are index ranges into it, so
+// neighbouring boxes exchange nothing: the pull reads the neighbour box's
+// cells directly (periodic wrap at the domain faces).
+struct Lattice {
+  int nx, ny, nz;
+  long long qs;  // population stride in elements (multiple of 64)
+};
+
+struct Relax {
+  double omega_s, omega_b;
+  double rho0;  // FP32-shifted storage reference density
+};
+
+// Region of a lattice (inclusive lo, extent n) for box-wise transfers.
+struct Region {
+  int lo[3];
+  int n[3];
+};
+
+enum StepKind { STEP_FUSED = 0, STEP_COLLIDE = 1, STEP_STREAM = 2 };
+
+// Launch configuration of the lattice stream-collide kernels.
+struct LaunchShape {
+  int bx, by;  // block extent in x and y (bx*by <= 256)
+};
+
+LaunchShape default_shape(const Lattice& L);
+
+// FP64 storage
+hipError_t launch_step_f64(StepKind kind, const double* src, double* dst, const Lattice& L,
+                           const Relax& r, LaunchShape s, hipStream_t st);
+// FP32 shifted storage
+hipError_t launch_step_f32s(StepKind kind, const float* src, float* dst, const Lattice& L,
+                            const Relax& r, LaunchShape s, hipStream_t st);
+
+// Equilibrium of a region from device rho[nz][ny][nx], u[3][nz][ny][nx].
+hipError_t launch_equilibrium(int precision, void* f, const Lattice& L, const Region& R,
+                              const double* rho, const double* u, double rho0, hipStream_t st);
+// Copy a region between the lattice and a packed FP64 [15][n2][n1][n0] buffer.
+hipError_t launch_pack_f(int precision, const void* f, const Lattice& L, const Region& R,
+                         double* out, double rho0, hipStream_t st);
+hipError_t launch_unpack_f(int precision, void* f, const Lattice& L, const Region& R,
+                           const double* in, double rho0, hipStream_t st);
+// Density and velocity of a region into packed FP64 buffers.
+hipError_t launch_hydro(int precision, const void* f, const Lattice& L, const Region& R,
+                        double* rho, double* u, double rho0, hipStream_t st);
+// FP64 sums over the whole lattice: partials[nblocks*4] then out[4].
+hipError_t launch_conservation(int precision, const void* f, const Lattice& L, double rho0,
+                               double* partials, int nblocks, double* out, hipStream_t st);
+
+}  // namespace lbx

@@ -1,0 +1,342 @@
+// lbx_lattice.hip -- stream-collide and field kernels on periodic lattices
+// (gfx950).
+//
+// The reference step (src/AmrSim.cpp:124-135, 109-122; include/AmrSim.h:89-94)
+// is: copy now -> next with ghosts, collide valid cells, refill ghosts,
+// pull-stream valid+1 ghost layer into a fresh array, swap.  That is four
+// passes over 15 populations per step.  Here one pass does pull + collide:
+// h'(x) = C(S(h))(x), reading population q of the upstream cell x - c_q and
+// writing the collided populations of x.  Populations are SoA (one plane
+// per q, x fastest), so every load and store of a wavefront is a contiguous
+// 64-element run (shifted by at most one element for c_x = +-1).
+//
+// Bandwidth bound: 15 loads + 15 stores per lattice update and ~430 FP64
+// add/mul (+5 divisions).  No MFMA: there is no dense contraction here.
+#include <hip/hip_runtime.h>
+
+#include "d3q15.h"
+#include "lbx_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace lbx {
+namespace {
+
+// Source offsets of the 15 populations of cell (x, y, z) on a periodic
+// lattice.  xs/ys/zs are indexed by 1 + c: the upstream coordinate of
+// x is x - c, so index 0 (c = -1) holds x+1, 1 holds x, 2 (c = +1) holds x-1.
+struct Upstream {
+  int xs[3], ys[3], zs[3];
+};
+
+__device__ __forceinline__ Upstream upstream(int x, int y, int z, const Lattice& L) {
+  Upstream u;
+  const int row = L.nx;
+  const int plane = L.nx * L.ny;
+  u.xs[0] = (x == L.nx - 1) ? 0 : x + 1;
+  u.xs[1] = x;
+  u.xs[2] = (x == 0) ? L.nx - 1 : x - 1;
+  u.ys[0] = ((y == L.ny - 1) ? 0 : y + 1) * row;
+  u.ys[1] = y * row;
+  u.ys[2] = ((y == 0) ? L.ny - 1 : y - 1) * row;
+  u.zs[0] = ((z == L.nz - 1) ? 0 : z + 1) * plane;
+  u.zs[1] = z * plane;
+  u.zs[2] = ((z == 0) ? L.nz - 1 : z - 1) * plane;
+  return u;
+}
+
+template <typename S, bool STREAM>
+__device__ __forceinline__ void load_cell(const S* __restrict__ src, long long qs, int self,
+                                          const Upstream& u, S* f) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const int off = STREAM ? (u.zs[1 + kC[q][2]] + u.ys[1 + kC[q][1]] + u.xs[1 + kC[q][0]]) : self;
+    f[q] = src[q * qs + off];
+  }
+}
+
+template <typename S>
+__device__ __forceinline__ void store_cell(S* __restrict__ dst, long long qs, int self, const S* f) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) __builtin_nontemporal_store(f[q], dst + q * qs + self);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ src,
+                                                  double* __restrict__ dst, Lattice L, Relax r) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= L.nx || y >= L.ny) return;
+  const int self = (z * L.ny + y) * L.nx + x;
+  double f[NV];
+  if (KIND == STEP_COLLIDE) {
+    load_cell<double, false>(src, L.qs, self, Upstream{}, f);
+  } else {
+    const Upstream u = upstream(x, y, z, L);
+    load_cell<double, true>(src, L.qs, self, u, f);
+  }
+  if (KIND != STEP_STREAM) collide_f64(f, r.omega_s, r.omega_b);
+  store_cell<double>(dst, L.qs, self, f);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_step_f32s(const float* __restrict__ src,
+                                                   float* __restrict__ dst, Lattice L, Relax r) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= L.nx || y >= L.ny) return;
+  const int self = (z * L.ny + y) * L.nx + x;
+  float f[NV];
+  if (KIND == STEP_COLLIDE) {
+    load_cell<float, false>(src, L.qs, self, Upstream{}, f);
+  } else {
+    const Upstream u = upstream(x, y, z, L);
+    load_cell<float, true>(src, L.qs, self, u, f);
+  }
+  if (KIND != STEP_STREAM)
+    collide_f32_shifted(f, float(r.rho0), float(r.omega_s), float(r.omega_b));
+  store_cell<float>(dst, L.qs, self, f);
+}
+
+dim3 grid_of(const Lattice& L, LaunchShape s) {
+  return dim3((L.nx + s.bx - 1) / s.bx, (L.ny + s.by - 1) / s.by, L.nz);
+}
+
+// ------------------------------------------------------------ field kernels
+
+__device__ __forceinline__ int region_cell(const Region& R, int i, int& x, int& y, int& z) {
+  const int n0 = R.n[0], n1 = R.n[1];
+  const int a = i % n0;
+  const int t = i / n0;
+  const int b = t % n1;
+  const int c = t / n1;
+  x = R.lo[0] + a;
+  y = R.lo[1] + b;
+  z = R.lo[2] + c;
+  return i;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_equilibrium(S* __restrict__ f, Lattice L, Region R,
+                                                     const double* __restrict__ rho,
+                                                     const double* __restrict__ u, double rho0) {
+  const int n = R.n[0] * R.n[1] * R.n[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  region_cell(R, i, x, y, z);
+  const int self = (z * L.ny + y) * L.nx + x;
+  double fe[NV];
+  equilibrium_f64(rho[i], u[i], u[n + i], u[2 * n + i], fe);
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if constexpr (sizeof(S) == 8)
+      f[q * L.qs + self] = fe[q];
+    else
+      f[q * L.qs + self] = float(fe[q] - kT.W[q] * rho0);
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_pack(const S* __restrict__ f, Lattice L, Region R,
+                                              double* __restrict__ out, double rho0) {
+  const int n = R.n[0] * R.n[1] * R.n[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  region_cell(R, i, x, y, z);
+  const int self = (z * L.ny + y) * L.nx + x;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if constexpr (sizeof(S) == 8)
+      out[q * (long long)n + i] = f[q * L.qs + self];
+    else
+      out[q * (long long)n + i] = kT.W[q] * rho0 + double(f[q * L.qs + self]);
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_unpack(S* __restrict__ f, Lattice L, Region R,
+                                                const double* __restrict__ in, double rho0) {
+  const int n = R.n[0] * R.n[1] * R.n[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  region_cell(R, i, x, y, z);
+  const int self = (z * L.ny + y) * L.nx + x;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if constexpr (sizeof(S) == 8)
+      f[q * L.qs + self] = in[q * (long long)n + i];
+    else
+      f[q * L.qs + self] = float(in[q * (long long)n + i] - kT.W[q] * rho0);
+  }
+}
+
+// FP64 populations of one cell from either storage.
+template <typename S>
+__device__ __forceinline__ void cell_f64(const S* __restrict__ f, long long qs, int self,
+                                         double rho0, double* g) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if constexpr (sizeof(S) == 8)
+      g[q] = f[q * qs + self];
+    else
+      g[q] = kT.W[q] * rho0 + double(f[q * qs + self]);
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_hydro(const S* __restrict__ f, Lattice L, Region R,
+                                               double* __restrict__ rho, double* __restrict__ u,
+                                               double rho0) {
+  const int n = R.n[0] * R.n[1] * R.n[2];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int x, y, z;
+  region_cell(R, i, x, y, z);
+  const int self = (z * L.ny + y) * L.nx + x;
+  double g[NV];
+  cell_f64<S>(f, L.qs, self, rho0, g);
+  double r, u0, u1, u2;
+  hydro_f64(g, r, u0, u1, u2);
+  rho[i] = r;
+  u[i] = u0;
+  u[n + i] = u1;
+  u[2 * n + i] = u2;
+}
+
+// Block sum of 4 doubles in a fixed order: wavefront butterfly, then the
+// wavefront partials in index order.
+__device__ __forceinline__ void block_sum4(double v[4], double* out) {
+  __shared__ double part[4][4];  // [wave][component], 256 threads = 4 waves
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double s = v[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    v[c] = s;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int c = 0; c < 4; ++c) part[wave][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 4; ++c) out[c] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_conservation(const S* __restrict__ f, Lattice L,
+                                                      double rho0, double* __restrict__ partials) {
+  const long long n = (long long)L.nx * L.ny * L.nz;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    double g[NV];
+    cell_f64<S>(f, L.qs, int(i), rho0, g);
+    acc[0] += mode_of<0>(g, std::make_integer_sequence<int, NV>{});
+    acc[1] += mode_of<1>(g, std::make_integer_sequence<int, NV>{});
+    acc[2] += mode_of<2>(g, std::make_integer_sequence<int, NV>{});
+    acc[3] += mode_of<3>(g, std::make_integer_sequence<int, NV>{});
+  }
+  block_sum4(acc, partials + 4 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partials,
+                                                         int nblocks, double* __restrict__ out) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += 256)
+    for (int c = 0; c < 4; ++c) acc[c] += partials[4 * b + c];
+  __shared__ double res[4];
+  block_sum4(acc, res);
+  __syncthreads();
+  if (threadIdx.x < 4) out[threadIdx.x] = res[threadIdx.x];
+}
+
+int blocks_for(long long n) { return int((n + 255) / 256); }
+
+}  // namespace
+
+LaunchShape default_shape(const Lattice& L) {
+  LaunchShape s;
+  s.bx = L.nx < 256 ? L.nx : 256;
+  s.by = 256 / s.bx;
+  if (s.by > L.ny) s.by = L.ny;
+  if (s.by < 1) s.by = 1;
+  return s;
+}
+
+hipError_t launch_step_f64(StepKind kind, const double* src, double* dst, const Lattice& L,
+                           const Relax& r, LaunchShape s, hipStream_t st) {
+  const dim3 grid = grid_of(L, s), block(s.bx, s.by, 1);
+  switch (kind) {
+    case STEP_FUSED: k_step_f64<STEP_FUSED><<<grid, block, 0, st>>>(src, dst, L, r); break;
+    case STEP_COLLIDE: k_step_f64<STEP_COLLIDE><<<grid, block, 0, st>>>(src, dst, L, r); break;
+    case STEP_STREAM: k_step_f64<STEP_STREAM><<<grid, block, 0, st>>>(src, dst, L, r); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_step_f32s(StepKind kind, const float* src, float* dst, const Lattice& L,
+                            const Relax& r, LaunchShape s, hipStream_t st) {
+  const dim3 grid = grid_of(L, s), block(s.bx, s.by, 1);
+  switch (kind) {
+    case STEP_FUSED: k_step_f32s<STEP_FUSED><<<grid, block, 0, st>>>(src, dst, L, r); break;
+    case STEP_COLLIDE: k_step_f32s<STEP_COLLIDE><<<grid, block, 0, st>>>(src, dst, L, r); break;
+    case STEP_STREAM: k_step_f32s<STEP_STREAM><<<grid, block, 0, st>>>(src, dst, L, r); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_equilibrium(int precision, void* f, const Lattice& L, const Region& R,
+                              const double* rho, const double* u, double rho0, hipStream_t st) {
+  const long long n = (long long)R.n[0] * R.n[1] * R.n[2];
+  if (precision == 0)
+    k_equilibrium<double><<<blocks_for(n), 256, 0, st>>>((double*)f, L, R, rho, u, rho0);
+  else
+    k_equilibrium<float><<<blocks_for(n), 256, 0, st>>>((float*)f, L, R, rho, u, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_f(int precision, const void* f, const Lattice& L, const Region& R,
+                         double* out, double rho0, hipStream_t st) {
+  const long long n = (long long)R.n[0] * R.n[1] * R.n[2];
+  if (precision == 0)
+    k_pack<double><<<blocks_for(n), 256, 0, st>>>((const double*)f, L, R, out, rho0);
+  else
+    k_pack<float><<<blocks_for(n), 256, 0, st>>>((const float*)f, L, R, out, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_f(int precision, void* f, const Lattice& L, const Region& R,
+                           const double* in, double rho0, hipStream_t st) {
+  const long long n = (long long)R.n[0] * R.n[1] * R.n[2];
+  if (precision == 0)
+    k_unpack<double><<<blocks_for(n), 256, 0, st>>>((double*)f, L, R, in, rho0);
+  else
+    k_unpack<float><<<blocks_for(n), 256, 0, st>>>((float*)f, L, R, in, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_hydro(int precision, const void* f, const Lattice& L, const Region& R,
+                        double* rho, double* u, double rho0, hipStream_t st) {
+  const long long n = (long long)R.n[0] * R.n[1] * R.n[2];
+  if (precision == 0)
+    k_hydro<double><<<blocks_for(n), 256, 0, st>>>((const double*)f, L, R, rho, u, rho0);
+  else
+    k_hydro<float><<<blocks_for(n), 256, 0, st>>>((const float*)f, L, R, rho, u, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_conservation(int precision, const void* f, const Lattice& L, double rho0,
+                               double* partials, int nblocks, double* out, hipStream_t st) {
+  if (precision == 0)
+    k_conservation<double><<<nblocks, 256, 0, st>>>((const double*)f, L, rho0, partials);
+  else
+    k_conservation<float><<<nblocks, 256, 0, st>>>((const float*)f, L, rho0, partials);
+  k_reduce_partials<<<1, 256, 0, st>>>(partials, nblocks, out);
+  return hipGetLastError();
+}
+
+}  // namespace lbx

@@ -1,0 +1,263 @@
+// AmrSim.cpp -- the reference's AmrSim API (src/AmrSim.cpp) as a facade over
+// the C ABI; all distribution-function work happens on the GPU.
+#include "AmrSim.h"
+
+#include <cmath>
+#include <limits>
+
+namespace {
+constexpr int kRefRatio = 2;  // ratio 2 on every level (src/AmrSim.cpp:766)
+}
+
+AmrSim::AmrSim(int nx, int ny, int nz, int max_ref_level,
+               const std::array<int, NDIMS>& periodicity, double tau_s, double tau_b)
+    : NX(nx), NY(ny), NZ(nz), NUMEL(nx * ny * nz), PERIODICITY(periodicity),
+      max_level_(max_ref_level) {
+  if (nx < 1 || ny < 1 || nz < 1) throw lambrex::Error("domain extents must be positive");
+  if (max_ref_level < 0) throw lambrex::Error("max_ref_level must be >= 0");
+  for (int d = 0; d < NDIMS; ++d)
+    if (!PERIODICITY[d])  // src/AmrSim.cpp:788-797
+      throw lambrex::Error("Currently only periodic boundary conditions allowed.");
+  const int num_levels = max_level_ + 1;
+  tau_s_.assign(num_levels, 0.0);
+  tau_b_.assign(num_levels, 0.0);
+  mass_.assign(num_levels, 0.0);
+  levels_.resize(num_levels);
+  tau_s_.at(0) = tau_s;
+  tau_b_.at(0) = tau_b;
+}
+
+AmrSim::~AmrSim() {
+  if (sim_) lbx_sim_destroy(sim_);
+}
+
+void AmrSim::Check(int rc) const {
+  if (rc != LBX_OK) {
+    const char* msg = lbx_last_error(lambrex::Context());
+    throw lambrex::Error(std::string("lbx: ") + (msg && *msg ? msg : "error") + " (code " +
+                         std::to_string(rc) + ")");
+  }
+}
+
+void AmrSim::SetMaxGridSize(int max_grid_size) {
+  if (max_grid_size < 1) throw lambrex::Error("max_grid_size must be >= 1");
+  max_grid_size_ = max_grid_size;
+}
+
+void AmrSim::SetPrecision(int precision, double rho0) {
+  if (sim_) throw lambrex::Error("SetPrecision must be called before InitFromScratch");
+  if (precision != LBX_FP64 && precision != LBX_FP32_SHIFTED)
+    throw lambrex::Error("unknown precision");
+  precision_ = precision;
+  rho0_ = rho0;
+}
+
+// src/AmrSim.cpp:804-822
+void AmrSim::SetInitialDensity(double rho_init) { initial_density.assign(NUMEL, rho_init); }
+void AmrSim::SetInitialDensity(std::vector<double> rho_init) { initial_density = std::move(rho_init); }
+void AmrSim::SetInitialVelocity(double u_init) { initial_velocity.assign(3 * NUMEL, u_init); }
+void AmrSim::SetInitialVelocity(std::vector<double> u_init) { initial_velocity = std::move(u_init); }
+
+std::vector<std::array<int, NDIMS>> AmrSim::refRatio() const {
+  return std::vector<std::array<int, NDIMS>>(max_level_ + 1, {kRefRatio, kRefRatio, kRefRatio});
+}
+
+std::array<int, NDIMS> AmrSim::refRatio(int level) const {
+  if (level < 0 || level > max_level_) throw lambrex::Error("refRatio: level out of range");
+  return {kRefRatio, kRefRatio, kRefRatio};
+}
+
+// src/AmrSim.cpp:297-322
+void AmrSim::ComputeDt(int level) {
+  LevelHost& L = levels_.at(level);
+  if (level) {
+    const LevelHost& C = levels_.at(level - 1);
+    const int r = kRefRatio;
+    L.dt = C.dt / r;
+    mass_.at(level) = mass_.at(level - 1) / r;
+    tau_s_.at(level) = r * (tau_s_.at(level - 1) - 0.5) + 0.5;
+    tau_b_.at(level) = r * (tau_b_.at(level - 1) - 0.5) + 0.5;
+  } else {
+    L.dt = 1.0;
+    mass_.at(level) = 1.0;
+  }
+}
+
+long long AmrSim::LocalIndex(const lambrex::IBox& b, int i, int j, int k) const {
+  return ((long long)(k - b.lo[2]) * b.length(1) + (j - b.lo[1])) * b.length(0) + (i - b.lo[0]);
+}
+
+int AmrSim::BoxOf(int level, int i, int j, int k) const {
+  const LevelHost& L = levels_.at(level);
+  for (size_t b = 0; b < L.boxes.size(); ++b)
+    if (L.boxes[b].contains(i, j, k)) return (int)b;
+  return -1;
+}
+
+// src/AmrSim.cpp:138-215: C-order density into the level's boxes.
+void AmrSim::InitDensity(int level) {
+  if (level) throw lambrex::Error("Only level 0 should be initialised from scratch currently.");
+  LevelHost& L = levels_.at(level);
+  for (size_t b = 0; b < L.boxes.size(); ++b) {
+    const lambrex::IBox& B = L.boxes[b];
+    auto& rho = L.rho[L.cur][b];
+    for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+      for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+          rho[LocalIndex(B, i, j, k)] = initial_density.at((size_t)i * NY * NZ + (size_t)j * NZ + k);
+  }
+}
+
+// src/AmrSim.cpp:217-295: C-order velocity (component fastest).
+void AmrSim::InitVelocity(int level) {
+  if (level) throw lambrex::Error("Only LEVEL 0 should be initialised from scratch currently.");
+  LevelHost& L = levels_.at(level);
+  for (size_t b = 0; b < L.boxes.size(); ++b) {
+    const lambrex::IBox& B = L.boxes[b];
+    auto& u = L.vel[b];
+    const long long n = B.numPts();
+    for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+      for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+          for (int c = 0; c < NDIMS; ++c)
+            u[c * n + LocalIndex(B, i, j, k)] =
+                initial_velocity.at(((size_t)i * NY * NZ + (size_t)j * NZ + k) * NDIMS + c);
+  }
+}
+
+// src/AmrSim.cpp:665-687
+void AmrSim::MakeNewLevelFromScratch(int level, double time, const std::vector<lambrex::IBox>& ba) {
+  LevelHost& L = levels_.at(level);
+  L.boxes = ba;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (int s = 0; s < 2; ++s) {
+    L.rho[s].assign(ba.size(), {});
+    for (size_t b = 0; b < ba.size(); ++b) L.rho[s][b].assign(ba[b].numPts(), nan);
+  }
+  L.vel.assign(ba.size(), {});
+  for (size_t b = 0; b < ba.size(); ++b) L.vel[b].assign(3 * ba[b].numPts(), nan);
+  L.cur = 0;
+  const std::vector<int32_t> lohi = lambrex::to_lohi(ba);
+  Check(lbx_level_define(sim_, level, (int)ba.size(), lohi.data()));
+  L.time = time;
+  ComputeDt(level);
+  L.step = 0;
+  Check(lbx_set_relax(sim_, level, 1.0 / (tau_s_.at(level) + 0.5), 1.0 / (tau_b_.at(level) + 0.5)));
+  if (!level) {
+    InitDensity(level);
+    InitVelocity(level);
+    CalcEquilibriumDist(level);
+  }
+}
+
+// AmrCore::InitFromScratch: base grids then level 0 from scratch.
+void AmrSim::InitFromScratch(double time) {
+  if (!sim_) {
+    const int32_t dims[3] = {NX, NY, NZ};
+    const int32_t per[3] = {PERIODICITY[0], PERIODICITY[1], PERIODICITY[2]};
+    Check(lbx_sim_create(lambrex::Context(), dims, per, precision_, rho0_, &sim_));
+  }
+  lambrex::IBox domain;
+  domain.lo = {0, 0, 0};
+  domain.hi = {NX - 1, NY - 1, NZ - 1};
+  MakeNewLevelFromScratch(0, time, lambrex::max_size_chop(domain, max_grid_size_));
+  finest_level_ = 0;
+}
+
+// src/AmrSim.cpp:746-751
+void AmrSim::ClearLevel(int level) {
+  Check(lbx_level_clear(sim_, level));
+  LevelHost& L = levels_.at(level);
+  L = LevelHost{};
+}
+
+// src/AmrSim.cpp:845-936 (the arithmetic runs on the GPU in the same order).
+void AmrSim::CalcEquilibriumDist(int level) {
+  LevelHost& L = levels_.at(level);
+  for (size_t b = 0; b < L.boxes.size(); ++b)
+    Check(lbx_init_equilibrium(sim_, level, (int)b, L.rho[L.cur][b].data(), L.vel[b].data()));
+}
+
+// src/AmrSim.cpp:938-979
+void AmrSim::CalcHydroVars(int level) {
+  LevelHost& L = levels_.at(level);
+  for (size_t b = 0; b < L.boxes.size(); ++b)
+    Check(lbx_hydro(sim_, level, (int)b, L.rho[L.cur][b].data(), L.vel[b].data()));
+}
+
+// src/AmrSim.cpp:824-843
+double AmrSim::GetDensity(int i, int j, int k, int level) const {
+  const LevelHost& L = levels_.at(level);
+  const int b = BoxOf(level, i, j, k);
+  if (b < 0 || L.rho[L.cur].empty()) return NL_DENSITY;
+  return L.rho[L.cur][b][LocalIndex(L.boxes[b], i, j, k)];
+}
+
+double AmrSim::GetVelocity(int i, int j, int k, int n, int level) const {
+  const LevelHost& L = levels_.at(level);
+  const int b = BoxOf(level, i, j, k);
+  if (b < 0 || L.vel.empty()) return NL_VELOCITY;
+  return L.vel[b][n * L.boxes[b].numPts() + LocalIndex(L.boxes[b], i, j, k)];
+}
+
+// src/AmrSim.cpp:981-993 with IterateLevel (:324-333) per step.
+void AmrSim::Iterate(int nsteps) {
+  if (!sim_) throw lambrex::Error("Iterate before InitFromScratch");
+  if (nsteps <= 0) return;
+  if (finest_level_ == 0) {
+    Check(lbx_iterate(sim_, nsteps));
+    LevelHost& L = levels_.at(0);
+    for (int t = 0; t < nsteps; ++t) {
+      L.time += L.dt;
+      ++L.step;
+    }
+    if (nsteps % 2) L.cur = 1 - L.cur;  // UpdateNow swaps Density too
+  } else {
+    throw lambrex::Error("two-level iteration is not available in this build");
+  }
+}
+
+// src/AmrSim.cpp:995-1017
+void AmrSim::SetStaticRefinement(int level, const std::array<int, NDIMS>& lo_corner,
+                                 const std::array<int, NDIMS>& hi_corner) {
+  if (level < 0 || level >= max_level_)
+    throw lambrex::Error("SetStaticRefinement: no finer level allowed below this one");
+  throw lambrex::Error("two-level runs are not available in this build");
+  (void)lo_corner;
+  (void)hi_corner;
+}
+
+void AmrSim::UnsetStaticRefinement(int level) {
+  if (level < 0 || level >= max_level_)
+    throw lambrex::Error("UnsetStaticRefinement: level out of range");
+  if (finest_level_ > level) {
+    for (int l = finest_level_; l > level; --l) ClearLevel(l);
+    finest_level_ = level;
+  }
+}
+
+// src/AmrSim.cpp:1019-1030
+std::pair<std::array<int, NDIMS>, std::array<int, NDIMS>> AmrSim::GetExtent(int level) const {
+  const lambrex::IBox m = lambrex::minimal_box(levels_.at(level).boxes);
+  return {m.lo, m.hi};
+}
+
+std::vector<double> AmrSim::DistFn(int level, int box) const {
+  const LevelHost& L = levels_.at(level);
+  std::vector<double> f((size_t)NMODES * L.boxes.at(box).numPts());
+  Check(lbx_download_f(sim_, level, box, f.data()));
+  return f;
+}
+
+void AmrSim::SetDistFn(int level, int box, const std::vector<double>& f) {
+  const LevelHost& L = levels_.at(level);
+  if (f.size() != (size_t)NMODES * L.boxes.at(box).numPts())
+    throw lambrex::Error("SetDistFn: wrong buffer size");
+  Check(lbx_upload_f(sim_, level, box, f.data()));
+}
+
+std::array<double, 4> AmrSim::Conservation(int level) const {
+  std::array<double, 4> out{};
+  Check(lbx_conservation(sim_, level, out.data()));
+  return out;
+}

@@ -1,0 +1,139 @@
+// AmrSim.h -- LAMBReX's AmrSim API (include/AmrSim.h:23-155 in the
+// reference) over the MI355X C ABI (include/lbx.h).
+//
+// Same public names, argument meaning and state semantics as the reference:
+//   - domain [0,nx)x[0,ny)x[0,nz), fully periodic (anything else is
+//     rejected, src/AmrSim.cpp:788-797);
+//   - initial density / velocity vectors in C order, i slowest, velocity
+//     component fastest (include/AmrSim.h:79-83);
+//   - omega = 1/(tau + 1/2) (src/AmrSim.cpp:125-126);
+//   - GetDensity returns the density array of the current state, which
+//     Iterate swaps every single-level step (include/multilevel.h:75-79), so
+//     callers run CalcHydroVars after Iterate as the reference's tests do;
+//     GetVelocity returns the last computed velocity;
+//   - off-domain queries return the NL_DENSITY / NL_VELOCITY sentinels.
+// Differences, all documented where they matter:
+//   - errors throw lambrex::Error (carrying the reference's message where it
+//     has one) instead of amrex::Abort ending the process;
+//   - AMReX's AmrCore is not available: the members callers used from it
+//     (InitFromScratch, boxArray, refRatio, maxLevel, finestLevel) are
+//     provided here, with base grids chopped like AMReX's
+//     BoxList::maxSize (max_grid_size 32 by default);
+//   - the distribution function lives on the GPU; DistFn()/SetDistFn() and
+//     Device() expose it for tests and drivers.
+#pragma once
+
+#include <array>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "boxes.h"
+#include "lbx.h"
+
+#define NDIMS 3
+#define NMODES 15
+
+namespace lambrex {
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// The process-wide device context created by lambrexInit().
+lbx_ctx* Context();
+
+}  // namespace lambrex
+
+class AmrSim {
+ public:
+  AmrSim(int nx, int ny, int nz, int max_ref_level, const std::array<int, NDIMS>& periodicity,
+         double tau_s, double tau_b);
+  virtual ~AmrSim();
+  AmrSim(const AmrSim&) = delete;
+  AmrSim& operator=(const AmrSim&) = delete;
+
+  // ---- public API of the reference (include/AmrSim.h:128-154)
+  double GetTime(int level) const { return levels_.at(level).time; }
+  int GetTimeStep(int level) const { return levels_.at(level).step; }
+  std::array<int, NDIMS> GetDims() const { return {NX, NY, NZ}; }
+  bool OnProcessDensity(double rho) const { return rho != NL_DENSITY; }
+  bool OnProcessVelocity(double u) const { return u != NL_VELOCITY; }
+  void SetInitialDensity(double rho_init);
+  void SetInitialDensity(std::vector<double> rho_init);
+  void SetInitialVelocity(double u_init);
+  void SetInitialVelocity(std::vector<double> u_init);
+  double GetDensity(int i, int j, int k, int level) const;
+  double GetVelocity(int i, int j, int k, int n, int level) const;
+  void CalcEquilibriumDist(int level);
+  void CalcHydroVars(int level);
+  void Iterate(int nsteps);
+  void SetStaticRefinement(int level, const std::array<int, NDIMS>& lo_corner,
+                           const std::array<int, NDIMS>& hi_corner);
+  void UnsetStaticRefinement(int level);
+  std::pair<std::array<int, NDIMS>, std::array<int, NDIMS>> GetExtent(int level) const;
+
+  // ---- members callers used from amrex::AmrCore
+  void InitFromScratch(double time);
+  const std::vector<lambrex::IBox>& boxArray(int level) const { return levels_.at(level).boxes; }
+  int maxLevel() const { return max_level_; }
+  int finestLevel() const { return finest_level_; }
+  std::vector<std::array<int, NDIMS>> refRatio() const;
+  std::array<int, NDIMS> refRatio(int level) const;
+
+  // ---- extensions (no reference counterpart)
+  // Largest box edge used when the base grids are chopped (AMReX default 32).
+  void SetMaxGridSize(int max_grid_size);
+  // LBX_FP64 (default) or LBX_FP32_SHIFTED with reference density rho0.
+  void SetPrecision(int precision, double rho0);
+  lbx_sim* Device() const { return sim_; }
+  // Populations of the current state of one box, [15][nz][ny][nx] FP64.
+  std::vector<double> DistFn(int level, int box) const;
+  void SetDistFn(int level, int box, const std::vector<double>& f);
+  // {sum rho, sum rho u_x, sum rho u_y, sum rho u_z} over the level, FP64.
+  std::array<double, 4> Conservation(int level) const;
+  double GetTauS(int level) const { return tau_s_.at(level); }
+  double GetTauB(int level) const { return tau_b_.at(level); }
+  double GetDt(int level) const { return levels_.at(level).dt; }
+  double GetMass(int level) const { return mass_.at(level); }
+
+  static constexpr double CS2 = 1.0 / 3.0;
+  static constexpr double NL_DENSITY = -1.0;
+  static constexpr double NL_VELOCITY = -3E8;
+
+ protected:
+  struct LevelHost {
+    std::vector<lambrex::IBox> boxes;
+    // Host copies of the derived fields, per box, x fastest.  Density has
+    // two arrays because the reference swaps it with the state
+    // (include/multilevel.h:75-79); rho[cur] is the current one.
+    std::vector<std::vector<double>> rho[2];
+    std::vector<std::vector<double>> vel;  // [3][n] per box
+    int cur = 0;
+    double time = 0.0, dt = 0.0;
+    int step = 0;
+  };
+
+  void ComputeDt(int level);
+  void MakeNewLevelFromScratch(int level, double time, const std::vector<lambrex::IBox>& ba);
+  void ClearLevel(int level);
+  void InitDensity(int level);
+  void InitVelocity(int level);
+  void Check(int rc) const;
+  int BoxOf(int level, int i, int j, int k) const;
+  long long LocalIndex(const lambrex::IBox& b, int i, int j, int k) const;
+
+  const int NX, NY, NZ, NUMEL;
+  std::array<int, NDIMS> PERIODICITY;
+  int max_level_;
+  int finest_level_ = 0;
+  int max_grid_size_ = 32;
+  int precision_ = LBX_FP64;
+  double rho0_ = 1.0;
+  std::vector<double> initial_density;
+  std::vector<double> initial_velocity;
+  std::vector<double> tau_s_, tau_b_, mass_;
+  std::vector<LevelHost> levels_;
+  lbx_sim* sim_ = nullptr;
+};

@@ -1,0 +1,233 @@
+"""ctypes binding of the C ABI in include/lbx.h (liblbx.so, built in-tree).
+
+This is the Python face of the same drop-in boundary the C++ AmrSim facade
+uses.  There is no fallback: if liblbx.so is missing or no GPU is visible,
+every entry point raises LbxError.
+"""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "liblbx.so"
+
+LBX_OK = 0
+LBX_E_ARG = -1
+LBX_E_STATE = -2
+LBX_E_HIP = -3
+LBX_E_NOMEM = -4
+LBX_E_UNSUPPORTED = -5
+LBX_FP64 = 0
+LBX_FP32_SHIFTED = 1
+
+# Every extern "C" entry point of include/lbx.h with its ctypes signature.
+_vp = ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "lbx_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "lbx_ctx_destroy": (ctypes.c_int, [_vp]),
+    "lbx_last_error": (ctypes.c_char_p, [_vp]),
+    "lbx_sync": (ctypes.c_int, [_vp]),
+    "lbx_mem_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "lbx_sim_create": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "lbx_sim_destroy": (ctypes.c_int, [_vp]),
+    "lbx_level_define": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
+    "lbx_level_clear": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbx_set_relax": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "lbx_init_equilibrium": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "lbx_upload_f": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "lbx_download_f": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "lbx_iterate": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbx_hydro": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "lbx_conservation": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "lbx_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbx_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
+}
+
+_LIB = None
+
+
+class LbxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lbx error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load liblbx.so and bind every entry point (no GPU call is made)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise LbxError(LBX_E_STATE, f"{LIB_PATH} is missing: build it first "
+                                        "(python -c 'import __graft_entry__ as g; g.build()')")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _lohi(boxes):
+    arr = np.array([list(lo) + list(hi) for lo, hi in boxes], dtype=np.int32).reshape(-1)
+    return arr
+
+
+class Context:
+    """A device context: one HIP stream and scratch on one GPU."""
+
+    def __init__(self, device=0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        rc = self._lib.lbx_ctx_create(int(device), ctypes.byref(h))
+        if rc != LBX_OK:
+            raise LbxError(rc, self._lib.lbx_last_error(None).decode())
+        self.h = h
+
+    def check(self, rc):
+        if rc != LBX_OK:
+            raise LbxError(rc, self._lib.lbx_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(self._lib.lbx_sync(self.h))
+
+    def mem_info(self):
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        self.check(self._lib.lbx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
+    def profile_enable(self, on=True):
+        self.check(self._lib.lbx_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self.check(self._lib.lbx_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            self._lib.lbx_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Sim:
+    """A periodic D3Q15 simulation held on the GPU."""
+
+    def __init__(self, ctx, dims, precision=LBX_FP64, rho0=1.0, periodic=(1, 1, 1)):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.dims = tuple(int(d) for d in dims)
+        self.precision = precision
+        d = np.array(self.dims, dtype=np.int32)
+        p = np.array(periodic, dtype=np.int32)
+        h = ctypes.c_void_p()
+        ctx.check(self._lib.lbx_sim_create(ctx.h, _ptr(d, ctypes.c_int32), _ptr(p, ctypes.c_int32),
+                                           int(precision), float(rho0), ctypes.byref(h)))
+        self.h = h
+        self.boxes = {}
+
+    def close(self):
+        if self.h:
+            self._lib.lbx_sim_destroy(self.h)
+            self.h = None
+
+    def define_level(self, level, boxes):
+        lohi = _lohi(boxes)
+        self.ctx.check(self._lib.lbx_level_define(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))
+        self.boxes[level] = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+
+    def set_relax(self, level, omega_s, omega_b):
+        self.ctx.check(self._lib.lbx_set_relax(self.h, level, float(omega_s), float(omega_b)))
+
+    def set_tau(self, level, tau_s, tau_b=None):
+        """omega = 1/(tau + 1/2) in double, as src/AmrSim.cpp:125-126."""
+        tau_b = tau_s if tau_b is None else tau_b
+        self.set_relax(level, 1.0 / (tau_s + 0.5), 1.0 / (tau_b + 0.5))
+
+    @staticmethod
+    def _shape(box):
+        lo, hi = box
+        return (hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+
+    def init_equilibrium(self, level, box, rho, u):
+        shape = self._shape(self.boxes[level][box])
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(shape)
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape((3,) + shape)
+        self.ctx.check(self._lib.lbx_init_equilibrium(self.h, level, box, _ptr(rho, ctypes.c_double),
+                                                      _ptr(u, ctypes.c_double)))
+
+    def upload_f(self, level, box, f):
+        shape = (15,) + self._shape(self.boxes[level][box])
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(shape)
+        self.ctx.check(self._lib.lbx_upload_f(self.h, level, box, _ptr(f, ctypes.c_double)))
+
+    def download_f(self, level, box):
+        f = np.empty((15,) + self._shape(self.boxes[level][box]))
+        self.ctx.check(self._lib.lbx_download_f(self.h, level, box, _ptr(f, ctypes.c_double)))
+        return f
+
+    def iterate(self, n):
+        self.ctx.check(self._lib.lbx_iterate(self.h, int(n)))
+
+    def hydro(self, level, box):
+        shape = self._shape(self.boxes[level][box])
+        rho = np.empty(shape)
+        u = np.empty((3,) + shape)
+        self.ctx.check(self._lib.lbx_hydro(self.h, level, box, _ptr(rho, ctypes.c_double),
+                                           _ptr(u, ctypes.c_double)))
+        return rho, u
+
+    def conservation(self, level=0):
+        out = np.zeros(4)
+        self.ctx.check(self._lib.lbx_conservation(self.h, level, _ptr(out, ctypes.c_double)))
+        return out
+
+    # --- whole-level helpers (level 0 tiles the domain)
+    def level_array(self, level, per_box, ncomp):
+        """Assemble per-box arrays ([ncomp][nz][ny][nx] or [nz][ny][nx]) into
+        one domain array."""
+        nx, ny, nz = self.dims
+        out = np.empty(((ncomp,) if ncomp else ()) + (nz, ny, nx))
+        for (lo, hi), a in zip(self.boxes[level], per_box):
+            out[..., lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = a
+        return out
+
+    def get_f(self, level=0):
+        return self.level_array(level, [self.download_f(level, b) for b in range(len(self.boxes[level]))], 15)
+
+    def set_f(self, f, level=0):
+        for b, (lo, hi) in enumerate(self.boxes[level]):
+            self.upload_f(level, b, f[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+
+    def get_hydro(self, level=0):
+        parts = [self.hydro(level, b) for b in range(len(self.boxes[level]))]
+        return (self.level_array(level, [p[0] for p in parts], 0),
+                self.level_array(level, [p[1] for p in parts], 3))
+
+    def init_from_fields(self, rho, u, level=0):
+        """rho [nz][ny][nx], u [3][nz][ny][nx] over the domain."""
+        for b, (lo, hi) in enumerate(self.boxes[level]):
+            sl = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
+            self.init_equilibrium(level, b, rho[sl], u[(slice(None),) + sl])
+
+
+def c_order_to_xfast(rho_c, u_c, dims):
+    """Reference C-order inputs (i slowest, velocity component fastest,
+    include/AmrSim.h:79-83) to x-fastest rho[nz][ny][nx], u[3][nz][ny][nx]."""
+    nx, ny, nz = dims
+    rho = np.asarray(rho_c, dtype=np.float64).reshape(nx, ny, nz).transpose(2, 1, 0)
+    u = np.asarray(u_c, dtype=np.float64).reshape(nx, ny, nz, 3).transpose(3, 2, 1, 0)
+    return np.ascontiguousarray(rho), np.ascontiguousarray(u)
