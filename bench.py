@@ -30,6 +30,7 @@ METRIC = "MLUPS & % HBM roofline (FP64 vs FP32-shifted); p50/p99 latency of batc
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LU = {lbx.LBX_FP64: 240, lbx.LBX_FP32_SHIFTED: 120}
 TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
+COPY_BYTES = 2 << 30  # 2 GiB read + 2 GiB written per copy dispatch
 
 
 def log(msg):
@@ -118,8 +119,18 @@ def main():
     ap.add_argument("--max-grid-size", type=int, default=64)
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-shifted measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run the copy kernel (counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     args = ap.parse_args()
+
+    if args.copy_only:
+        with lbx.Context(0) as ctx:
+            gbs = ctx.bench_copy(COPY_BYTES, reps=3)
+        print(json.dumps({"copy_bytes_read_per_dispatch": COPY_BYTES,
+                          "copy_bytes_written_per_dispatch": COPY_BYTES, "copy_gbs": round(gbs, 1)}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -135,6 +146,10 @@ def main():
     rho, u = workloads.random_state(dims, seed=seed, rho_sigma=1e-3, u_amp=0.01)
 
     with lbx.Context(0) as ctx:
+        copy_gbs = None
+        if not args.no_copy:
+            copy_gbs = ctx.bench_copy(COPY_BYTES, reps=10)
+            log(f"[bench] copy ceiling: {copy_gbs:.0f} GB/s (2 GiB read + 2 GiB written per pass)")
         log(f"[bench] FP64 {n}^3 boxes of {args.max_grid_size}^3: warmup {args.warmup}, steps {args.steps}")
         r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup)
         log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
@@ -176,6 +191,8 @@ def main():
             "kernel": kernel,
             "kernel_avg_ms": round(r64["kernel_avg_ms"], 5),
             "algorithmic_bytes_per_launch": BYTES_PER_LU[lbx.LBX_FP64] * cells,
+            "copy_ceiling_gbs": round(copy_gbs, 1) if copy_gbs else None,
+            "frac_of_copy_ceiling": round(r64["kernel_gbs"] / copy_gbs, 4) if copy_gbs else None,
         },
         "mass_drift_rel": r64["mass_drift_rel"],
     }

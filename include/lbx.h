@@ -77,12 +77,28 @@ int lbx_sim_destroy(lbx_sim* sim);
  * (src/AmrSim.cpp:665-713, include/multilevel.h:62-67). */
 int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
 
+/* Level 1 (refinement ratio 2, fine index space): boxes must lie in
+ * [0, 2*dims), be aligned to the ratio (lo even, hi odd) and not overlap.
+ * Each fine box is stored with 2 ghost layers like the reference's FABs.
+ * Defining level 0 again releases level 1. */
+
 /* Release a level's device storage (ClearLevel, src/AmrSim.cpp:746-751). */
 int lbx_level_clear(lbx_sim* sim, int level);
 
 /* Relaxation rates of a level, omega = 1/(tau + 1/2) computed by the caller
  * exactly as src/AmrSim.cpp:125-126. */
 int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b);
+
+/* Mask of the coarse collide (MakeFineMask, src/AmrSim.cpp:419-428):
+ * coarse cells inside coarsen(box, 2) of any of the given boxes (fine index
+ * space) have their populations zeroed instead of collided in the
+ * two-level cycle.  The reference passes the coarse BoxArray itself as
+ * these boxes; passing the fine BoxArray masks the refined region. */
+int lbx_set_fine_mask(lbx_sim* sim, int n_boxes, const int32_t* lohi_fine);
+
+/* Fill level 1's valid cells from level 0 by piecewise-constant
+ * interpolation (DistFnFillFromCoarse, src/AmrSim.cpp:393-411). */
+int lbx_fill_from_coarse(lbx_sim* sim, int level);
 
 /* Equilibrium initialisation of one box from host density and velocity
  * (CalcEquilibriumDist, src/AmrSim.cpp:845-936, same operation order). */
@@ -97,8 +113,10 @@ int lbx_download_f(lbx_sim* sim, int level, int box, double* f);
 
 /* Advance nsteps level-0 steps (Iterate, src/AmrSim.cpp:981-993): the
  * single-level collide-stream step (src/AmrSim.cpp:124-135, 109-122,
- * 324-333) on every box.  Stream-ordered: may return before the GPU
- * finishes; lbx_sync() waits. */
+ * 324-333) on every box, or, when level 1 is defined, one Rohde cycle per
+ * step (src/AmrSim.cpp:430-631: masked coarse collide and stream, two fine
+ * substeps, average-down and add).  Stream-ordered: may return before the
+ * GPU finishes; lbx_sync() waits. */
 int lbx_iterate(lbx_sim* sim, int nsteps);
 
 /* Density and velocity of one box's valid cells (CalcHydroVars,
@@ -116,6 +134,13 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]);
  * reset. */
 int lbx_profile_enable(lbx_ctx* ctx, int enable);
 int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches);
+
+/* Achievable read+write bandwidth of this device: a plain streaming copy
+ * of `bytes` (read) into a second buffer (write), repeated `reps` times;
+ * *gbs = 2*bytes*reps / time.  No reference counterpart: it is the measured
+ * ceiling reported beside the stream-collide kernel's bandwidth, and the
+ * calibration kernel for counter-based traffic figures. */
+int lbx_bench_copy(lbx_ctx* ctx, size_t bytes, int reps, double* gbs);
 
 #ifdef __cplusplus
 }

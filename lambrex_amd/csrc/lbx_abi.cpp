@@ -2,6 +2,11 @@
 //
 // Owns device memory and the per-context HIP stream; never aborts: every
 // failure becomes a negative status plus a message in lbx_last_error().
+//
+// Level 0 is a periodic level array (its boxes are index ranges into one
+// SoA array).  Level 1, when defined, is a set of patches, one per fine box,
+// each grown by 2 ghost layers; the tables that drive its ghost fill and its
+// average-down are built here on the host once per level definition.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +22,7 @@
 
 using lbx::Lattice;
 using lbx::LaunchShape;
+using lbx::PatchDesc;
 using lbx::Region;
 using lbx::Relax;
 
@@ -26,19 +32,34 @@ thread_local std::string g_global_error;
 
 struct Box {
   int lo[3], hi[3];
-  long long cells() const {
-    return (long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+  int n(int d) const { return hi[d] - lo[d] + 1; }
+  long long cells() const { return (long long)n(0) * n(1) * n(2); }
+  bool contains(int x, int y, int z) const {
+    return x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2];
   }
 };
 
 struct Level {
   bool defined = false;
+  bool patched = false;  // per-box ghosted storage (fine level)
   std::vector<Box> boxes;
-  Lattice lat{};
-  void* buf[2] = {nullptr, nullptr};  // state lives in buf[cur]
+  Lattice lat{};         // level-array geometry (level 0)
+  void* buf[3] = {nullptr, nullptr, nullptr};  // state in buf[cur]; buf[2] scratch (patched)
   int cur = 0;
   double omega_s = 0.0, omega_b = 0.0;
   bool relax_set = false;
+  // device box/patch descriptors
+  std::vector<PatchDesc> desc;
+  PatchDesc* d_desc = nullptr;
+  int max_cells = 0;
+  // level 0 with a fine level: coarse cells masked in the coarse collide
+  unsigned char* d_mask = nullptr;
+  // level 1: ghost fill sources and the average-down contributions
+  long long* d_fill = nullptr;
+  int* d_sum_cell = nullptr;
+  int* d_sum_start = nullptr;
+  int4* d_sum_contrib = nullptr;
+  int sum_ncell = 0;
 };
 
 constexpr int kPartials = 1024;  // blocks of the conservation reduction
@@ -85,9 +106,9 @@ int hip_fail(lbx_ctx* ctx, hipError_t e, const char* what) {
   return fail(ctx, LBX_E_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define LBX_HIP(ctx, call)                                \
-  do {                                                    \
-    hipError_t e_ = (call);                               \
+#define LBX_HIP(ctx, call)                                 \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
     if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
   } while (0)
 
@@ -99,36 +120,53 @@ bool box_ok(const Box& b, const int* dom) {
   return true;
 }
 
+bool overlap(const Box& a, const Box& b) {
+  for (int d = 0; d < 3; ++d)
+    if (a.hi[d] < b.lo[d] || b.hi[d] < a.lo[d]) return false;
+  return true;
+}
+
+bool any_overlap(const std::vector<Box>& boxes) {
+  for (size_t a = 0; a < boxes.size(); ++a)
+    for (size_t c = a + 1; c < boxes.size(); ++c)
+      if (overlap(boxes[a], boxes[c])) return true;
+  return false;
+}
+
 Region region_of(const Box& b) {
   Region r;
   for (int d = 0; d < 3; ++d) {
     r.lo[d] = b.lo[d];
-    r.n[d] = b.hi[d] - b.lo[d] + 1;
+    r.n[d] = b.n(d);
   }
   return r;
 }
 
-// Boxes of level 0 must tile [0,dims) exactly: disjoint and covering.
-bool tiles_domain(const std::vector<Box>& boxes, const int* dims) {
-  long long total = 0;
-  for (const Box& b : boxes) total += b.cells();
-  if (total != (long long)dims[0] * dims[1] * dims[2]) return false;
-  for (size_t a = 0; a < boxes.size(); ++a)
-    for (size_t c = a + 1; c < boxes.size(); ++c) {
-      bool overlap = true;
-      for (int d = 0; d < 3; ++d)
-        if (boxes[a].hi[d] < boxes[c].lo[d] || boxes[c].hi[d] < boxes[a].lo[d]) overlap = false;
-      if (overlap) return false;
-    }
-  return true;
+int wrap(int v, int n) {
+  const int r = v % n;
+  return r < 0 ? r + n : r;
+}
+int floor_div2(int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+template <typename T>
+hipError_t upload(T** dptr, const std::vector<T>& h, hipStream_t st) {
+  if (h.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(dptr, h.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(*dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
 }
 
 void free_level(Level& L) {
   for (void*& p : L.buf)
-    if (p) {
-      (void)hipFree(p);
-      p = nullptr;
-    }
+    if (p) (void)hipFree(p);
+  if (L.d_desc) (void)hipFree(L.d_desc);
+  if (L.d_mask) (void)hipFree(L.d_mask);
+  if (L.d_fill) (void)hipFree(L.d_fill);
+  if (L.d_sum_cell) (void)hipFree(L.d_sum_cell);
+  if (L.d_sum_start) (void)hipFree(L.d_sum_start);
+  if (L.d_sum_contrib) (void)hipFree(L.d_sum_contrib);
   L = Level{};
 }
 
@@ -178,21 +216,36 @@ int drain_profile(lbx_ctx* ctx) {
   return LBX_OK;
 }
 
+// Event pair around the timed region of one step (when profiling is on).
+struct ProfileScope {
+  lbx_ctx* ctx;
+  hipEvent_t e1 = nullptr;
+  int rc = LBX_OK;
+  ProfileScope(lbx_ctx* c, bool timed) : ctx(c) {
+    if (!ctx->prof || !timed) return;
+    if (ctx->ev_used + 2 > 4096 && (rc = drain_profile(ctx)) != LBX_OK) return;
+    hipEvent_t e0 = take_event(ctx);
+    e1 = take_event(ctx);
+    if (!e0 || !e1) {
+      rc = fail(ctx, LBX_E_HIP, "hipEventCreate failed");
+      e1 = nullptr;
+      return;
+    }
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = fail(ctx, LBX_E_HIP, "hipEventRecord");
+  }
+  int close() {
+    if (e1 && rc == LBX_OK && hipEventRecord(e1, ctx->stream) != hipSuccess)
+      rc = fail(ctx, LBX_E_HIP, "hipEventRecord");
+    return rc;
+  }
+};
+
 int launch_step(lbx_sim* sim, Level& L, lbx::StepKind kind) {
   lbx_ctx* ctx = sim->ctx;
   Relax r{L.omega_s, L.omega_b, sim->rho0};
   const LaunchShape shape = lbx::default_shape(L.lat);
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (ctx->prof && kind == lbx::STEP_FUSED) {
-    if (ctx->ev_used + 2 > 4096) {
-      int rc = drain_profile(ctx);
-      if (rc) return rc;
-    }
-    e0 = take_event(ctx);
-    e1 = take_event(ctx);
-    if (!e0 || !e1) return fail(ctx, LBX_E_HIP, "hipEventCreate failed");
-    LBX_HIP(ctx, hipEventRecord(e0, ctx->stream));
-  }
+  ProfileScope ps(ctx, kind == lbx::STEP_FUSED);
+  if (ps.rc) return ps.rc;
   hipError_t e;
   void* src = L.buf[L.cur];
   void* dst = L.buf[1 - L.cur];
@@ -201,8 +254,227 @@ int launch_step(lbx_sim* sim, Level& L, lbx::StepKind kind) {
   else
     e = lbx::launch_step_f32s(kind, (const float*)src, (float*)dst, L.lat, r, shape, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "stream-collide launch");
-  if (e1) LBX_HIP(ctx, hipEventRecord(e1, ctx->stream));
   L.cur = 1 - L.cur;
+  return ps.close();
+}
+
+// One coarse step of the two-level cycle (src/AmrSim.cpp:430-469).
+int launch_two_level_step(lbx_sim* sim) {
+  lbx_ctx* ctx = sim->ctx;
+  Level& C = sim->lev[0];
+  Level& F = sim->lev[1];
+  const Relax rc{C.omega_s, C.omega_b, sim->rho0};
+  const Relax rf{F.omega_s, F.omega_b, sim->rho0};
+  ProfileScope ps(ctx, true);
+  if (ps.rc) return ps.rc;
+  const double* cnow = (const double*)C.buf[C.cur];
+  double* cnext = (double*)C.buf[1 - C.cur];
+  double* fnow = (double*)F.buf[F.cur];
+  double* fnext = (double*)F.buf[1 - F.cur];
+  double* ftmp = (double*)F.buf[2];
+  const int np = (int)F.boxes.size();
+  LBX_HIP(ctx, lbx::launch_fine_fill_collide(fnow, fnext, cnow, C.lat, F.d_desc, np, F.max_cells,
+                                             F.d_fill, rf, ctx->stream));
+  LBX_HIP(ctx, lbx::launch_fine_stream(0, fnext, ftmp, F.d_desc, np, F.max_cells, rf, ctx->stream));
+  LBX_HIP(ctx, lbx::launch_fine_stream(1, ftmp, fnext, F.d_desc, np, F.max_cells, rf, ctx->stream));
+  F.cur = 1 - F.cur;
+  LBX_HIP(ctx, lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_desc, (int)C.boxes.size(), C.max_cells,
+                                       C.d_mask, rc, ctx->stream));
+  LBX_HIP(ctx, lbx::launch_sum_fine_to_coarse((const double*)F.buf[F.cur], cnext, C.lat, F.d_desc,
+                                              F.d_sum_cell, F.d_sum_start, F.d_sum_contrib,
+                                              F.sum_ncell, ctx->stream));
+  C.cur = 1 - C.cur;
+  return ps.close();
+}
+
+std::vector<Box> read_boxes(int n, const int32_t* lohi) {
+  std::vector<Box> boxes(n);
+  for (int b = 0; b < n; ++b)
+    for (int d = 0; d < 3; ++d) {
+      boxes[b].lo[d] = lohi[6 * b + d];
+      boxes[b].hi[d] = lohi[6 * b + 3 + d];
+    }
+  return boxes;
+}
+
+int define_level0(lbx_sim* sim, const std::vector<Box>& boxes) {
+  lbx_ctx* ctx = sim->ctx;
+  long long total = 0;
+  for (size_t b = 0; b < boxes.size(); ++b) {
+    if (!box_ok(boxes[b], sim->dims))
+      return fail(ctx, LBX_E_ARG, "box %zu lies outside the domain or is empty", b);
+    total += boxes[b].cells();
+  }
+  if (total != (long long)sim->dims[0] * sim->dims[1] * sim->dims[2] || any_overlap(boxes))
+    return fail(ctx, LBX_E_ARG, "level-0 boxes must tile the periodic domain exactly");
+  Level& L = sim->lev[0];
+  free_level(sim->lev[1]);
+  free_level(L);
+  L.boxes = boxes;
+  L.lat.nx = sim->dims[0];
+  L.lat.ny = sim->dims[1];
+  L.lat.nz = sim->dims[2];
+  L.lat.qs = (total + 63) / 64 * 64;
+  const size_t bytes = (size_t)L.lat.qs * lbx::NV_TOTAL * elem_size(sim->precision);
+  for (int i = 0; i < 2; ++i) {
+    hipError_t e = hipMalloc(&L.buf[i], bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(L.buf[i], 0, bytes, ctx->stream);
+    if (e != hipSuccess) {
+      free_level(L);
+      return fail(ctx, LBX_E_NOMEM, "allocation of %zu bytes for level 0 failed: %s", bytes,
+                  hipGetErrorString(e));
+    }
+  }
+  for (const Box& b : boxes) {
+    PatchDesc d{};
+    for (int k = 0; k < 3; ++k) {
+      d.lo[k] = b.lo[k];
+      d.hi[k] = b.hi[k];
+      d.g[k] = b.n(k);
+    }
+    d.cells = (int)b.cells();
+    L.max_cells = std::max(L.max_cells, d.cells);
+    L.desc.push_back(d);
+  }
+  hipError_t e = upload(&L.d_desc, L.desc, ctx->stream);
+  if (e != hipSuccess) {
+    free_level(L);
+    return hip_fail(ctx, e, "level-0 box table");
+  }
+  L.defined = true;
+  return LBX_OK;
+}
+
+// Fine level: patches grown by 2; ghost-fill sources; average-down table.
+int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
+  lbx_ctx* ctx = sim->ctx;
+  Level& C = sim->lev[0];
+  if (!C.defined) return fail(ctx, LBX_E_STATE, "define level 0 before level 1");
+  if (sim->precision != LBX_FP64)
+    return fail(ctx, LBX_E_UNSUPPORTED, "two-level runs use FP64 storage");
+  const int fdom[3] = {2 * sim->dims[0], 2 * sim->dims[1], 2 * sim->dims[2]};
+  for (size_t b = 0; b < boxes.size(); ++b) {
+    if (!box_ok(boxes[b], fdom))
+      return fail(ctx, LBX_E_ARG, "fine box %zu lies outside the refined domain or is empty", b);
+    for (int d = 0; d < 3; ++d)
+      if (boxes[b].lo[d] % 2 != 0 || boxes[b].hi[d] % 2 != 1)
+        return fail(ctx, LBX_E_ARG, "fine box %zu is not aligned to refinement ratio 2", b);
+  }
+  if (any_overlap(boxes)) return fail(ctx, LBX_E_ARG, "fine boxes overlap");
+  Level& F = sim->lev[1];
+  free_level(F);
+  F.boxes = boxes;
+  F.patched = true;
+  long long base = 0, fill_base = 0;
+  for (const Box& b : boxes) {
+    PatchDesc d{};
+    for (int k = 0; k < 3; ++k) {
+      d.lo[k] = b.lo[k];
+      d.hi[k] = b.hi[k];
+      d.g[k] = b.n(k) + 4;
+    }
+    const long long cells = (long long)d.g[0] * d.g[1] * d.g[2];
+    if (cells >= (1ll << 31)) return fail(ctx, LBX_E_ARG, "fine box too large");
+    d.cells = (int)cells;
+    d.qs = (cells + 63) / 64 * 64;
+    d.base = base;
+    d.fill_base = fill_base;
+    base += d.qs * lbx::NV_TOTAL;
+    fill_base += cells;
+    F.max_cells = std::max(F.max_cells, d.cells);
+    F.desc.push_back(d);
+  }
+  const size_t bytes = (size_t)base * sizeof(double);
+  for (int i = 0; i < 3; ++i) {
+    hipError_t e = hipMalloc(&F.buf[i], bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(F.buf[i], 0, bytes, ctx->stream);
+    if (e != hipSuccess) {
+      free_level(F);
+      return fail(ctx, LBX_E_NOMEM, "allocation of %zu bytes for level 1 failed: %s", bytes,
+                  hipGetErrorString(e));
+    }
+  }
+  // Ghost fill sources (FillPatchTwoLevels with piecewise-constant
+  // interpolation, src/AmrSim.cpp:373-388): the fine valid cell holding the
+  // periodic image if a fine box has it, else the coarse parent.
+  std::vector<long long> fill((size_t)fill_base, 0);
+  int hint = 0;
+  for (size_t b = 0; b < boxes.size(); ++b) {
+    const PatchDesc& P = F.desc[b];
+    for (int gz = 0; gz < P.g[2]; ++gz)
+      for (int gy = 0; gy < P.g[1]; ++gy)
+        for (int gx = 0; gx < P.g[0]; ++gx) {
+          const int x = P.lo[0] - 2 + gx, y = P.lo[1] - 2 + gy, z = P.lo[2] - 2 + gz;
+          if (boxes[b].contains(x, y, z)) continue;
+          const int wx = wrap(x, fdom[0]), wy = wrap(y, fdom[1]), wz = wrap(z, fdom[2]);
+          int src = -1;
+          if (boxes[hint].contains(wx, wy, wz)) {
+            src = hint;
+          } else {
+            for (size_t s = 0; s < boxes.size(); ++s)
+              if (boxes[s].contains(wx, wy, wz)) {
+                src = (int)s;
+                break;
+              }
+          }
+          long long d;
+          if (src >= 0) {
+            hint = src;
+            const PatchDesc& S = F.desc[src];
+            const long long li = ((long long)(wz - S.lo[2] + 2) * S.g[1] + (wy - S.lo[1] + 2)) * S.g[0] +
+                                 (wx - S.lo[0] + 2);
+            d = ((long long)src << 32) | li;
+          } else {
+            const long long c = ((long long)(wz / 2) * C.lat.ny + (wy / 2)) * C.lat.nx + (wx / 2);
+            d = -1 - c;
+          }
+          fill[P.fill_base + ((long long)gz * P.g[1] + gy) * P.g[0] + gx] = d;
+        }
+  }
+  // Average-down contributions (sum_fine_to_coarse, src/AmrSim.cpp:592-602):
+  // each fine box's coarsened box grown by one coarse cell, periodic image
+  // in the coarse level; per coarse cell, contributions in box order and,
+  // within a box, in (i, j, k) loop order.
+  struct Contrib {
+    int cell, box, ci, cj, ck;
+  };
+  std::vector<Contrib> list;
+  for (size_t b = 0; b < boxes.size(); ++b) {
+    const Box& B = boxes[b];
+    for (int ci = floor_div2(B.lo[0]) - 1; ci <= floor_div2(B.hi[0]) + 1; ++ci)
+      for (int cj = floor_div2(B.lo[1]) - 1; cj <= floor_div2(B.hi[1]) + 1; ++cj)
+        for (int ck = floor_div2(B.lo[2]) - 1; ck <= floor_div2(B.hi[2]) + 1; ++ck) {
+          const int c = (wrap(ck, C.lat.nz) * C.lat.ny + wrap(cj, C.lat.ny)) * C.lat.nx + wrap(ci, C.lat.nx);
+          list.push_back({c, (int)b, ci, cj, ck});
+        }
+  }
+  std::stable_sort(list.begin(), list.end(), [](const Contrib& a, const Contrib& b) { return a.cell < b.cell; });
+  std::vector<int> cell, start;
+  std::vector<int4> contrib;
+  for (size_t j = 0; j < list.size(); ++j) {
+    if (j == 0 || list[j].cell != list[j - 1].cell) {
+      cell.push_back(list[j].cell);
+      start.push_back((int)j);
+    }
+    contrib.push_back(make_int4(list[j].box, list[j].ci, list[j].cj, list[j].ck));
+  }
+  start.push_back((int)list.size());
+  F.sum_ncell = (int)cell.size();
+  hipError_t e = upload(&F.d_desc, F.desc, ctx->stream);
+  if (e == hipSuccess) e = upload(&F.d_fill, fill, ctx->stream);
+  if (e == hipSuccess) e = upload(&F.d_sum_cell, cell, ctx->stream);
+  if (e == hipSuccess) e = upload(&F.d_sum_start, start, ctx->stream);
+  if (e == hipSuccess) e = upload(&F.d_sum_contrib, contrib, ctx->stream);
+  if (e == hipSuccess && !C.d_mask) {
+    const size_t ncell = (size_t)C.lat.nx * C.lat.ny * C.lat.nz;
+    e = hipMalloc(&C.d_mask, ncell);
+    if (e == hipSuccess) e = hipMemsetAsync(C.d_mask, 0, ncell, ctx->stream);
+  }
+  if (e != hipSuccess) {
+    free_level(F);
+    return hip_fail(ctx, e, "level-1 tables");
+  }
+  F.defined = true;
   return LBX_OK;
 }
 
@@ -299,52 +571,19 @@ int lbx_sim_destroy(lbx_sim* sim) {
 int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) {
   if (int rc = check_sim(sim)) return rc;
   lbx_ctx* ctx = sim->ctx;
-  if (level != 0)
-    return fail(ctx, LBX_E_UNSUPPORTED, "only level 0 is supported by this build");
+  if (level < 0 || level > 1) return fail(ctx, LBX_E_UNSUPPORTED, "levels 0 and 1 are supported");
   if (n_boxes < 1 || !lohi) return fail(ctx, LBX_E_ARG, "need at least one box");
-  std::vector<Box> boxes(n_boxes);
-  for (int b = 0; b < n_boxes; ++b) {
-    for (int d = 0; d < 3; ++d) {
-      boxes[b].lo[d] = lohi[6 * b + d];
-      boxes[b].hi[d] = lohi[6 * b + 3 + d];
-    }
-    if (!box_ok(boxes[b], sim->dims))
-      return fail(ctx, LBX_E_ARG, "box %d lies outside the domain or is empty", b);
-  }
-  if (!tiles_domain(boxes, sim->dims))
-    return fail(ctx, LBX_E_ARG, "level-0 boxes must tile the periodic domain exactly");
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  Level& L = sim->lev[level];
-  free_level(L);
-  L.boxes = boxes;
-  L.lat.nx = sim->dims[0];
-  L.lat.ny = sim->dims[1];
-  L.lat.nz = sim->dims[2];
-  const long long n = (long long)L.lat.nx * L.lat.ny * L.lat.nz;
-  L.lat.qs = (n + 63) / 64 * 64;
-  const size_t bytes = (size_t)L.lat.qs * lbx::NV_TOTAL * elem_size(sim->precision);
-  for (void*& p : L.buf) {
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-      free_level(L);
-      return fail(ctx, LBX_E_NOMEM, "hipMalloc of %zu bytes for level %d failed: %s", bytes, level,
-                  hipGetErrorString(e));
-    }
-    e = hipMemsetAsync(p, 0, bytes, ctx->stream);
-    if (e != hipSuccess) {
-      free_level(L);
-      return hip_fail(ctx, e, "hipMemsetAsync");
-    }
-  }
-  L.defined = true;
-  return LBX_OK;
+  const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
+  return level == 0 ? define_level0(sim, boxes) : define_level1(sim, boxes);
 }
 
 int lbx_level_clear(lbx_sim* sim, int level) {
   if (int rc = check_sim(sim)) return rc;
   if (level < 0 || level > 1) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
   (void)hipStreamSynchronize(sim->ctx->stream);
+  if (level == 0) free_level(sim->lev[1]);
   free_level(sim->lev[level]);
   return LBX_OK;
 }
@@ -358,12 +597,49 @@ int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b) {
   return LBX_OK;
 }
 
+int lbx_set_fine_mask(lbx_sim* sim, int n_boxes, const int32_t* lohi_fine) {
+  if (int rc = check_sim(sim)) return rc;
+  lbx_ctx* ctx = sim->ctx;
+  Level& C = sim->lev[0];
+  if (!C.defined) return fail(ctx, LBX_E_STATE, "level 0 is not defined");
+  if (n_boxes < 0 || (n_boxes > 0 && !lohi_fine)) return fail(ctx, LBX_E_ARG, "bad mask boxes");
+  const size_t ncell = (size_t)C.lat.nx * C.lat.ny * C.lat.nz;
+  std::vector<unsigned char> mask(ncell, 0);
+  const std::vector<Box> boxes = read_boxes(n_boxes, lohi_fine);
+  for (const Box& b : boxes)
+    for (int z = std::max(0, floor_div2(b.lo[2])); z <= std::min(C.lat.nz - 1, floor_div2(b.hi[2])); ++z)
+      for (int y = std::max(0, floor_div2(b.lo[1])); y <= std::min(C.lat.ny - 1, floor_div2(b.hi[1])); ++y)
+        for (int x = std::max(0, floor_div2(b.lo[0])); x <= std::min(C.lat.nx - 1, floor_div2(b.hi[0])); ++x)
+          mask[((size_t)z * C.lat.ny + y) * C.lat.nx + x] = 1;
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  if (!C.d_mask) LBX_HIP(ctx, hipMalloc(&C.d_mask, ncell));
+  LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, mask.data(), ncell, hipMemcpyHostToDevice, ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LBX_OK;
+}
+
+int lbx_fill_from_coarse(lbx_sim* sim, int level) {
+  if (int rc = check_sim(sim)) return rc;
+  lbx_ctx* ctx = sim->ctx;
+  if (level != 1) return fail(ctx, LBX_E_ARG, "only level 1 is filled from a coarser level");
+  Level& C = sim->lev[0];
+  Level& F = sim->lev[1];
+  if (!C.defined || !F.defined) return fail(ctx, LBX_E_STATE, "levels 0 and 1 must be defined");
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  LBX_HIP(ctx, lbx::launch_fine_from_coarse((double*)F.buf[F.cur], (const double*)C.buf[C.cur], C.lat,
+                                            F.d_desc, (int)F.boxes.size(), F.max_cells, ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LBX_OK;
+}
+
 int lbx_init_equilibrium(lbx_sim* sim, int level, int box, const double* rho, const double* u) {
   if (int rc = check_sim(sim)) return rc;
   if (int rc = check_box(sim, level, box)) return rc;
   if (!rho || !u) return fail(sim->ctx, LBX_E_ARG, "null field buffer");
   lbx_ctx* ctx = sim->ctx;
   Level& L = sim->lev[level];
+  if (L.patched)
+    return fail(ctx, LBX_E_UNSUPPORTED, "level %d is filled from the coarser level", level);
   const Region R = region_of(L.boxes[box]);
   const size_t n = (size_t)L.boxes[box].cells();
   DeviceTemp t;
@@ -389,8 +665,12 @@ int lbx_upload_f(lbx_sim* sim, int level, int box, const double* f) {
   LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
   LBX_HIP(ctx, hipMemcpyAsync(t.p, f, n * lbx::NV_TOTAL * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
-  LBX_HIP(ctx, lbx::launch_unpack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
-                                    (const double*)t.p, sim->rho0, ctx->stream));
+  if (L.patched)
+    LBX_HIP(ctx, lbx::launch_patch_io(1, (double*)L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
+                                      nullptr, ctx->stream));
+  else
+    LBX_HIP(ctx, lbx::launch_unpack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
+                                      (const double*)t.p, sim->rho0, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }
@@ -404,8 +684,12 @@ int lbx_download_f(lbx_sim* sim, int level, int box, double* f) {
   const size_t n = (size_t)L.boxes[box].cells();
   DeviceTemp t;
   LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
-  LBX_HIP(ctx, lbx::launch_pack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
-                                  (double*)t.p, sim->rho0, ctx->stream));
+  if (L.patched)
+    LBX_HIP(ctx, lbx::launch_patch_io(0, (double*)L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
+                                      nullptr, ctx->stream));
+  else
+    LBX_HIP(ctx, lbx::launch_pack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
+                                    (double*)t.p, sim->rho0, ctx->stream));
   LBX_HIP(ctx, hipMemcpyAsync(f, t.p, n * lbx::NV_TOTAL * sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -419,10 +703,14 @@ int lbx_iterate(lbx_sim* sim, int nsteps) {
   Level& L = sim->lev[0];
   if (!L.defined) return fail(ctx, LBX_E_STATE, "level 0 is not defined");
   if (!L.relax_set) return fail(ctx, LBX_E_STATE, "relaxation rates of level 0 are not set");
-  if (sim->lev[1].defined)
-    return fail(ctx, LBX_E_UNSUPPORTED, "two-level iteration is not available in this build");
   if (nsteps == 0) return LBX_OK;
   LBX_HIP(ctx, hipSetDevice(ctx->device));
+  if (sim->lev[1].defined) {
+    if (!sim->lev[1].relax_set) return fail(ctx, LBX_E_STATE, "relaxation rates of level 1 are not set");
+    for (int s = 0; s < nsteps; ++s)
+      if (int rc = launch_two_level_step(sim)) return rc;
+    return LBX_OK;
+  }
   // Reference order per step is f <- S(C(f)).  Over n steps:
   //   f_n = S (C S)^(n-1) C f_0
   // so: one collide-only pass, n-1 fused pull+collide passes, one
@@ -446,8 +734,12 @@ int lbx_hydro(lbx_sim* sim, int level, int box, double* rho, double* u) {
   LBX_HIP(ctx, hipMalloc(&t.p, n * 4 * sizeof(double)));
   double* drho = (double*)t.p;
   double* du = drho + n;
-  LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]), drho,
-                                 du, sim->rho0, ctx->stream));
+  if (L.patched)
+    LBX_HIP(ctx, lbx::launch_patch_io(2, (double*)L.buf[L.cur], L.d_desc, box, (int)n, drho, du,
+                                      ctx->stream));
+  else
+    LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]), drho,
+                                   du, sim->rho0, ctx->stream));
   LBX_HIP(ctx, hipMemcpyAsync(rho, drho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LBX_HIP(ctx, hipMemcpyAsync(u, du, 3 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -460,13 +752,33 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]) {
   if (level < 0 || level > 1 || !sim->lev[level].defined)
     return fail(sim->ctx, LBX_E_STATE, "level %d is not defined", level);
   lbx_ctx* ctx = sim->ctx;
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
   Level& L = sim->lev[level];
   double* partials = ctx->scratch;
   double* res = ctx->scratch + kPartials * 4;
-  LBX_HIP(ctx, lbx::launch_conservation(sim->precision, L.buf[L.cur], L.lat, sim->rho0, partials,
-                                        kPartials, res, ctx->stream));
-  LBX_HIP(ctx, hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (!L.patched) {
+    LBX_HIP(ctx, lbx::launch_conservation(sim->precision, L.buf[L.cur], L.lat, sim->rho0, partials,
+                                          kPartials, res, ctx->stream));
+    LBX_HIP(ctx, hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LBX_OK;
+  }
+  // Fine level: each box's valid cells packed, summed, then boxes in order.
+  for (int c = 0; c < 4; ++c) out[c] = 0.0;
+  for (size_t b = 0; b < L.boxes.size(); ++b) {
+    const long long n = L.boxes[b].cells();
+    DeviceTemp t;
+    LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
+    LBX_HIP(ctx, lbx::launch_patch_io(0, (double*)L.buf[L.cur], L.d_desc, (int)b, (int)n, (double*)t.p,
+                                      nullptr, ctx->stream));
+    Lattice packed{(int)n, 1, 1, n};
+    LBX_HIP(ctx, lbx::launch_conservation(LBX_FP64, t.p, packed, 0.0, partials, kPartials, res,
+                                          ctx->stream));
+    double part[4];
+    LBX_HIP(ctx, hipMemcpyAsync(part, res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < 4; ++c) out[c] += part[c];
+  }
   return LBX_OK;
 }
 
@@ -484,6 +796,36 @@ int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches) {
   if (int rc = drain_profile(ctx)) return rc;
   *ms = ctx->prof_ms;
   *launches = ctx->prof_launches;
+  return LBX_OK;
+}
+
+int lbx_bench_copy(lbx_ctx* ctx, size_t bytes, int reps, double* gbs) {
+  if (!ctx || !gbs || reps < 1 || bytes < 8) return fail(ctx, LBX_E_ARG, "bad copy benchmark arguments");
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  const long long n = (long long)(bytes / 8);
+  DeviceTemp a, b;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LBX_HIP(ctx, hipMalloc(&a.p, n * 8));
+  LBX_HIP(ctx, hipMalloc(&b.p, n * 8));
+  LBX_HIP(ctx, hipMemsetAsync(a.p, 0, n * 8, ctx->stream));
+  LBX_HIP(ctx, lbx::launch_copy((const double*)a.p, (double*)b.p, n, ctx->stream));  // warm-up
+  LBX_HIP(ctx, hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_fail(ctx, e, "hipEventCreate");
+  }
+  e = hipEventRecord(e0, ctx->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    e = lbx::launch_copy((const double*)a.p, (double*)b.p, n, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(ctx, e, "copy benchmark");
+  *gbs = 2.0 * (double)n * 8.0 * reps / (ms * 1e-3) / 1e9;
   return LBX_OK;
 }
 

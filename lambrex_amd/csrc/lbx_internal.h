@@ -62,4 +62,39 @@ hipError_t launch_hydro(int precision, const void* f, const Lattice& L, const Re
 hipError_t launch_conservation(int precision, const void* f, const Lattice& L, double rho0,
                                double* partials, int nblocks, double* out, hipStream_t st);
 
+// A box stored on its own with 2 ghost layers (fine level), or a box of a
+// level array (coarse push: only lo/hi/cells used).  Population q of grown
+// local cell i is at base + q*qs + i.
+struct PatchDesc {
+  int lo[3], hi[3];  // valid box, level index space (inclusive)
+  int g[3];          // grown extents (n + 4)
+  int cells;         // grown cells (or valid cells for level-array boxes)
+  long long base;    // element offset of the patch in the level buffer
+  long long qs;      // population stride (cells rounded up to 64)
+  long long fill_base;  // offset of this patch's rows in the fill table
+};
+
+// Two-level kernels (lbx_twolevel.hip), FP64.
+hipError_t launch_fine_fill_collide(const double* fnow, double* fnext, const double* cnow,
+                                    const Lattice& clat, const PatchDesc* patches, int npatch,
+                                    int max_cells, const long long* fill, const Relax& r,
+                                    hipStream_t st);
+hipError_t launch_fine_stream(int mode, const double* in, double* out, const PatchDesc* patches,
+                              int npatch, int max_cells, const Relax& r, hipStream_t st);
+hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L,
+                              const PatchDesc* boxes, int nbox, int max_cells,
+                              const unsigned char* mask, const Relax& r, hipStream_t st);
+hipError_t launch_sum_fine_to_coarse(const double* fnow, double* cnext, const Lattice& L,
+                                     const PatchDesc* patches, const int* cell, const int* start,
+                                     const int4* contrib, int ncell, hipStream_t st);
+hipError_t launch_fine_from_coarse(double* fnow, const double* cnow, const Lattice& clat,
+                                   const PatchDesc* patches, int npatch, int max_cells,
+                                   hipStream_t st);
+// op 0: pack valid cells of patch b into a[15][n]; 1: unpack; 2: hydro into a (rho), u.
+hipError_t launch_patch_io(int op, double* f, const PatchDesc* patches, int b, int valid_cells,
+                           double* a, double* u, hipStream_t st);
+
+// Streaming copy of n doubles (bandwidth ceiling / counter calibration).
+hipError_t launch_copy(const double* a, double* b, long long n, hipStream_t st);
+
 }  // namespace lbx

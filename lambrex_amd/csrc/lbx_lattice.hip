@@ -256,6 +256,16 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
 
 int blocks_for(long long n) { return int((n + 255) / 256); }
 
+// Plain streaming copy (one read + one write per element): the achievable
+// read+write bandwidth ceiling, and the calibration kernel for counter-based
+// traffic measurements.
+__global__ __launch_bounds__(256) void k_copy(const double* __restrict__ a, double* __restrict__ b,
+                                              long long n) {
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    __builtin_nontemporal_store(a[i], b + i);
+}
+
 }  // namespace
 
 LaunchShape default_shape(const Lattice& L) {
@@ -336,6 +346,11 @@ hipError_t launch_conservation(int precision, const void* f, const Lattice& L, d
   else
     k_conservation<float><<<nblocks, 256, 0, st>>>((const float*)f, L, rho0, partials);
   k_reduce_partials<<<1, 256, 0, st>>>(partials, nblocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const double* a, double* b, long long n, hipStream_t st) {
+  k_copy<<<8192, 256, 0, st>>>(a, b, n);
   return hipGetLastError();
 }
 

@@ -36,6 +36,8 @@ SIGNATURES = {
     "lbx_level_define": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
     "lbx_level_clear": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_set_relax": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "lbx_set_fine_mask": (ctypes.c_int, [_vp, ctypes.c_int, _i32p]),
+    "lbx_fill_from_coarse": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_init_equilibrium": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "lbx_upload_f": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "lbx_download_f": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
@@ -44,6 +46,7 @@ SIGNATURES = {
     "lbx_conservation": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "lbx_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
+    "lbx_bench_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
 }
 
 _LIB = None
@@ -111,6 +114,12 @@ class Context:
         self.check(self._lib.lbx_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def bench_copy(self, nbytes, reps=10):
+        """Measured read+write copy bandwidth in GB/s."""
+        g = ctypes.c_double()
+        self.check(self._lib.lbx_bench_copy(self.h, int(nbytes), int(reps), ctypes.byref(g)))
+        return g.value
+
     def close(self):
         if self.h:
             self._lib.lbx_ctx_destroy(self.h)
@@ -151,6 +160,15 @@ class Sim:
 
     def set_relax(self, level, omega_s, omega_b):
         self.ctx.check(self._lib.lbx_set_relax(self.h, level, float(omega_s), float(omega_b)))
+
+    def set_fine_mask(self, boxes):
+        """Coarse cells under coarsen(box, 2) of these (fine index space)
+        boxes are masked in the two-level coarse collide."""
+        lohi = _lohi(boxes) if boxes else np.zeros(6, dtype=np.int32)
+        self.ctx.check(self._lib.lbx_set_fine_mask(self.h, len(boxes), _ptr(lohi, ctypes.c_int32)))
+
+    def fill_from_coarse(self, level=1):
+        self.ctx.check(self._lib.lbx_fill_from_coarse(self.h, level))
 
     def set_tau(self, level, tau_s, tau_b=None):
         """omega = 1/(tau + 1/2) in double, as src/AmrSim.cpp:125-126."""
@@ -207,6 +225,18 @@ class Sim:
 
     def get_f(self, level=0):
         return self.level_array(level, [self.download_f(level, b) for b in range(len(self.boxes[level]))], 15)
+
+    def get_f_bbox(self, level):
+        """Populations over the bounding box of the level's boxes (cells
+        outside every box are zero): [15][nz][ny][nx] and the box lo."""
+        bl = self.boxes[level]
+        lo = [min(b[0][d] for b in bl) for d in range(3)]
+        hi = [max(b[1][d] for b in bl) for d in range(3)]
+        out = np.zeros((15, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1))
+        for b, (blo, bhi) in enumerate(bl):
+            out[:, blo[2] - lo[2]:bhi[2] - lo[2] + 1, blo[1] - lo[1]:bhi[1] - lo[1] + 1,
+                blo[0] - lo[0]:bhi[0] - lo[0] + 1] = self.download_f(level, b)
+        return out, lo
 
     def set_f(self, f, level=0):
         for b, (lo, hi) in enumerate(self.boxes[level]):

@@ -242,8 +242,10 @@ def test_error_paths(ctx):
     with pytest.raises(lbx.LbxError) as e:
         sim.iterate(1)  # relaxation not set
     assert e.value.code == lbx.LBX_E_STATE
-    with pytest.raises(lbx.LbxError) as e:
-        sim.hydro(0, 3)
-    assert e.value.code == lbx.LBX_E_ARG
+    rho = np.zeros(512)
+    u = np.zeros(3 * 512)
+    rc = lib.lbx_hydro(sim.h, 0, 3, rho.ctypes.data_as(lbx._dp), u.ctypes.data_as(lbx._dp))
+    assert rc == lbx.LBX_E_ARG  # box 3 does not exist
+    assert b"box 3" in lib.lbx_last_error(ctx.h)
     assert lib.lbx_iterate(sim.h, -1) == lbx.LBX_E_ARG
     sim.close()

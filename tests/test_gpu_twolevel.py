@@ -1,0 +1,162 @@
+"""Two-level Rohde cycle on the GPU against the oracle, bitwise.
+
+The reference pins none of this (its ml_pulse test cannot pass as written:
+MakeFineMask passes the coarse BoxArray, src/AmrSim.cpp:419-428), so the
+oracle's restatement of src/AmrSim.cpp:430-631 is the arbiter.  Cases cover
+the reference's mask (coarse boxes coarsened by 2) and the fine-box mask,
+fine boxes that see each other through ghosts and through periodic images,
+and a fine box that wraps onto itself across the periodic faces.
+"""
+import numpy as np
+import pytest
+
+from lambrex_amd import lbx, workloads
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def c_order(rho_x, u_x):
+    return (np.ascontiguousarray(rho_x.transpose(2, 1, 0)).reshape(-1),
+            np.ascontiguousarray(u_x.transpose(3, 2, 1, 0)).reshape(-1))
+
+
+def fine_boxes_in(lo, hi, mgs):
+    """Fine boxes covering [lo, hi] (fine index space) cut at max_grid_size."""
+    ext = [hi[d] - lo[d] + 1 for d in range(3)]
+    return [(tuple(a + lo[d] for d, a in enumerate(b0)), tuple(a + lo[d] for d, a in enumerate(b1)))
+            for b0, b1 in workloads.chop(ext, mgs)]
+
+
+def run_both(ctx, dims, cmgs, tau, rho, u, fine_lo, fine_hi, fmgs, mask, steps):
+    cboxes = workloads.chop(dims, cmgs)
+    fboxes = fine_boxes_in(fine_lo, fine_hi, fmgs)
+    mboxes = [(tuple(2 * a for a in b0), tuple(2 * a + 1 for a in b1)) for b0, b1 in cboxes] \
+        if mask == "coarse" else fboxes
+    tau_f = 2 * (tau - 0.5) + 0.5  # ComputeDt, src/AmrSim.cpp:313-314
+    # oracle
+    o = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in cboxes])
+    o.init(*c_order(rho, u))
+    o.add_fine([(list(a), list(b)) for a, b in fboxes], [(list(a), list(b)) for a, b in mboxes])
+    o.iterate(steps)
+    fo0, fo1 = o.get_f(0), o.get_f(1)
+    ho1 = o.hydro(1)
+    assert o.tau(1) == tau_f
+    o.close()
+    # GPU
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, tau)
+        sim.init_from_fields(rho, u)
+        sim.define_level(1, fboxes)
+        sim.set_tau(1, tau_f)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(mboxes)
+        sim.iterate(steps)
+        fg0 = sim.get_f(0)
+        fg1, _ = sim.get_f_bbox(1)
+        parts = [sim.hydro(1, b) for b in range(len(fboxes))]
+        cons1 = sim.conservation(1)
+    finally:
+        sim.close()
+    return (fo0, fo1, ho1), (fg0, fg1, parts, cons1), fboxes
+
+
+def assert_bitwise(a, b, what):
+    same = np.ascontiguousarray(a).view(np.uint64) == np.ascontiguousarray(b).view(np.uint64)
+    if not same.all():
+        idx = tuple(np.argwhere(~same)[0])
+        pytest.fail(f"{what}: {int((~same).sum())} of {same.size} differ; first at {idx}: "
+                    f"gpu {b[idx]!r} oracle {a[idx]!r}")
+
+
+@pytest.mark.parametrize("mask", ["coarse", "fine"])
+@pytest.mark.parametrize("steps", [1, 4])
+def test_pulse_two_level_bitwise(ctx, mask, steps):
+    """Coarse 16x8x8 in boxes of 8^3; fine region = coarse x in [4,8) over all
+    y, z (fine 8x16x16 in boxes of 8^3); pulse rho = 1.01 on coarse x = 6."""
+    dims = (16, 8, 8)
+    rho, u = workloads.plane_pulse(dims, 0.01, axis=0, plane=6)
+    (fo0, fo1, _), (fg0, fg1, _, _), _ = run_both(ctx, dims, 8, 0.8, rho, u, (8, 0, 0), (15, 15, 15), 8,
+                                                  mask, steps)
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
+
+
+def test_random_two_level_bitwise_uneven_boxes(ctx):
+    """Random state; coarse 12x10x8 cut at 5; fine region in the middle of the
+    domain, cut at 6 (boxes of different sizes, ghost fill from neighbours,
+    from periodic images and from the coarse level)."""
+    dims = (12, 10, 8)
+    rho, u = workloads.random_state(dims, seed=17, rho_sigma=1e-2, u_amp=0.02)
+    (fo0, fo1, ho1), (fg0, fg1, parts, cons1), fboxes = run_both(
+        ctx, dims, 5, 0.7, rho, u, (4, 0, 2), (15, 19, 11), 6, "fine", 3)
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
+    # hydro of each fine box equals the oracle's
+    lo = [min(b[0][d] for b in fboxes) for d in range(3)]
+    for (blo, bhi), (r, uu) in zip(fboxes, parts):
+        sl = (slice(blo[2] - lo[2], bhi[2] - lo[2] + 1), slice(blo[1] - lo[1], bhi[1] - lo[1] + 1),
+              slice(blo[0] - lo[0], bhi[0] - lo[0] + 1))
+        assert_bitwise(ho1[0][sl], r, "fine rho")
+        assert_bitwise(ho1[1][(slice(None),) + sl], uu, "fine u")
+    assert abs(cons1[0] - ho1[0].sum()) <= 1e-12 * abs(cons1[0])
+
+
+def test_fine_box_wrapping_onto_itself(ctx):
+    """One fine box spanning the whole domain in y and z: its ghosts are its
+    own periodic images and its average-down wraps onto itself."""
+    dims = (8, 4, 4)
+    rho, u = workloads.random_state(dims, seed=23, rho_sigma=1e-2, u_amp=0.02)
+    (fo0, fo1, _), (fg0, fg1, _, _), fboxes = run_both(ctx, dims, 4, 0.9, rho, u, (4, 0, 0), (11, 7, 7), 64,
+                                                       "coarse", 3)
+    assert len(fboxes) == 1
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
+
+
+def test_fine_init_is_piecewise_constant(ctx):
+    """DistFnFillFromCoarse: each fine cell starts with its coarse parent's
+    populations (tests/catch2AMRTests.cpp:162-186 checks uniform fields)."""
+    dims = (8, 6, 4)
+    rho, u = workloads.random_state(dims, seed=2, rho_sigma=1e-2, u_amp=0.02)
+    sim = lbx.Sim(ctx, dims)
+    sim.define_level(0, workloads.chop(dims, 4))
+    sim.set_tau(0, 0.7)
+    sim.init_from_fields(rho, u)
+    fboxes = fine_boxes_in((2, 2, 0), (9, 7, 7), 4)
+    sim.define_level(1, fboxes)
+    sim.fill_from_coarse(1)
+    fc = sim.get_f(0)
+    ff, lo = sim.get_f_bbox(1)
+    sim.close()
+    for z in range(ff.shape[1]):
+        for y in range(ff.shape[2]):
+            for x in range(ff.shape[3]):
+                X, Y, Z = x + lo[0], y + lo[1], z + lo[2]
+                assert (ff[:, z, y, x] == fc[:, Z // 2, Y // 2, X // 2]).all()
+
+
+def test_level1_errors(ctx):
+    sim = lbx.Sim(ctx, (8, 8, 8))
+    with pytest.raises(lbx.LbxError) as e:
+        sim.define_level(1, [((0, 0, 0), (3, 3, 3))])  # no level 0 yet
+    assert e.value.code == lbx.LBX_E_STATE
+    sim.define_level(0, workloads.chop((8, 8, 8), 8))
+    with pytest.raises(lbx.LbxError) as e:
+        sim.define_level(1, [((1, 0, 0), (4, 3, 3))])  # not aligned to ratio 2
+    assert e.value.code == lbx.LBX_E_ARG
+    with pytest.raises(lbx.LbxError) as e:
+        sim.define_level(1, [((0, 0, 0), (17, 3, 3))])  # outside the refined domain
+    assert e.value.code == lbx.LBX_E_ARG
+    with pytest.raises(lbx.LbxError) as e:
+        sim.define_level(1, [((0, 0, 0), (3, 3, 3)), ((2, 0, 0), (5, 3, 3))])  # overlap
+    assert e.value.code == lbx.LBX_E_ARG
+    sim.close()
+    s32 = lbx.Sim(ctx, (8, 8, 8), precision=lbx.LBX_FP32_SHIFTED)
+    s32.define_level(0, workloads.chop((8, 8, 8), 8))
+    with pytest.raises(lbx.LbxError) as e:
+        s32.define_level(1, [((0, 0, 0), (3, 3, 3))])
+    assert e.value.code == lbx.LBX_E_UNSUPPORTED
+    s32.close()
