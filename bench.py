@@ -100,6 +100,55 @@ def cpu_baseline(dims, mgs, tau, rho, u, steps):
     return cells * steps / (t1 - t0) / 1e6, t1 - t0
 
 
+def serving_trace(n_requests, seed, rate_hz):
+    """Requests of the serving workload: cubic edge from {32,48,64,96,128},
+    tau ~ U(0.55, 1.5), plane-pulse amplitude log-uniform in [1e-4, 1e-2] on
+    plane x = N/2, steps ~ U{100..2000}, Poisson arrivals at rate_hz."""
+    rng = np.random.default_rng(seed)
+    edges = rng.choice([32, 48, 64, 96, 128], size=n_requests)
+    taus = rng.uniform(0.55, 1.5, size=n_requests)
+    amps = 10.0 ** rng.uniform(-4.0, -2.0, size=n_requests)
+    steps = rng.integers(100, 2001, size=n_requests)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, size=n_requests))
+    arrivals -= arrivals[0]
+    return [dict(edge=int(e), tau=float(t), amp=float(a), steps=int(s), arrival=float(at))
+            for e, t, a, s, at in zip(edges, taus, amps, steps, arrivals)]
+
+
+def serve(ctx, trace, max_active):
+    """Replay the trace in real time against one server; returns aggregate
+    MLUPS over the makespan and completion latencies (finish - arrival)."""
+    srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=16 << 30, max_active=max_active)
+    results = []
+    submitted = 0
+    arrival_of = {}
+    t0 = time.perf_counter()
+    clock0 = None
+    while len(results) < len(trace):
+        now = time.perf_counter() - t0
+        while submitted < len(trace) and trace[submitted]["arrival"] <= now:
+            r = trace[submitted]
+            rid, _ = srv.submit((r["edge"],) * 3, r["steps"], r["tau"], amplitude=r["amp"], axis=0)
+            arrival_of[rid] = r["arrival"]
+            submitted += 1
+        srv.progress(2)
+        for res in srv.poll():
+            results.append(res)
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    # completion latency against the scheduled arrival (host clock of the server)
+    lat = []
+    for res in results:
+        if clock0 is None:
+            clock0 = res["t_submit"] - arrival_of[res["id"]]
+        lat.append(res["t_done"] - (clock0 + arrival_of[res["id"]]))
+    lu = sum(r["cells"] * r["steps"] for r in results)
+    return {"mlups": lu / makespan / 1e6, "makespan_s": makespan, "lattice_updates": lu,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "steps_launched": st["steps_launched"]}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -123,6 +172,9 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the copy kernel (counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--serve-requests", type=int, default=512,
+                    help="requests of the serving measurement (0 skips it)")
+    ap.add_argument("--serve-only", action="store_true")
     args = ap.parse_args()
 
     if args.copy_only:
@@ -130,6 +182,15 @@ def main():
             gbs = ctx.bench_copy(COPY_BYTES, reps=3)
         print(json.dumps({"copy_bytes_read_per_dispatch": COPY_BYTES,
                           "copy_bytes_written_per_dispatch": COPY_BYTES, "copy_gbs": round(gbs, 1)}))
+        return
+
+    if args.serve_only:
+        trace = serving_trace(args.serve_requests, seed=7, rate_hz=200.0)
+        with lbx.Context(0) as ctx:
+            batched = serve(ctx, trace, max_active=44)
+            seq = serve(ctx, trace, max_active=1)
+        print(json.dumps({"serving": {"requests": len(trace), "batched": batched, "sequential": seq,
+                                      "speedup_aggregate_mlups": round(batched["mlups"] / seq["mlups"], 3)}}))
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -154,6 +215,20 @@ def main():
         r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup)
         log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
             f"({r64['kernel_gbs']:.0f} GB/s)")
+        serving = None
+        if args.serve_requests:
+            trace = serving_trace(args.serve_requests, seed=7, rate_hz=200.0)
+            log(f"[bench] serving: {len(trace)} requests, Poisson 200/s, batched (<= 44 active)")
+            batched = serve(ctx, trace, max_active=44)
+            log(f"[bench] batched: {batched['mlups']:.0f} MLUPS, p50 {batched['p50_ms']:.0f} ms, "
+                f"p99 {batched['p99_ms']:.0f} ms, makespan {batched['makespan_s']:.1f} s")
+            log("[bench] serving: same trace, one run at a time")
+            seq = serve(ctx, trace, max_active=1)
+            log(f"[bench] sequential: {seq['mlups']:.0f} MLUPS, p50 {seq['p50_ms']:.0f} ms, "
+                f"p99 {seq['p99_ms']:.0f} ms, makespan {seq['makespan_s']:.1f} s")
+            serving = {"requests": len(trace), "arrival": "Poisson 200/s (seed 7)", "precision": "FP32-shifted",
+                       "batched": batched, "sequential": seq,
+                       "speedup_aggregate_mlups": round(batched["mlups"] / seq["mlups"], 3)}
         r32 = None
         if not args.no_fp32:
             log("[bench] FP32-shifted, same workload")
@@ -207,6 +282,8 @@ def main():
                          "kernel": "k_step_f32s<STEP_FUSED>",
                          "kernel_avg_ms": round(r32["kernel_avg_ms"], 5)},
         }
+    if serving:
+        out["serving"] = serving
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle, 1 thread, {n}^3 as {r64['boxes']} boxes, {args.cpu_steps} step(s)")
         mlups, secs = cpu_baseline(dims, args.max_grid_size, tau, rho, u, args.cpu_steps)

@@ -142,6 +142,61 @@ int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches);
  * calibration kernel for counter-based traffic figures. */
 int lbx_bench_copy(lbx_ctx* ctx, size_t bytes, int reps, double* gbs);
 
+/* ---------------------------------------------------------------------
+ * Batched serving of independent runs.  No reference counterpart: the
+ * reference runs one AmrSim at a time.  Each request is a periodic
+ * single-level run (x extent <= 256, at most 128^3 cells) with its own tau,
+ * initial state and step count.  Requests are admitted in arrival order
+ * while the server's device arena has room; all admitted runs advance
+ * together, one persistent-kernel launch per step over the patch list of
+ * every active run.  Each run computes exactly what lbx_iterate computes for
+ * it alone, bit for bit. */
+
+#define LBX_INIT_PLANE_PULSE 0  /* rho = 1 + amplitude on plane `plane` normal to `axis`, u = 0 */
+#define LBX_INIT_FIELDS 1       /* rho [nz][ny][nx] and u [3][nz][ny][nx] given (copied at submit) */
+
+typedef struct lbx_server lbx_server;
+
+typedef struct {
+  int32_t dims[3];
+  int32_t steps;           /* >= 1 */
+  double tau_s, tau_b;     /* omega = 1/(tau + 1/2) */
+  int32_t init_kind;
+  int32_t axis, plane;     /* LBX_INIT_PLANE_PULSE */
+  double amplitude;
+  const double* rho;       /* LBX_INIT_FIELDS */
+  const double* u;
+  double* out_rho;         /* optional, filled at completion: [nz][ny][nx] */
+  double* out_u;           /* optional: [3][nz][ny][nx] */
+  double* out_f;           /* optional: FP64 populations [15][nz][ny][nx] */
+} lbx_request;
+
+typedef struct {
+  uint64_t id;
+  int32_t status;
+  int32_t steps;
+  int64_t cells;
+  double mass;             /* FP64 sum of rho at completion */
+  double momentum[3];
+  double t_submit, t_admit, t_done;  /* host steady-clock seconds */
+} lbx_result;
+
+/* arena_bytes = 0 takes half of the free device memory; max_active <= 44. */
+int lbx_serve_create(lbx_ctx* ctx, int precision, double rho0, size_t arena_bytes, int max_active,
+                     lbx_server** out);
+int lbx_serve_destroy(lbx_server* srv);
+const char* lbx_serve_last_error(const lbx_server* srv);
+int lbx_serve_submit(lbx_server* srv, const lbx_request* req, uint64_t* id);
+/* Retire finished runs, admit queued ones, launch up to max_steps batched
+ * steps (at most 4 ahead of the GPU). */
+int lbx_serve_progress(lbx_server* srv, int max_steps, int* active, int* queued);
+/* Progress until every submitted request has completed. */
+int lbx_serve_drain(lbx_server* srv);
+/* Move up to max completed results into out; *n = number returned. */
+int lbx_serve_poll(lbx_server* srv, lbx_result* out, int max, int* n);
+int lbx_serve_stats(lbx_server* srv, int64_t* lattice_updates, int64_t* steps_launched,
+                    size_t* arena_free_largest);
+
 #ifdef __cplusplus
 }
 #endif

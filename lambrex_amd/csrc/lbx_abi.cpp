@@ -480,6 +480,8 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
 
 }  // namespace
 
+int lbx::ctx_device(const lbx_ctx* ctx) { return ctx->device; }
+
 extern "C" {
 
 const char* lbx_last_error(const lbx_ctx* ctx) {

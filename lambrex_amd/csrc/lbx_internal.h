@@ -6,9 +6,14 @@
 
 #include <cstdint>
 
+struct lbx_ctx;
+
 namespace lbx {
 
-constexpr int NV_TOTAL = 15;  // populations per cell (D3Q15)
+constexpr int NV_TOTAL = 15;
+
+// Device ordinal of a context (lbx_abi.cpp).
+int ctx_device(const lbx_ctx* ctx);  // populations per cell (D3Q15)
 
 // A periodic lattice: one level stored as a single SoA array over the whole
 // domain, population-major, x fastest.  Population q of cell (x,y,z) lives at
@@ -93,6 +98,29 @@ hipError_t launch_fine_from_coarse(double* fnow, const double* cnow, const Latti
 // op 0: pack valid cells of patch b into a[15][n]; 1: unpack; 2: hydro into a (rho), u.
 hipError_t launch_patch_io(int op, double* f, const PatchDesc* patches, int b, int valid_cells,
                            double* a, double* u, hipStream_t st);
+
+// Serving: one active run of the batch, passed by value in the kernel
+// arguments (kServeMaxRuns * sizeof(ServeRun) stays under 4 KB).
+struct ServeRun {
+  void* buf[2];      // state buffers (pass p reads buf[p & 1])
+  int n[3];          // lattice extents (n[0] <= 256)
+  int rows;          // x-rows per work item (256 / n[0])
+  int item_begin;    // first work item of this run in the batch
+  int steps;         // LB steps of the run (passes 0..steps)
+  long long start;   // global step of pass 0
+  long long qs;      // population stride
+  double omega_s, omega_b, rho0;
+};
+constexpr int kServeMaxRuns = 44;
+struct ServeBatch {
+  ServeRun run[kServeMaxRuns];
+  int nruns;
+  int total_items;
+};
+hipError_t launch_serve_step(int precision, const ServeBatch& b, long long step, int grid,
+                             hipStream_t st);
+hipError_t launch_serve_init_pulse(int precision, void* f, const int* n, long long qs, int axis,
+                                   int plane, double amplitude, double rho0, hipStream_t st);
 
 // Streaming copy of n doubles (bandwidth ceiling / counter calibration).
 hipError_t launch_copy(const double* a, double* b, long long n, hipStream_t st);

@@ -2,6 +2,7 @@
 // the C ABI; all distribution-function work happens on the GPU.
 #include "AmrSim.h"
 
+#include <algorithm>
 #include <cmath>
 #include <limits>
 
@@ -169,6 +170,7 @@ void AmrSim::ClearLevel(int level) {
   Check(lbx_level_clear(sim_, level));
   LevelHost& L = levels_.at(level);
   L = LevelHost{};
+  if (level <= finest_level_) finest_level_ = level > 0 ? level - 1 : 0;
 }
 
 // src/AmrSim.cpp:845-936 (the arithmetic runs on the GPU in the same order).
@@ -200,40 +202,102 @@ double AmrSim::GetVelocity(int i, int j, int k, int n, int level) const {
   return L.vel[b][n * L.boxes[b].numPts() + LocalIndex(L.boxes[b], i, j, k)];
 }
 
-// src/AmrSim.cpp:981-993 with IterateLevel (:324-333) per step.
+// src/AmrSim.cpp:981-993 with IterateLevel (:324-333) per step, or one
+// RohdeCycle(0) per step (:430-469) when a fine level exists.
 void AmrSim::Iterate(int nsteps) {
   if (!sim_) throw lambrex::Error("Iterate before InitFromScratch");
   if (nsteps <= 0) return;
+  Check(lbx_iterate(sim_, nsteps));
+  LevelHost& C = levels_.at(0);
   if (finest_level_ == 0) {
-    Check(lbx_iterate(sim_, nsteps));
-    LevelHost& L = levels_.at(0);
     for (int t = 0; t < nsteps; ++t) {
-      L.time += L.dt;
-      ++L.step;
+      C.time += C.dt;
+      ++C.step;
     }
-    if (nsteps % 2) L.cur = 1 - L.cur;  // UpdateNow swaps Density too
+    if (nsteps % 2) C.cur = 1 - C.cur;  // UpdateNow swaps Density too
   } else {
-    throw lambrex::Error("two-level iteration is not available in this build");
+    // UpdateDistribution (:619-631): the finest level advances two steps.
+    LevelHost& F = levels_.at(1);
+    for (int t = 0; t < nsteps; ++t) {
+      F.time += 2 * F.dt;
+      F.step += 2;
+      C.time += C.dt;
+      ++C.step;
+    }
   }
 }
 
-// src/AmrSim.cpp:995-1017
+// src/AmrSim.cpp:689-713: the fine level from the coarse one.
+void AmrSim::MakeNewLevelFromCoarse(int level, double time, const std::vector<lambrex::IBox>& ba) {
+  if (!level) throw lambrex::Error("Cannot construct level 0 from a coarser level.");
+  if (level != 1) throw lambrex::Error("only one refined level is supported");
+  LevelHost& L = levels_.at(level);
+  L.boxes = ba;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (int s = 0; s < 2; ++s) {
+    L.rho[s].assign(ba.size(), {});
+    for (size_t b = 0; b < ba.size(); ++b) L.rho[s][b].assign(ba[b].numPts(), nan);
+  }
+  L.vel.assign(ba.size(), {});
+  for (size_t b = 0; b < ba.size(); ++b) L.vel[b].assign(3 * ba[b].numPts(), nan);
+  L.cur = 0;
+  const std::vector<int32_t> lohi = lambrex::to_lohi(ba);
+  Check(lbx_level_define(sim_, level, (int)ba.size(), lohi.data()));
+  L.time = time;
+  ComputeDt(level);
+  L.step = 0;
+  Check(lbx_set_relax(sim_, level, 1.0 / (tau_s_.at(level) + 0.5), 1.0 / (tau_b_.at(level) + 0.5)));
+  Check(lbx_fill_from_coarse(sim_, level));  // DistFnFillFromCoarse
+  finest_level_ = level;
+  CalcHydroVars(level);
+  MakeFineMask(level - 1);
+}
+
+// src/AmrSim.cpp:419-428.  The reference builds the mask from the coarse
+// level's own BoxArray (its fba argument), i.e. the coarse cells covered by
+// coarsen(coarse boxes, 2); SetFineMaskFromFineGrids(true) uses the fine
+// BoxArray instead.
+void AmrSim::MakeFineMask(int coarse_level) {
+  if (coarse_level != 0 || finest_level_ < 1) return;
+  const std::vector<lambrex::IBox>& fba =
+      mask_from_fine_grids_ ? levels_.at(1).boxes : levels_.at(0).boxes;
+  const std::vector<int32_t> lohi = lambrex::to_lohi(fba);
+  Check(lbx_set_fine_mask(sim_, (int)fba.size(), lohi.data()));
+}
+
+void AmrSim::SetFineMaskFromFineGrids(bool on) {
+  mask_from_fine_grids_ = on;
+  MakeFineMask(0);
+}
+
+// src/AmrSim.cpp:995-1017.  AMReX's regrid (tagging, error buffer,
+// clustering) is not reproduced: the fine level is exactly the refined tag
+// box, clipped to the domain and cut at max_grid_size.
 void AmrSim::SetStaticRefinement(int level, const std::array<int, NDIMS>& lo_corner,
                                  const std::array<int, NDIMS>& hi_corner) {
-  if (level < 0 || level >= max_level_)
-    throw lambrex::Error("SetStaticRefinement: no finer level allowed below this one");
-  throw lambrex::Error("two-level runs are not available in this build");
-  (void)lo_corner;
-  (void)hi_corner;
+  if (!sim_) throw lambrex::Error("SetStaticRefinement before InitFromScratch");
+  if (level != 0 || max_level_ < 1)
+    throw lambrex::Error("SetStaticRefinement: only level 0 can be refined (max_level >= 1)");
+  lambrex::IBox tag;
+  for (int d = 0; d < NDIMS; ++d) {
+    tag.lo[d] = std::max(lo_corner[d], 0);
+    tag.hi[d] = std::min(hi_corner[d], GetDims()[d] - 1);
+  }
+  if (!tag.ok()) throw lambrex::Error("SetStaticRefinement: empty refinement region");
+  static_tag_ = tag;
+  if (finest_level_ >= 1) ClearLevel(1);
+  finest_level_ = 0;
+  const std::vector<lambrex::IBox> fba =
+      lambrex::max_size_chop(lambrex::refine(tag, kRefRatio), max_grid_size_);
+  MakeNewLevelFromCoarse(1, GetTime(0), fba);
+  MakeFineMask(level);
 }
 
 void AmrSim::UnsetStaticRefinement(int level) {
-  if (level < 0 || level >= max_level_)
-    throw lambrex::Error("UnsetStaticRefinement: level out of range");
-  if (finest_level_ > level) {
-    for (int l = finest_level_; l > level; --l) ClearLevel(l);
-    finest_level_ = level;
-  }
+  if (level != 0 || max_level_ < 1) throw lambrex::Error("UnsetStaticRefinement: level out of range");
+  static_tag_ = lambrex::IBox{};
+  if (finest_level_ >= 1) ClearLevel(1);
+  finest_level_ = 0;
 }
 
 // src/AmrSim.cpp:1019-1030

@@ -87,6 +87,9 @@ class AmrSim {
   void SetMaxGridSize(int max_grid_size);
   // LBX_FP64 (default) or LBX_FP32_SHIFTED with reference density rho0.
   void SetPrecision(int precision, double rho0);
+  // Mask the coarse collide with the fine BoxArray instead of the coarse one
+  // the reference passes to makeFineMask (src/AmrSim.cpp:419-428).
+  void SetFineMaskFromFineGrids(bool on);
   lbx_sim* Device() const { return sim_; }
   // Populations of the current state of one box, [15][nz][ny][nx] FP64.
   std::vector<double> DistFn(int level, int box) const;
@@ -117,6 +120,8 @@ class AmrSim {
 
   void ComputeDt(int level);
   void MakeNewLevelFromScratch(int level, double time, const std::vector<lambrex::IBox>& ba);
+  void MakeNewLevelFromCoarse(int level, double time, const std::vector<lambrex::IBox>& ba);
+  void MakeFineMask(int coarse_level);
   void ClearLevel(int level);
   void InitDensity(int level);
   void InitVelocity(int level);
@@ -131,6 +136,8 @@ class AmrSim {
   int max_grid_size_ = 32;
   int precision_ = LBX_FP64;
   double rho0_ = 1.0;
+  bool mask_from_fine_grids_ = false;
+  lambrex::IBox static_tag_;
   std::vector<double> initial_density;
   std::vector<double> initial_velocity;
   std::vector<double> tau_s_, tau_b_, mass_;

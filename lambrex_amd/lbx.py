@@ -49,6 +49,38 @@ SIGNATURES = {
     "lbx_bench_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
 }
 
+# Serving structs (include/lbx.h).
+class Request(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32 * 3), ("steps", ctypes.c_int32),
+                ("tau_s", ctypes.c_double), ("tau_b", ctypes.c_double),
+                ("init_kind", ctypes.c_int32), ("axis", ctypes.c_int32), ("plane", ctypes.c_int32),
+                ("amplitude", ctypes.c_double),
+                ("rho", _dp), ("u", _dp), ("out_rho", _dp), ("out_u", _dp), ("out_f", _dp)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("cells", ctypes.c_int64), ("mass", ctypes.c_double), ("momentum", ctypes.c_double * 3),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+LBX_INIT_PLANE_PULSE = 0
+LBX_INIT_FIELDS = 1
+
+SIGNATURES.update({
+    "lbx_serve_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_int,
+                                        ctypes.POINTER(_vp)]),
+    "lbx_serve_destroy": (ctypes.c_int, [_vp]),
+    "lbx_serve_last_error": (ctypes.c_char_p, [_vp]),
+    "lbx_serve_submit": (ctypes.c_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    "lbx_serve_progress": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "lbx_serve_drain": (ctypes.c_int, [_vp]),
+    "lbx_serve_poll": (ctypes.c_int, [_vp, ctypes.POINTER(Result), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "lbx_serve_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+})
+
 _LIB = None
 
 
@@ -252,6 +284,93 @@ class Sim:
         for b, (lo, hi) in enumerate(self.boxes[level]):
             sl = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1), slice(lo[0], hi[0] + 1))
             self.init_equilibrium(level, b, rho[sl], u[(slice(None),) + sl])
+
+
+class Server:
+    """Batched serving of independent periodic runs (lbx_serve_*)."""
+
+    def __init__(self, ctx, precision=LBX_FP32_SHIFTED, rho0=1.0, arena_bytes=0, max_active=0):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        h = ctypes.c_void_p()
+        rc = self._lib.lbx_serve_create(ctx.h, int(precision), float(rho0), int(arena_bytes),
+                                        int(max_active), ctypes.byref(h))
+        if rc != LBX_OK:
+            raise LbxError(rc, "lbx_serve_create failed")
+        self.h = h
+        self._keep = {}  # request id -> buffers that must outlive the run
+
+    def _check(self, rc):
+        if rc != LBX_OK:
+            raise LbxError(rc, self._lib.lbx_serve_last_error(self.h).decode())
+
+    def submit(self, dims, steps, tau_s, tau_b=None, amplitude=0.0, axis=0, plane=None,
+               rho=None, u=None, want_rho=False, want_f=False):
+        """Queue one run; returns (id, outputs) where outputs holds the numpy
+        arrays the server fills at completion."""
+        nx, ny, nz = (int(d) for d in dims)
+        r = Request()
+        r.dims[:] = (nx, ny, nz)
+        r.steps = int(steps)
+        r.tau_s = float(tau_s)
+        r.tau_b = float(tau_s if tau_b is None else tau_b)
+        keep = {}
+        if rho is not None:
+            keep["rho_in"] = np.ascontiguousarray(rho, dtype=np.float64).reshape(nz, ny, nx)
+            keep["u_in"] = np.ascontiguousarray(u, dtype=np.float64).reshape(3, nz, ny, nx)
+            r.init_kind = LBX_INIT_FIELDS
+            r.rho = _ptr(keep["rho_in"], ctypes.c_double)
+            r.u = _ptr(keep["u_in"], ctypes.c_double)
+        else:
+            r.init_kind = LBX_INIT_PLANE_PULSE
+            r.axis = int(axis)
+            r.plane = int(dims[axis] // 2 if plane is None else plane)
+            r.amplitude = float(amplitude)
+        outs = {}
+        if want_rho:
+            outs["rho"] = np.empty((nz, ny, nx))
+            outs["u"] = np.empty((3, nz, ny, nx))
+            r.out_rho = _ptr(outs["rho"], ctypes.c_double)
+            r.out_u = _ptr(outs["u"], ctypes.c_double)
+        if want_f:
+            outs["f"] = np.empty((15, nz, ny, nx))
+            r.out_f = _ptr(outs["f"], ctypes.c_double)
+        rid = ctypes.c_uint64()
+        self._check(self._lib.lbx_serve_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
+        keep.update(outs)
+        self._keep[rid.value] = keep
+        return rid.value, outs
+
+    def progress(self, max_steps=8):
+        a, q = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.lbx_serve_progress(self.h, int(max_steps), ctypes.byref(a), ctypes.byref(q)))
+        return a.value, q.value
+
+    def drain(self):
+        self._check(self._lib.lbx_serve_drain(self.h))
+
+    def poll(self, max_results=1024):
+        buf = (Result * max_results)()
+        n = ctypes.c_int()
+        self._check(self._lib.lbx_serve_poll(self.h, buf, max_results, ctypes.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = buf[i]
+            out.append({"id": r.id, "status": r.status, "steps": r.steps, "cells": r.cells, "mass": r.mass,
+                        "momentum": tuple(r.momentum), "t_submit": r.t_submit, "t_admit": r.t_admit,
+                        "t_done": r.t_done})
+            self._keep.pop(r.id, None)
+        return out
+
+    def stats(self):
+        lu, st, fr = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_size_t()
+        self._check(self._lib.lbx_serve_stats(self.h, ctypes.byref(lu), ctypes.byref(st), ctypes.byref(fr)))
+        return {"lattice_updates": lu.value, "steps_launched": st.value, "arena_largest_free": fr.value}
+
+    def close(self):
+        if self.h:
+            self._lib.lbx_serve_destroy(self.h)
+            self.h = None
 
 
 def c_order_to_xfast(rho_c, u_c, dims):

@@ -1,0 +1,111 @@
+// amr_tests.cpp -- the reference's two-level AMR tests
+// (tests/catch2AMRTests.cpp:118-423) against this build's AmrSim facade.
+// Sections that inspect AMReX TagBoxArrays (ErrorEst) have no counterpart
+// here: tagging is reduced to the static refinement box.
+#include "check.h"
+#include "lambrex.h"
+
+namespace {
+const int NX = 48, NY = 24, NZ = 12, MAX_LEVEL = 1;
+const double DENSITY = 0.8, VELOCITY = 0.1, TAU = 0.01;
+
+AmrSim* make() {
+  AmrSim* sim = new AmrSim(NX, NY, NZ, MAX_LEVEL, {1, 1, 1}, TAU, TAU);
+  sim->SetInitialDensity(DENSITY);
+  sim->SetInitialVelocity(VELOCITY);
+  sim->InitFromScratch(0.0);
+  return sim;
+}
+}  // namespace
+
+TEST_CASE("TwoLevel Initialisation") {
+  std::unique_ptr<AmrSim> sim(make());
+  REQUIRE(sim->refRatio().size() == (size_t)MAX_LEVEL + 1);
+  for (int d = 0; d < NDIMS; ++d) REQUIRE(sim->refRatio(0)[d] == 2);
+  REQUIRE(sim->maxLevel() == MAX_LEVEL);
+  REQUIRE(sim->finestLevel() == 0);
+  REQUIRE_FALSE(sim->boxArray(0).empty());
+  REQUIRE(sim->boxArray(1).empty());
+}
+
+TEST_CASE("TwoLevel FineGrids") {
+  std::unique_ptr<AmrSim> sim(make());
+  const std::array<int, NDIMS> REF_LO{NX / 4, NY / 4, NZ / 4};
+  const std::array<int, NDIMS> REF_HI{3 * NX / 4, 3 * NY / 4, 3 * NZ / 4};
+  sim->SetStaticRefinement(0, REF_LO, REF_HI);
+  REQUIRE(sim->finestLevel() == 1);
+  long long num_pts = 1;
+  for (int d = 0; d < NDIMS; ++d) num_pts *= REF_HI[d] - REF_LO[d];
+  int ratio = 1;
+  for (int level = 0; level <= MAX_LEVEL; ++level) {
+    auto ext = sim->GetExtent(level);
+    lambrex::IBox m;
+    m.lo = ext.first;
+    m.hi = ext.second;
+    REQUIRE(m.contains(ratio * REF_LO[0], ratio * REF_LO[1], ratio * REF_LO[2]));
+    REQUIRE(m.contains(ratio * REF_HI[0], ratio * REF_HI[1], ratio * REF_HI[2]));
+    REQUIRE(m.numPts() >= (long long)ratio * ratio * ratio * num_pts);
+    ratio *= 2;
+  }
+  // ComputeDt (tests/catch2AMRTests.cpp:348-353)
+  REQUIRE(sim->GetDt(1) == 0.5);
+  REQUIRE(sim->GetMass(1) == 0.5);
+  REQUIRE(sim->GetTauS(1) == Approx(2 * (sim->GetTauS(0) - 0.5) + 0.5));
+  REQUIRE(sim->GetTauB(1) == Approx(2 * (sim->GetTauB(0) - 0.5) + 0.5));
+  REQUIRE(sim->GetTime(1) == 0.0);
+  REQUIRE(sim->GetTimeStep(1) == 0);
+}
+
+TEST_CASE("TwoLevel MakeNewLevelFromCoarse keeps uniform fields") {
+  std::unique_ptr<AmrSim> sim(make());
+  sim->SetStaticRefinement(0, {0, 0, 0}, {NX - 1, NY - 1, NZ - 1});
+  for (int level = 0; level <= MAX_LEVEL; ++level) {
+    const int r = level ? 2 : 1;
+    for (int k = 0; k < r * NZ; k += 3)
+      for (int j = 0; j < r * NY; j += 2)
+        for (int i = 0; i < r * NX; ++i) {
+          REQUIRE(sim->GetDensity(i, j, k, level) == Approx(DENSITY));
+          for (int d = 0; d < NDIMS; ++d) REQUIRE(sim->GetVelocity(i, j, k, d, level) == Approx(VELOCITY));
+        }
+  }
+}
+
+TEST_CASE("TwoLevel Iterate advances both levels") {
+  std::unique_ptr<AmrSim> sim(make());
+  sim->SetStaticRefinement(0, {8, 4, 2}, {23, 15, 9});
+  sim->Iterate(3);
+  REQUIRE(sim->GetTime(0) == 3.0);
+  REQUIRE(sim->GetTimeStep(0) == 3);
+  REQUIRE(sim->GetTime(1) == 3.0);  // 2 fine steps of dt 1/2 per coarse step
+  REQUIRE(sim->GetTimeStep(1) == 6);
+  sim->CalcHydroVars(0);
+  sim->CalcHydroVars(1);
+  REQUIRE(sim->OnProcessDensity(sim->GetDensity(16, 8, 4, 1)));
+  REQUIRE(!sim->OnProcessDensity(sim->GetDensity(0, 0, 0, 1)));  // not refined there
+}
+
+TEST_CASE("TwoLevel UnsetStaticRefinement") {
+  std::unique_ptr<AmrSim> sim(make());
+  sim->SetStaticRefinement(0, {8, 4, 2}, {23, 15, 9});
+  REQUIRE(sim->finestLevel() == 1);
+  sim->UnsetStaticRefinement(0);
+  REQUIRE(sim->finestLevel() == 0);
+  REQUIRE(sim->boxArray(1).empty());
+  sim->Iterate(2);
+  REQUIRE(sim->GetTimeStep(0) == 2);
+}
+
+TEST_CASE("Refinement needs max_level >= 1") {
+  AmrSim sim(8, 8, 8, 0, {1, 1, 1}, 0.5, 0.5);
+  sim.SetInitialDensity(1.0);
+  sim.SetInitialVelocity(0.0);
+  sim.InitFromScratch(0.0);
+  REQUIRE_THROWS(sim.SetStaticRefinement(0, {0, 0, 0}, {3, 3, 3}));
+}
+
+int main() {
+  lambrexInit();
+  const int rc = check::run_all();
+  lambrexFinalise();
+  return rc;
+}

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <functional>
 #include <string>
+#include <memory>
 #include <vector>
 
 namespace check {

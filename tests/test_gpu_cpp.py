@@ -29,6 +29,13 @@ def test_init_and_single_level_cases():
     assert "0 failed" in out
 
 
+def test_two_level_amr_cases():
+    rc, out = run("amr_tests")
+    print(out)
+    assert rc == 0, out
+    assert "0 failed" in out
+
+
 def test_amr_pulse_prints_the_reference_goldens():
     """examples/amr_pulse.cpp re-run through the facade: every printed value
     (precision 6, the format the reference's golden header was made with)
