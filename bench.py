@@ -149,6 +149,42 @@ def serve(ctx, trace, max_active):
             "steps_launched": st["steps_launched"]}
 
 
+def two_level(ctx, steps, warmup):
+    """Two-level configuration: coarse 128^3 periodic (boxes of 32^3), fine
+    level over coarse x in [48, 80) and all y, z (fine 64x256x256 in boxes of
+    32^3), tau_coarse 0.8 (fine tau 1.1, src/AmrSim.cpp:313-314), plane pulse
+    rho = 1.01 at coarse x = 64, the reference's fine mask.  A step is one
+    coarse step: 128^3 + 2 x 64x256x256 lattice updates."""
+    dims = (128, 128, 128)
+    cboxes = workloads.chop(dims, 32)
+    ext = (64, 256, 256)
+    fboxes = [((96 + a[0], a[1], a[2]), (96 + b[0], b[1], b[2])) for a, b in workloads.chop(ext, 32)]
+    mask = [((2 * a[0], 2 * a[1], 2 * a[2]), (2 * b[0] + 1, 2 * b[1] + 1, 2 * b[2] + 1)) for a, b in cboxes]
+    rho, u = workloads.plane_pulse(dims, 0.01, axis=0, plane=64)
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, 0.8)
+        sim.init_from_fields(rho, u)
+        sim.define_level(1, fboxes)
+        sim.set_tau(1, 2 * (0.8 - 0.5) + 0.5)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(mask)
+        sim.iterate(warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        sim.iterate(steps)
+        ctx.sync()
+        t1 = time.perf_counter()
+    finally:
+        sim.close()
+    lu = (128 ** 3 + 2 * 64 * 256 * 256) * steps
+    return {"workload": "coarse 128^3 (64 boxes of 32^3) + fine 64x256x256 (128 boxes of 32^3), "
+                        "ratio 2, 2 fine substeps per coarse step, FP64",
+            "coarse_steps": steps, "mlups": lu / (t1 - t0) / 1e6, "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
+            "graph": "one coarse step captured as a HIP graph"}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -171,10 +207,12 @@ def main():
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the copy kernel (counter calibration)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--two-level-steps", type=int, default=100,
+                    help="coarse steps of the two-level measurement (0 skips it)")
     args = ap.parse_args()
 
     if args.copy_only:
@@ -215,6 +253,11 @@ def main():
         r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup)
         log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
             f"({r64['kernel_gbs']:.0f} GB/s)")
+        twolev = None
+        if args.two_level_steps:
+            log(f"[bench] two-level: {args.two_level_steps} coarse steps")
+            twolev = two_level(ctx, args.two_level_steps, 5)
+            log(f"[bench] two-level: {twolev['mlups']:.0f} MLUPS, {twolev['ms_per_coarse_step']:.3f} ms per coarse step")
         serving = None
         if args.serve_requests:
             trace = serving_trace(args.serve_requests, seed=7, rate_hz=200.0)
@@ -282,6 +325,8 @@ def main():
                          "kernel": "k_step_f32s<STEP_FUSED>",
                          "kernel_avg_ms": round(r32["kernel_avg_ms"], 5)},
         }
+    if twolev:
+        out["two_level"] = twolev
     if serving:
         out["serving"] = serving
     if not args.no_cpu:

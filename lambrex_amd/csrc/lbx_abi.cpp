@@ -62,6 +62,19 @@ struct Level {
   int sum_ncell = 0;
 };
 
+// One captured two-level coarse step per (coarse parity, fine parity).
+struct StepGraphs {
+  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  void clear() {
+    for (auto& row : exec)
+      for (auto& e : row)
+        if (e) {
+          (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+  }
+};
+
 constexpr int kPartials = 1024;  // blocks of the conservation reduction
 
 }  // namespace
@@ -85,6 +98,8 @@ struct lbx_sim {
   int precision = LBX_FP64;
   double rho0 = 1.0;
   Level lev[2];
+  StepGraphs graphs;
+  bool use_graphs = true;
 };
 
 namespace {
@@ -258,33 +273,68 @@ int launch_step(lbx_sim* sim, Level& L, lbx::StepKind kind) {
   return ps.close();
 }
 
-// One coarse step of the two-level cycle (src/AmrSim.cpp:430-469).
-int launch_two_level_step(lbx_sim* sim) {
-  lbx_ctx* ctx = sim->ctx;
+// The kernels of one coarse step of the two-level cycle
+// (src/AmrSim.cpp:430-469) for the current buffer parities.
+hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   Level& C = sim->lev[0];
   Level& F = sim->lev[1];
   const Relax rc{C.omega_s, C.omega_b, sim->rho0};
   const Relax rf{F.omega_s, F.omega_b, sim->rho0};
-  ProfileScope ps(ctx, true);
-  if (ps.rc) return ps.rc;
   const double* cnow = (const double*)C.buf[C.cur];
   double* cnext = (double*)C.buf[1 - C.cur];
   double* fnow = (double*)F.buf[F.cur];
   double* fnext = (double*)F.buf[1 - F.cur];
   double* ftmp = (double*)F.buf[2];
   const int np = (int)F.boxes.size();
-  LBX_HIP(ctx, lbx::launch_fine_fill_collide(fnow, fnext, cnow, C.lat, F.d_desc, np, F.max_cells,
-                                             F.d_fill, rf, ctx->stream));
-  LBX_HIP(ctx, lbx::launch_fine_stream(0, fnext, ftmp, F.d_desc, np, F.max_cells, rf, ctx->stream));
-  LBX_HIP(ctx, lbx::launch_fine_stream(1, ftmp, fnext, F.d_desc, np, F.max_cells, rf, ctx->stream));
+  hipError_t e = lbx::launch_fine_fill_collide(fnow, fnext, cnow, C.lat, F.d_desc, np, F.max_cells,
+                                               F.d_fill, rf, st);
+  if (e == hipSuccess) e = lbx::launch_fine_stream(0, fnext, ftmp, F.d_desc, np, F.max_cells, rf, st);
+  if (e == hipSuccess) e = lbx::launch_fine_stream(1, ftmp, fnext, F.d_desc, np, F.max_cells, rf, st);
+  if (e == hipSuccess)
+    e = lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_desc, (int)C.boxes.size(), C.max_cells, C.d_mask,
+                                rc, st);
+  // the fine state after the step is fnext
+  if (e == hipSuccess)
+    e = lbx::launch_sum_fine_to_coarse(fnext, cnext, C.lat, F.d_desc, F.d_sum_cell, F.d_sum_start,
+                                       F.d_sum_contrib, F.sum_ncell, st);
+  return e;
+}
+
+// One coarse step: replayed from a captured graph, or launched eagerly
+// while profiling (event records cannot sit inside the capture).
+int launch_two_level_step(lbx_sim* sim) {
+  lbx_ctx* ctx = sim->ctx;
+  Level& C = sim->lev[0];
+  Level& F = sim->lev[1];
+  if (sim->use_graphs && !ctx->prof) {
+    hipGraphExec_t& ex = sim->graphs.exec[C.cur][F.cur];
+    if (!ex) {
+      hipGraph_t g = nullptr;
+      LBX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t ek = enqueue_two_level_step(sim, ctx->stream);
+      const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+      if (ek != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(ctx, ek, "two-level step capture");
+      }
+      if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture");
+      const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        ex = nullptr;
+        return hip_fail(ctx, ei, "hipGraphInstantiate");
+      }
+    }
+    LBX_HIP(ctx, hipGraphLaunch(ex, ctx->stream));
+  } else {
+    ProfileScope ps(ctx, true);
+    if (ps.rc) return ps.rc;
+    LBX_HIP(ctx, enqueue_two_level_step(sim, ctx->stream));
+    if (int rc = ps.close()) return rc;
+  }
   F.cur = 1 - F.cur;
-  LBX_HIP(ctx, lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_desc, (int)C.boxes.size(), C.max_cells,
-                                       C.d_mask, rc, ctx->stream));
-  LBX_HIP(ctx, lbx::launch_sum_fine_to_coarse((const double*)F.buf[F.cur], cnext, C.lat, F.d_desc,
-                                              F.d_sum_cell, F.d_sum_start, F.d_sum_contrib,
-                                              F.sum_ncell, ctx->stream));
   C.cur = 1 - C.cur;
-  return ps.close();
+  return LBX_OK;
 }
 
 std::vector<Box> read_boxes(int n, const int32_t* lohi) {
@@ -565,6 +615,7 @@ int lbx_sim_destroy(lbx_sim* sim) {
   if (!sim) return LBX_OK;
   (void)hipSetDevice(sim->ctx->device);
   (void)hipStreamSynchronize(sim->ctx->stream);
+  sim->graphs.clear();
   for (Level& L : sim->lev) free_level(L);
   delete sim;
   return LBX_OK;
@@ -578,6 +629,7 @@ int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) 
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
+  sim->graphs.clear();
   return level == 0 ? define_level0(sim, boxes) : define_level1(sim, boxes);
 }
 
@@ -585,6 +637,7 @@ int lbx_level_clear(lbx_sim* sim, int level) {
   if (int rc = check_sim(sim)) return rc;
   if (level < 0 || level > 1) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
   (void)hipStreamSynchronize(sim->ctx->stream);
+  sim->graphs.clear();
   if (level == 0) free_level(sim->lev[1]);
   free_level(sim->lev[level]);
   return LBX_OK;
@@ -596,6 +649,7 @@ int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b) {
   sim->lev[level].omega_s = omega_s;
   sim->lev[level].omega_b = omega_b;
   sim->lev[level].relax_set = true;
+  sim->graphs.clear();
   return LBX_OK;
 }
 
@@ -804,7 +858,7 @@ int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches) {
 int lbx_bench_copy(lbx_ctx* ctx, size_t bytes, int reps, double* gbs) {
   if (!ctx || !gbs || reps < 1 || bytes < 8) return fail(ctx, LBX_E_ARG, "bad copy benchmark arguments");
   LBX_HIP(ctx, hipSetDevice(ctx->device));
-  const long long n = (long long)(bytes / 8);
+  const long long n = (long long)(bytes / 16) * 2;  // whole 16-byte elements
   DeviceTemp a, b;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   LBX_HIP(ctx, hipMalloc(&a.p, n * 8));

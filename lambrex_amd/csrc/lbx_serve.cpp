@@ -8,8 +8,10 @@
 // walks the patch list of every active run.  At most kInFlight steps are
 // queued ahead of the GPU, so admission and completion are observed within a
 // few steps.  A run finishes with its stream-only pass; once the GPU has
-// executed it, its density, velocity, populations and FP64 sums are copied
-// to the caller's buffers and its arena space is released.
+// executed it, its FP64 sums (reduced on the stream right behind its final
+// pass and copied to pinned host memory) and any requested fields are
+// handed to the caller and its arena space is released; nothing in the
+// steady state waits for the GPU except the in-flight bound.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -100,6 +102,8 @@ struct Run {
   double t_admit = 0.0;
   bool launched_last = false;
   long long last_step = 0;
+  int slot = -1;          // per-run result slot (device sums, pinned copy, event)
+  bool finish_queued = false;
 };
 
 }  // namespace
@@ -116,7 +120,10 @@ struct lbx_server {
   void* arena_mem = nullptr;
   double* staging = nullptr;  // device: hydro and packed populations of a finishing run
   size_t staging_bytes = 0;
-  double* sums = nullptr;     // device: conservation partials + result
+  double* sums = nullptr;     // device: conservation partials + per-slot results
+  double* pinned = nullptr;   // host pinned: per-slot results
+  hipEvent_t slot_ev[lbx::kServeMaxRuns] = {};
+  std::vector<int> free_slots;
   std::deque<Request> queue;
   std::vector<Run> active;
   std::vector<lbx_result> done;
@@ -124,7 +131,6 @@ struct lbx_server {
   long long step = 0;  // next global step to launch
   hipEvent_t ev[kInFlight] = {};
   long long ev_step[kInFlight] = {-1, -1, -1, -1};
-  long long known_done = -1;  // every step <= known_done has completed
   std::string err;
   int64_t lattice_updates = 0;
 };
@@ -158,8 +164,10 @@ int admit(lbx_server* s) {
     const long long qs = (cells_of(q.req) + 63) / 64 * 64;
     const size_t bytes = (size_t)2 * lbx::NV_TOTAL * qs * esize(s->precision);
     size_t off = 0;
-    if (!s->arena.alloc(bytes, &off)) break;  // wait for space (FIFO admission)
+    if (s->free_slots.empty() || !s->arena.alloc(bytes, &off)) break;  // FIFO admission
     Run r;
+    r.slot = s->free_slots.back();
+    s->free_slots.pop_back();
     r.r = std::move(q);
     s->queue.pop_front();
     r.off = off;
@@ -192,14 +200,38 @@ int admit(lbx_server* s) {
   return LBX_OK;
 }
 
-// Results of runs whose final pass the GPU has executed.
-int retire(lbx_server* s, long long completed_step) {
+// Enqueue a finished run's FP64 sums right behind its final pass: reduce
+// into the run's slot, copy to pinned host memory, record the slot event.
+int queue_finish(lbx_server* s, Run& r) {
+  const lbx_request& rq = r.r.req;
+  lbx::Lattice L{rq.dims[0], rq.dims[1], rq.dims[2], r.qs};
+  char* fin = s->arena.base + r.off +
+              (size_t)((rq.steps + 1) & 1) * lbx::NV_TOTAL * r.qs * esize(s->precision);
+  double* out = s->sums + 1024 + 4 * r.slot;
+  SRV_HIP(s, lbx::launch_conservation(s->precision, fin, L, s->rho0, s->sums, 256, out, s->stream));
+  SRV_HIP(s, hipMemcpyAsync(s->pinned + 4 * r.slot, out, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                            s->stream));
+  SRV_HIP(s, hipEventRecord(s->slot_ev[r.slot], s->stream));
+  r.finish_queued = true;
+  return LBX_OK;
+}
+
+// Deliver runs whose sums have reached the host; optional field outputs are
+// copied synchronously (the run's arena space is still intact), then the
+// arena space and the slot are released.
+int collect(lbx_server* s, bool wait) {
   for (size_t i = 0; i < s->active.size();) {
     Run& r = s->active[i];
-    if (!(r.launched_last && r.last_step <= completed_step)) {
+    if (!r.finish_queued) {
       ++i;
       continue;
     }
+    hipError_t q = wait ? hipEventSynchronize(s->slot_ev[r.slot]) : hipEventQuery(s->slot_ev[r.slot]);
+    if (q == hipErrorNotReady) {
+      ++i;
+      continue;
+    }
+    if (q != hipSuccess) return sfail(s, LBX_E_HIP, "run completion: %s", hipGetErrorString(q));
     const lbx_request& rq = r.r.req;
     const long long n = cells_of(rq);
     lbx::Lattice L{rq.dims[0], rq.dims[1], rq.dims[2], r.qs};
@@ -211,6 +243,11 @@ int retire(lbx_server* s, long long completed_step) {
     res.status = LBX_OK;
     res.steps = rq.steps;
     res.cells = n;
+    res.mass = s->pinned[4 * r.slot];
+    for (int a = 0; a < 3; ++a) res.momentum[a] = s->pinned[4 * r.slot + 1 + a];
+    res.t_submit = r.r.t_submit;
+    res.t_admit = r.t_admit;
+    res.t_done = now_s();
     if (rq.out_rho || rq.out_u) {
       SRV_HIP(s, lbx::launch_hydro(s->precision, fin, L, R, s->staging, s->staging + n, s->rho0, s->stream));
       if (rq.out_rho)
@@ -226,44 +263,13 @@ int retire(lbx_server* s, long long completed_step) {
                                 hipMemcpyDeviceToHost, s->stream));
       SRV_HIP(s, hipStreamSynchronize(s->stream));
     }
-    SRV_HIP(s, lbx::launch_conservation(s->precision, fin, L, s->rho0, s->sums, 256, s->sums + 1024,
-                                        s->stream));
-    double sums[4];
-    SRV_HIP(s, hipMemcpyAsync(sums, s->sums + 1024, sizeof(sums), hipMemcpyDeviceToHost, s->stream));
-    SRV_HIP(s, hipStreamSynchronize(s->stream));
-    res.mass = sums[0];
-    for (int a = 0; a < 3; ++a) res.momentum[a] = sums[1 + a];
-    res.t_submit = r.r.t_submit;
-    res.t_admit = r.t_admit;
-    res.t_done = now_s();
     s->lattice_updates += n * (int64_t)rq.steps;
     s->arena.release(r.off, r.bytes);
+    s->free_slots.push_back(r.slot);
     s->done.push_back(res);
     s->active.erase(s->active.begin() + i);
   }
   return LBX_OK;
-}
-
-// Highest global step whose kernel the GPU has finished (queries only).
-long long completed_step(lbx_server* s, bool wait_oldest) {
-  long long done = s->known_done;
-  for (int k = 0; k < kInFlight; ++k) {
-    if (s->ev_step[k] < 0) continue;
-    hipError_t q = hipEventQuery(s->ev[k]);
-    if (q == hipSuccess) done = std::max(done, s->ev_step[k]);
-  }
-  if (wait_oldest) {
-    long long oldest = -1;
-    int ko = -1;
-    for (int k = 0; k < kInFlight; ++k)
-      if (s->ev_step[k] >= 0 && (oldest < 0 || s->ev_step[k] < oldest)) {
-        oldest = s->ev_step[k];
-        ko = k;
-      }
-    if (ko >= 0 && hipEventSynchronize(s->ev[ko]) == hipSuccess) done = std::max(done, oldest);
-  }
-  s->known_done = done;
-  return done;
 }
 
 int launch_step(lbx_server* s) {
@@ -293,13 +299,13 @@ int launch_step(lbx_server* s) {
   if (b.nruns == 0) return LBX_OK;
   // reuse the event slot of step - kInFlight once the GPU has passed it
   const int k = int(s->step % kInFlight);
-  if (s->ev_step[k] >= 0) {
-    SRV_HIP(s, hipEventSynchronize(s->ev[k]));
-    s->known_done = std::max(s->known_done, s->ev_step[k]);
-  }
+  if (s->ev_step[k] >= 0) SRV_HIP(s, hipEventSynchronize(s->ev[k]));
   SRV_HIP(s, lbx::launch_serve_step(s->precision, b, s->step, s->grid, s->stream));
   SRV_HIP(s, hipEventRecord(s->ev[k], s->stream));
   s->ev_step[k] = s->step;
+  for (Run& r : s->active)
+    if (r.launched_last && !r.finish_queued && r.last_step == s->step)
+      if (int rc = queue_finish(s, r)) return rc;
   ++s->step;
   return LBX_OK;
 }
@@ -336,7 +342,11 @@ int lbx_serve_create(lbx_ctx* ctx, int precision, double rho0, size_t arena_byte
   s->staging_bytes = (size_t)256 << 20;
   if (e == hipSuccess) e = hipMalloc(&s->arena_mem, arena_bytes);
   if (e == hipSuccess) e = hipMalloc(&s->staging, s->staging_bytes);
-  if (e == hipSuccess) e = hipMalloc(&s->sums, (1024 + 8) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&s->sums, (1024 + 4 * lbx::kServeMaxRuns) * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&s->pinned, 4 * lbx::kServeMaxRuns * sizeof(double));
+  for (int k = 0; k < lbx::kServeMaxRuns && e == hipSuccess; ++k)
+    e = hipEventCreateWithFlags(&s->slot_ev[k], hipEventDisableTiming);
+  for (int k = lbx::kServeMaxRuns - 1; k >= 0; --k) s->free_slots.push_back(k);
   for (int k = 0; k < kInFlight && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming);
   if (e != hipSuccess) {
     lbx_serve_destroy(s);
@@ -355,6 +365,9 @@ int lbx_serve_destroy(lbx_server* s) {
   if (s->arena_mem) (void)hipFree(s->arena_mem);
   if (s->staging) (void)hipFree(s->staging);
   if (s->sums) (void)hipFree(s->sums);
+  if (s->pinned) (void)hipHostFree(s->pinned);
+  for (int k = 0; k < lbx::kServeMaxRuns; ++k)
+    if (s->slot_ev[k]) (void)hipEventDestroy(s->slot_ev[k]);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return LBX_OK;
@@ -399,14 +412,14 @@ int lbx_serve_progress(lbx_server* s, int max_steps, int* active, int* queued) {
   if (!s) return LBX_E_ARG;
   SRV_HIP(s, hipSetDevice(s->device));
   for (int it = 0; it < std::max(max_steps, 1); ++it) {
-    if (int rc = retire(s, completed_step(s, false))) return rc;
+    if (int rc = collect(s, false)) return rc;
     if (int rc = admit(s)) return rc;
     bool pending = false;
     for (const Run& r : s->active) pending |= !r.launched_last;
     if (!pending) break;
     if (int rc = launch_step(s)) return rc;
   }
-  if (int rc = retire(s, completed_step(s, false))) return rc;
+  if (int rc = collect(s, false)) return rc;
   if (active) *active = (int)s->active.size();
   if (queued) *queued = (int)s->queue.size();
   return LBX_OK;
@@ -419,11 +432,8 @@ int lbx_serve_drain(lbx_server* s) {
     if (int rc = lbx_serve_progress(s, 16, &a, &q)) return rc;
     bool pending = false;
     for (const Run& r : s->active) pending |= !r.launched_last;
-    if (!pending && !s->active.empty()) {
-      SRV_HIP(s, hipStreamSynchronize(s->stream));
-      s->known_done = s->step - 1;
-      if (int rc = retire(s, s->known_done)) return rc;
-    }
+    if (!pending && !s->active.empty())
+      if (int rc = collect(s, true)) return rc;
   }
   return LBX_OK;
 }

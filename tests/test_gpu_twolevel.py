@@ -160,3 +160,37 @@ def test_level1_errors(ctx):
         s32.define_level(1, [((0, 0, 0), (3, 3, 3))])
     assert e.value.code == lbx.LBX_E_UNSUPPORTED
     s32.close()
+
+
+def test_graph_replays_match_eager_steps(ctx):
+    """The captured two-level step, replayed with eager downloads between the
+    replays, equals the eagerly launched step at every step (profiling forces
+    the eager path)."""
+    dims = (16, 8, 8)
+    rho, u = workloads.random_state(dims, seed=41, rho_sigma=1e-2, u_amp=0.02)
+    cboxes = workloads.chop(dims, 8)
+    fboxes = fine_boxes_in((8, 0, 0), (15, 15, 15), 8)
+
+    def make():
+        sim = lbx.Sim(ctx, dims)
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, 0.8)
+        sim.init_from_fields(rho, u)
+        sim.define_level(1, fboxes)
+        sim.set_tau(1, 1.1)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(fboxes)
+        return sim
+
+    eager, graph = make(), make()
+    try:
+        for step in range(5):
+            ctx.profile_enable(True)
+            eager.iterate(1)
+            ctx.profile_enable(False)
+            graph.iterate(1)
+            assert_bitwise(eager.get_f(0), graph.get_f(0), f"coarse after step {step + 1}")
+            assert_bitwise(eager.get_f_bbox(1)[0], graph.get_f_bbox(1)[0], f"fine after step {step + 1}")
+    finally:
+        eager.close()
+        graph.close()
