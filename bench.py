@@ -205,6 +205,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-shifted measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
+    ap.add_argument("--no-autotune", action="store_true", help="default workgroup shapes only")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the copy kernel (counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -245,6 +246,7 @@ def main():
     rho, u = workloads.random_state(dims, seed=seed, rho_sigma=1e-3, u_amp=0.01)
 
     with lbx.Context(0) as ctx:
+        ctx.set_autotune(not args.no_autotune)
         copy_gbs = None
         if not args.no_copy:
             copy_gbs = ctx.bench_copy(COPY_BYTES, reps=10)
@@ -253,6 +255,7 @@ def main():
         r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup)
         log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
             f"({r64['kernel_gbs']:.0f} GB/s)")
+        shape64 = ctx.tuned_shape(n, n, lbx.LBX_FP64)[0] if not args.no_autotune else None
         twolev = None
         if args.two_level_steps:
             log(f"[bench] two-level: {args.two_level_steps} coarse steps")
@@ -298,7 +301,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded SplitMix64 / Box-Muller initial state, equilibrium populations)",
         "config": {"workload": workload, "cells": cells, "boxes": r64["boxes"], "tau": tau,
-                   "seed": seed, "precision": "FP64"},
+                   "seed": seed, "precision": "FP64",
+                   "workgroup": list(shape64) if shape64 else "default"},
         "roofline": {
             "bound": "hbm",
             "achieved": round(r64["kernel_gbs"], 1),

@@ -135,6 +135,16 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]);
 int lbx_profile_enable(lbx_ctx* ctx, int enable);
 int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches);
 
+/* Autotuning of the stream-collide kernel's workgroup shape.  While
+ * enabled, the first launch for a new (nx, ny, precision) times every
+ * candidate shape on a small proxy lattice with the same rows and keeps the
+ * fastest for the context's lifetime; shapes never change results.  No
+ * reference counterpart.  lbx_tuned_shape reports the choice and its
+ * time per proxy launch (LBX_E_STATE if that geometry was never tuned). */
+int lbx_set_autotune(lbx_ctx* ctx, int enable);
+int lbx_tuned_shape(lbx_ctx* ctx, int nx, int ny, int precision, int32_t* bx, int32_t* by,
+                    float* ms_per_launch);
+
 /* Achievable read+write bandwidth of this device: a plain streaming copy
  * of `bytes` (read) into a second buffer (write), repeated `reps` times;
  * *gbs = 2*bytes*reps / time.  No reference counterpart: it is the measured

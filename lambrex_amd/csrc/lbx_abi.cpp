@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cstdarg>
+#include <map>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -90,6 +92,10 @@ struct lbx_ctx {
   size_t ev_used = 0;
   double prof_ms = 0.0;
   int64_t prof_launches = 0;
+  // autotuned workgroup shapes of the fused kernel, per (nx, ny, precision)
+  bool autotune = false;
+  std::map<std::tuple<int, int, int>, LaunchShape> shapes;
+  std::map<std::tuple<int, int, int>, std::vector<std::pair<LaunchShape, float>>> trials;
 };
 
 struct lbx_sim {
@@ -255,10 +261,89 @@ struct ProfileScope {
   }
 };
 
+// Autotuner: every candidate workgroup shape of the fused kernel is timed on
+// a proxy lattice with the same rows (nx, ny) and a few z-planes; the
+// fastest is kept for that (nx, ny, precision).  Shapes change only which
+// lanes compute which cells, never the arithmetic, so results do not depend
+// on the choice.
+std::vector<LaunchShape> shape_candidates(const Lattice& L) {
+  std::vector<LaunchShape> c;
+  for (int bx : {256, 128, 64, 32}) {
+    const int x = std::min(bx, L.nx);
+    LaunchShape s{x, std::max(1, std::min(256 / x, L.ny))};
+    bool seen = false;
+    for (const LaunchShape& o : c) seen |= (o.bx == s.bx && o.by == s.by);
+    if (!seen) c.push_back(s);
+  }
+  return c;
+}
+
+int tune_shape(lbx_ctx* ctx, const Lattice& L, int precision, LaunchShape* out) {
+  const auto key = std::make_tuple(L.nx, L.ny, precision);
+  auto it = ctx->shapes.find(key);
+  if (it != ctx->shapes.end()) {
+    *out = it->second;
+    return LBX_OK;
+  }
+  Lattice P = L;
+  P.nz = std::min(L.nz, std::max(4, (1 << 22) / std::max(1, L.nx * L.ny)));
+  P.qs = ((long long)P.nx * P.ny * P.nz + 63) / 64 * 64;
+  const size_t bytes = (size_t)P.qs * lbx::NV_TOTAL * elem_size(precision);
+  DeviceTemp a, b;
+  LBX_HIP(ctx, hipMalloc(&a.p, bytes));
+  LBX_HIP(ctx, hipMalloc(&b.p, bytes));
+  // a positive finite value in every element (0x3F800000 pairs / floats)
+  LBX_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)a.p, 0x3F800000, bytes / 4, ctx->stream));
+  LBX_HIP(ctx, hipMemsetD32Async((hipDeviceptr_t)b.p, 0x3F800000, bytes / 4, ctx->stream));
+  const Relax r{1.0 / 1.1, 1.0 / 1.1, 1.0};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LBX_HIP(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(ctx, LBX_E_HIP, "hipEventCreate");
+  }
+  std::vector<std::pair<LaunchShape, float>> trial;
+  LaunchShape best = lbx::default_shape(L);
+  float best_ms = 1e30f;
+  hipError_t e = hipSuccess;
+  for (const LaunchShape& sh : shape_candidates(P)) {
+    auto launch = [&](int k) {
+      hipError_t ee = hipSuccess;
+      for (int i = 0; i < k && ee == hipSuccess; ++i)
+        ee = precision == LBX_FP64
+                 ? lbx::launch_step_f64(lbx::STEP_FUSED, (const double*)a.p, (double*)b.p, P, r, sh, ctx->stream)
+                 : lbx::launch_step_f32s(lbx::STEP_FUSED, (const float*)a.p, (float*)b.p, P, r, sh, ctx->stream);
+      return ee;
+    };
+    e = launch(2);
+    if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+    if (e == hipSuccess) e = launch(6);
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) break;
+    trial.push_back({sh, ms / 6});
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = sh;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return hip_fail(ctx, e, "autotune");
+  ctx->shapes[key] = best;
+  ctx->trials[key] = trial;
+  *out = best;
+  return LBX_OK;
+}
+
 int launch_step(lbx_sim* sim, Level& L, lbx::StepKind kind) {
   lbx_ctx* ctx = sim->ctx;
   Relax r{L.omega_s, L.omega_b, sim->rho0};
-  const LaunchShape shape = lbx::default_shape(L.lat);
+  LaunchShape shape = lbx::default_shape(L.lat);
+  if (ctx->autotune)
+    if (int rc = tune_shape(ctx, L.lat, sim->precision, &shape)) return rc;
   ProfileScope ps(ctx, kind == lbx::STEP_FUSED);
   if (ps.rc) return ps.rc;
   hipError_t e;
@@ -852,6 +937,27 @@ int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches) {
   if (int rc = drain_profile(ctx)) return rc;
   *ms = ctx->prof_ms;
   *launches = ctx->prof_launches;
+  return LBX_OK;
+}
+
+int lbx_set_autotune(lbx_ctx* ctx, int enable) {
+  if (!ctx) return fail(nullptr, LBX_E_ARG, "null context");
+  ctx->autotune = enable != 0;
+  return LBX_OK;
+}
+
+int lbx_tuned_shape(lbx_ctx* ctx, int nx, int ny, int precision, int32_t* bx, int32_t* by,
+                    float* ms_per_launch) {
+  if (!ctx || !bx || !by) return fail(ctx, LBX_E_ARG, "null argument");
+  auto it = ctx->shapes.find(std::make_tuple(nx, ny, precision));
+  if (it == ctx->shapes.end()) return fail(ctx, LBX_E_STATE, "no tuned shape for %dx%d rows", nx, ny);
+  *bx = it->second.bx;
+  *by = it->second.by;
+  if (ms_per_launch) {
+    *ms_per_launch = 0.0f;
+    for (auto& t : ctx->trials[it->first])
+      if (t.first.bx == it->second.bx && t.first.by == it->second.by) *ms_per_launch = t.second;
+  }
   return LBX_OK;
 }
 

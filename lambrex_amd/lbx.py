@@ -47,6 +47,9 @@ SIGNATURES = {
     "lbx_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "lbx_bench_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
+    "lbx_set_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbx_tuned_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
+                                       ctypes.POINTER(ctypes.c_float)]),
 }
 
 # Serving structs (include/lbx.h).
@@ -151,6 +154,15 @@ class Context:
         g = ctypes.c_double()
         self.check(self._lib.lbx_bench_copy(self.h, int(nbytes), int(reps), ctypes.byref(g)))
         return g.value
+
+    def set_autotune(self, on=True):
+        self.check(self._lib.lbx_set_autotune(self.h, 1 if on else 0))
+
+    def tuned_shape(self, nx, ny, precision=LBX_FP64):
+        bx, by, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+        self.check(self._lib.lbx_tuned_shape(self.h, int(nx), int(ny), int(precision), ctypes.byref(bx),
+                                             ctypes.byref(by), ctypes.byref(ms)))
+        return (bx.value, by.value), ms.value
 
     def close(self):
         if self.h:

@@ -249,3 +249,20 @@ def test_error_paths(ctx):
     assert b"box 3" in lib.lbx_last_error(ctx.h)
     assert lib.lbx_iterate(sim.h, -1) == lbx.LBX_E_ARG
     sim.close()
+
+
+def test_autotuned_shapes_do_not_change_results(ctx):
+    """With autotuning on, the fused kernel runs with the fastest measured
+    workgroup shape; populations stay bit-identical to the default shape."""
+    dims = (48, 40, 24)
+    boxes = workloads.chop(dims, 16)
+    rho, u = workloads.random_state(dims, seed=77, rho_sigma=1e-2, u_amp=0.02)
+    f_default, _, _ = run_gpu(ctx, dims, boxes, 0.7, rho, u, 6)
+    ctx.set_autotune(True)
+    try:
+        f_tuned, _, _ = run_gpu(ctx, dims, boxes, 0.7, rho, u, 6)
+        (bx, by), ms = ctx.tuned_shape(48, 40)
+    finally:
+        ctx.set_autotune(False)
+    assert_bitwise(f_tuned, f_default, "autotuned vs default shape")
+    assert bx * by <= 256 and bx in (48, 32) and ms > 0

@@ -1,0 +1,205 @@
+"""Cross-check of the oracle's two-level Rohde cycle (oracle/lbx_oracle.c,
+restating src/AmrSim.cpp:430-631) against a second, independent restatement
+written here with numpy arrays per FAB.
+
+No reference test pins the two-level path, so this is an N-version check of
+the bookkeeping (FillPatchSingleLevel / FillPatchTwoLevels with
+piecewise-constant interpolation, the fine mask, the outer-ghost comp-0 zero,
+the two fine substeps without ghost refresh, ZeroInvalidComponents, and the
+order of sum_fine_to_coarse's additions).  The per-cell collision used on
+both sides is the oracle's, which test_oracle.py pins separately.
+"""
+import numpy as np
+import pytest
+
+from lambrex_amd import workloads
+from oracle import oracle as orc
+
+C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+              [1, 1, 1], [1, 1, -1], [1, -1, 1], [1, -1, -1],
+              [-1, 1, 1], [-1, 1, -1], [-1, -1, 1], [-1, -1, -1]])
+G = 2  # ghost layers
+
+
+class Level:
+    def __init__(self, boxes, dom, tau):
+        self.boxes = [(np.array(lo), np.array(hi)) for lo, hi in boxes]
+        self.dom = np.array(dom)
+        self.tau = tau
+        self.now = [self.fab(b) for b in self.boxes]
+        self.next = [self.fab(b) for b in self.boxes]
+
+    @staticmethod
+    def fab(b):
+        n = b[1] - b[0] + 1 + 2 * G
+        return np.zeros((15, n[2], n[1], n[0]))
+
+    def idx(self, b, p):
+        lo = self.boxes[b][0]
+        return (p[2] - lo[2] + G, p[1] - lo[1] + G, p[0] - lo[0] + G)
+
+    def valid(self, b, p):
+        lo, hi = self.boxes[b]
+        return bool(np.all(p >= lo) and np.all(p <= hi))
+
+    def owner(self, p):
+        w = np.mod(p, self.dom)
+        for b in range(len(self.boxes)):
+            if self.valid(b, w):
+                return b, w
+        return None, w
+
+    def grown(self, b):
+        lo, hi = self.boxes[b]
+        for k in range(lo[2] - G, hi[2] + G + 1):
+            for j in range(lo[1] - G, hi[1] + G + 1):
+                for i in range(lo[0] - G, hi[0] + G + 1):
+                    yield np.array([i, j, k])
+
+    def shell_depth(self, b, p):
+        """0 for valid cells, else the ghost layer (1 or 2)."""
+        lo, hi = self.boxes[b]
+        return int(max(np.max(lo - p), np.max(p - hi), 0))
+
+
+def fill_single(L, dst, src, ghosts_only):
+    for b in range(len(L.boxes)):
+        for p in L.grown(b):
+            if ghosts_only and L.valid(b, p):
+                continue
+            s, w = L.owner(p)
+            dst[b][(slice(None),) + L.idx(b, p)] = src[s][(slice(None),) + L.idx(s, w)]
+
+
+def collide(L, f, mask=None):
+    ws, wb = 1.0 / (L.tau + 0.5), 1.0 / (L.tau + 0.5)
+    for b in range(len(L.boxes)):
+        lo, hi = L.boxes[b]
+        for k in range(lo[2], hi[2] + 1):
+            for j in range(lo[1], hi[1] + 1):
+                for i in range(lo[0], hi[0] + 1):
+                    ix = (slice(None),) + L.idx(b, (i, j, k))
+                    if mask is not None and mask(i, j, k):
+                        f[b][ix] = 0.0
+                    else:
+                        f[b][ix] = orc.collide_cell(f[b][ix], ws, wb)
+
+
+def stream(L, f):
+    out = [np.zeros_like(a) for a in f]
+    for b in range(len(L.boxes)):
+        for p in L.grown(b):
+            if L.shell_depth(b, p) > 1:
+                continue  # outer layer of the fresh array: zero
+            for q in range(15):
+                out[b][(q,) + L.idx(b, p)] = f[b][(q,) + L.idx(b, p - C[q])]
+    return out
+
+
+def fill_two_levels(Cl, F, dst):
+    for b in range(len(F.boxes)):
+        for p in F.grown(b):
+            s, w = F.owner(p)
+            if s is not None:
+                dst[b][(slice(None),) + F.idx(b, p)] = F.now[s][(slice(None),) + F.idx(s, w)]
+            else:
+                cs, cw = Cl.owner(w // 2)
+                dst[b][(slice(None),) + F.idx(b, p)] = Cl.now[cs][(slice(None),) + Cl.idx(cs, cw)]
+            if F.shell_depth(b, p) == G:
+                dst[b][(0,) + F.idx(b, p)] = 0.0
+
+
+def zero_invalid(L, f):
+    for b in range(len(L.boxes)):
+        for p in L.grown(b):
+            if L.shell_depth(b, p) == 0:
+                continue
+            for m in range(15):
+                if not L.valid(b, p - 2 * C[m]):
+                    f[b][(m,) + L.idx(b, p)] = 0.0
+
+
+def sum_fine_to_coarse(F, Cl, cdst):
+    for b in range(len(F.boxes)):
+        lo, hi = F.boxes[b]
+        for ci in range(lo[0] // 2 - 1, hi[0] // 2 + 2):
+            for cj in range(lo[1] // 2 - 1, hi[1] // 2 + 2):
+                for ck in range(lo[2] // 2 - 1, hi[2] // 2 + 2):
+                    t, w = Cl.owner(np.array([ci, cj, ck]))
+                    for q in range(15):
+                        s = 0.0
+                        for dk in range(2):
+                            for dj in range(2):
+                                for di in range(2):
+                                    s += F.now[b][(q,) + F.idx(b, (2 * ci + di, 2 * cj + dj, 2 * ck + dk))]
+                        s *= 0.125
+                        cdst[t][(q,) + Cl.idx(t, w)] += s
+
+
+def rohde_cycle(Cl, F, mask):
+    fill_single(Cl, Cl.next, Cl.now, False)
+    collide(Cl, Cl.next, mask)
+    fill_two_levels(Cl, F, F.next)
+    collide(F, F.next)
+    F.next = stream(F, F.next)
+    collide(F, F.next)
+    F.next = stream(F, F.next)
+    zero_invalid(F, F.next)
+    F.now, F.next = F.next, F.now
+    Cl.next = stream(Cl, Cl.next)
+    sum_fine_to_coarse(F, Cl, Cl.next)
+    zero_invalid(Cl, Cl.next)
+    fill_single(Cl, Cl.next, Cl.next, True)
+    Cl.now, Cl.next = Cl.next, Cl.now
+
+
+def level_valid_array(L, bbox_lo, bbox_hi):
+    n = np.array(bbox_hi) - np.array(bbox_lo) + 1
+    out = np.zeros((15, n[2], n[1], n[0]))
+    for b in range(len(L.boxes)):
+        lo, hi = L.boxes[b]
+        out[:, lo[2] - bbox_lo[2]:hi[2] - bbox_lo[2] + 1, lo[1] - bbox_lo[1]:hi[1] - bbox_lo[1] + 1,
+            lo[0] - bbox_lo[0]:hi[0] - bbox_lo[0] + 1] = L.now[b][:, G:-G, G:-G, G:-G]
+    return out
+
+
+@pytest.mark.parametrize("mask_kind", ["coarse", "fine"])
+def test_two_level_cycle_matches_independent_restatement(mask_kind):
+    dims = (8, 4, 4)
+    tau = 0.8
+    cboxes = [(tuple(a), tuple(b)) for a, b in workloads.chop(dims, 4)]
+    fboxes = [((4, 0, 0), (7, 3, 7)), ((4, 4, 0), (7, 7, 7)), ((8, 0, 0), (11, 7, 3)), ((8, 0, 4), (11, 7, 7))]
+    mboxes = [(tuple(2 * x for x in a), tuple(2 * x + 1 for x in b)) for a, b in cboxes] \
+        if mask_kind == "coarse" else fboxes
+    rho, u = workloads.random_state(dims, seed=5, rho_sigma=1e-2, u_amp=0.02)
+    o = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in cboxes])
+    o.init(np.ascontiguousarray(rho.transpose(2, 1, 0)).reshape(-1),
+           np.ascontiguousarray(u.transpose(3, 2, 1, 0)).reshape(-1))
+    f0 = o.get_f(0)
+    o.add_fine([(list(a), list(b)) for a, b in fboxes], [(list(a), list(b)) for a, b in mboxes])
+
+    Cl = Level(cboxes, dims, tau)
+    for b, (lo, hi) in enumerate(cboxes):
+        Cl.now[b][:, G:-G, G:-G, G:-G] = f0[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+    F = Level(fboxes, tuple(2 * d for d in dims), 2 * (tau - 0.5) + 0.5)
+    for b in range(len(fboxes)):
+        lo, hi = F.boxes[b]
+        for k in range(lo[2], hi[2] + 1):
+            for j in range(lo[1], hi[1] + 1):
+                for i in range(lo[0], hi[0] + 1):
+                    cs, cw = Cl.owner(np.array([i // 2, j // 2, k // 2]))
+                    F.now[b][(slice(None),) + F.idx(b, (i, j, k))] = Cl.now[cs][(slice(None),) + Cl.idx(cs, cw)]
+
+    def mask(i, j, k):
+        return any(a[0] // 2 <= i <= b[0] // 2 and a[1] // 2 <= j <= b[1] // 2 and a[2] // 2 <= k <= b[2] // 2
+                   for a, b in mboxes)
+
+    for step in range(2):
+        rohde_cycle(Cl, F, mask)
+        o.iterate(1)
+        fc = level_valid_array(Cl, (0, 0, 0), tuple(d - 1 for d in dims))
+        bb = o.bbox(1)
+        ff = level_valid_array(F, tuple(bb[:3]), tuple(bb[3:]))
+        assert (fc.view(np.uint64) == o.get_f(0).view(np.uint64)).all(), f"coarse, step {step + 1}"
+        assert (ff.view(np.uint64) == o.get_f(1).view(np.uint64)).all(), f"fine, step {step + 1}"
+    o.close()
