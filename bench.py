@@ -205,7 +205,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-shifted measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
-    ap.add_argument("--no-autotune", action="store_true", help="default workgroup shapes only")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every workgroup shape first and use the fastest (default: the default shape)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the copy kernel (counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -246,7 +247,7 @@ def main():
     rho, u = workloads.random_state(dims, seed=seed, rho_sigma=1e-3, u_amp=0.01)
 
     with lbx.Context(0) as ctx:
-        ctx.set_autotune(not args.no_autotune)
+        ctx.set_autotune(args.autotune)
         copy_gbs = None
         if not args.no_copy:
             copy_gbs = ctx.bench_copy(COPY_BYTES, reps=10)
@@ -255,7 +256,7 @@ def main():
         r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup)
         log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
             f"({r64['kernel_gbs']:.0f} GB/s)")
-        shape64 = ctx.tuned_shape(n, n, lbx.LBX_FP64)[0] if not args.no_autotune else None
+        shape64 = ctx.tuned_shape(n, n, lbx.LBX_FP64)[0]
         twolev = None
         if args.two_level_steps:
             log(f"[bench] two-level: {args.two_level_steps} coarse steps")
@@ -280,6 +281,7 @@ def main():
             log("[bench] FP32-shifted, same workload")
             r32 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP32_SHIFTED,
                           args.steps, args.warmup)
+            shape32 = ctx.tuned_shape(n, n, lbx.LBX_FP32_SHIFTED)[0]
             log(f"[bench] FP32-shifted: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_avg_ms']:.4f} ms "
                 f"({r32['kernel_gbs']:.0f} GB/s)")
 
@@ -302,7 +304,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 / Box-Muller initial state, equilibrium populations)",
         "config": {"workload": workload, "cells": cells, "boxes": r64["boxes"], "tau": tau,
                    "seed": seed, "precision": "FP64",
-                   "workgroup": list(shape64) if shape64 else "default"},
+                   "workgroup": {"bx": shape64[0], "by": shape64[1], "max_workgroups_per_cu": shape64[2],
+                                 "autotuned": bool(args.autotune)}},
         "roofline": {
             "bound": "hbm",
             "achieved": round(r64["kernel_gbs"], 1),
@@ -326,8 +329,11 @@ def main():
             "speedup_vs_fp64": round(r32["mlups"] / r64["mlups"], 3),
             "roofline": {"bound": "hbm", "achieved": round(r32["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(r32["kernel_gbs"] / HBM_PEAK_GBS, 4),
+                         "traffic": traffic_for("k_step_f32s<STEP_FUSED>", f"{n}^3"),
                          "kernel": "k_step_f32s<STEP_FUSED>",
-                         "kernel_avg_ms": round(r32["kernel_avg_ms"], 5)},
+                         "kernel_avg_ms": round(r32["kernel_avg_ms"], 5),
+                         "algorithmic_bytes_per_launch": BYTES_PER_LU[lbx.LBX_FP32_SHIFTED] * cells},
+            "workgroup": {"bx": shape32[0], "by": shape32[1], "max_workgroups_per_cu": shape32[2]},
         }
     if twolev:
         out["two_level"] = twolev

@@ -135,15 +135,18 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]);
 int lbx_profile_enable(lbx_ctx* ctx, int enable);
 int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches);
 
-/* Autotuning of the stream-collide kernel's workgroup shape.  While
- * enabled, the first launch for a new (nx, ny, precision) times every
- * candidate shape on a small proxy lattice with the same rows and keeps the
+/* Autotuning of the stream-collide kernel's launch shape: workgroup extent
+ * (bx, by) and the cap on resident workgroups per CU (0 = none, else 2, 3,
+ * 4 or 6; enforced by reserving LDS the kernel does not use).
+ * While enabled, the first launch for a new (nx, ny, precision) times every
+ * candidate on a small proxy lattice with the same rows and keeps the
  * fastest for the context's lifetime; shapes never change results.  No
- * reference counterpart.  lbx_tuned_shape reports the choice and its
- * time per proxy launch (LBX_E_STATE if that geometry was never tuned). */
+ * reference counterpart.  lbx_tuned_shape reports the shape launches of
+ * that geometry use: the tuned choice and its time per proxy launch, or the
+ * default shape (time 0) if the geometry was never tuned. */
 int lbx_set_autotune(lbx_ctx* ctx, int enable);
 int lbx_tuned_shape(lbx_ctx* ctx, int nx, int ny, int precision, int32_t* bx, int32_t* by,
-                    float* ms_per_launch);
+                    int32_t* wg_per_cu, float* ms_per_launch);
 
 /* Achievable read+write bandwidth of this device: a plain streaming copy
  * of `bytes` (read) into a second buffer (write), repeated `reps` times;

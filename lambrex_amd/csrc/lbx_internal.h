@@ -15,15 +15,42 @@ constexpr int NV_TOTAL = 15;
 // Device ordinal of a context (lbx_abi.cpp).
 int ctx_device(const lbx_ctx* ctx);  // populations per cell (D3Q15)
 
-// A periodic lattice: one level stored as a single SoA array over the whole
-// domain, population-major, x fastest.  Population q of cell (x,y,z) lives at
-// q*qs + (z*ny + y)*nx + x.  Boxes of the level are index ranges into it, so
-// neighbouring boxes exchange nothing: the pull reads the neighbour box's
-// cells directly (periodic wrap at the domain faces).
+#if defined(__HIPCC__)
+#define LBX_IHD __host__ __device__ __forceinline__
+#else
+#define LBX_IHD inline
+#endif
+
+// A periodic lattice: one level stored as a single array over the whole
+// domain, z-plane major: each z-plane holds its 15 population planes one
+// after the other, x fastest.  Population q of cell (x,y,z) lives at
+// z*zs + q*ps + y*nx + x.  Keeping a plane's populations together keeps the
+// 30 streams of a stream-collide pass inside a few MB of address space
+// (measured ~4% faster than population-major at 256^3).  Boxes of the level
+// are index ranges into it, so neighbouring boxes exchange nothing: the pull
+// reads the neighbour box's cells directly (periodic wrap at the faces).
 struct Lattice {
   int nx, ny, nz;
-  long long qs;  // population stride in elements (multiple of 64)
+  long long ps;  // population-plane stride: nx*ny rounded up to 64 elements
+  long long zs;  // z-plane stride: 15 * ps
+  LBX_IHD long long at(int q, long long plane_cell, long long z) const { return z * zs + q * ps + plane_cell; }
+  // linear cell index (z*ny + y)*nx + x
+  LBX_IHD long long at_cell(int q, long long c) const {
+    const long long plane = (long long)nx * ny;
+    return at(q, c % plane, c / plane);
+  }
+  LBX_IHD long long elems() const { return nz * zs; }
 };
+
+inline Lattice make_lattice(int nx, int ny, int nz) {
+  Lattice L;
+  L.nx = nx;
+  L.ny = ny;
+  L.nz = nz;
+  L.ps = ((long long)nx * ny + 63) / 64 * 64;
+  L.zs = NV_TOTAL * L.ps;
+  return L;
+}
 
 struct Relax {
   double omega_s, omega_b;
@@ -40,10 +67,17 @@ enum StepKind { STEP_FUSED = 0, STEP_COLLIDE = 1, STEP_STREAM = 2 };
 
 // Launch configuration of the lattice stream-collide kernels.
 struct LaunchShape {
-  int bx, by;  // block extent in x and y (bx*by <= 256)
+  int bx, by;     // block extent in x and y (bx*by <= 256)
+  int cap;        // resident workgroups per CU (0 = no cap)
+  int lds_bytes;  // dynamic LDS reserved per workgroup to enforce `cap`
 };
 
-LaunchShape default_shape(const Lattice& L);
+// Workgroups of 256 lanes over whole x-rows; resident-workgroup cap 3 per
+// CU for FP64 (precision 0), 6 for FP32 (the measured optima at 256^3).
+LaunchShape default_shape(const Lattice& L, int precision);
+// Dynamic LDS bytes that limit residency to `cap` workgroups per CU on the
+// current device (0 for no cap, -1 if the device cannot reserve that much).
+int lds_bytes_for_cap(int cap);
 
 // FP64 storage
 hipError_t launch_step_f64(StepKind kind, const double* src, double* dst, const Lattice& L,
@@ -90,7 +124,7 @@ hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& 
                               const PatchDesc* boxes, int nbox, int max_cells,
                               const unsigned char* mask, const Relax& r, hipStream_t st);
 hipError_t launch_sum_fine_to_coarse(const double* fnow, double* cnext, const Lattice& L,
-                                     const PatchDesc* patches, const int* cell, const int* start,
+                                     const PatchDesc* patches, const long long* cell, const int* start,
                                      const int4* contrib, int ncell, hipStream_t st);
 hipError_t launch_fine_from_coarse(double* fnow, const double* cnow, const Lattice& clat,
                                    const PatchDesc* patches, int npatch, int max_cells,
@@ -108,7 +142,7 @@ struct ServeRun {
   int item_begin;    // first work item of this run in the batch
   int steps;         // LB steps of the run (passes 0..steps)
   long long start;   // global step of pass 0
-  long long qs;      // population stride
+  long long ps;      // population-plane stride of the run's lattice (zs = 15 ps)
   double omega_s, omega_b, rho0;
 };
 constexpr int kServeMaxRuns = 44;
@@ -119,8 +153,8 @@ struct ServeBatch {
 };
 hipError_t launch_serve_step(int precision, const ServeBatch& b, long long step, int grid,
                              hipStream_t st);
-hipError_t launch_serve_init_pulse(int precision, void* f, const int* n, long long qs, int axis,
-                                   int plane, double amplitude, double rho0, hipStream_t st);
+hipError_t launch_serve_init_pulse(int precision, void* f, const Lattice& L, int axis, int plane,
+                                   double amplitude, double rho0, hipStream_t st);
 
 // Streaming copy of n doubles (bandwidth ceiling / counter calibration).
 hipError_t launch_copy(const double* a, double* b, long long n, hipStream_t st);

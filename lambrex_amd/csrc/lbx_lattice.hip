@@ -6,9 +6,10 @@
 // pull-stream valid+1 ghost layer into a fresh array, swap.  That is four
 // passes over 15 populations per step.  Here one pass does pull + collide:
 // h'(x) = C(S(h))(x), reading population q of the upstream cell x - c_q and
-// writing the collided populations of x.  Populations are SoA (one plane
-// per q, x fastest), so every load and store of a wavefront is a contiguous
-// 64-element run (shifted by at most one element for c_x = +-1).
+// writing the collided populations of x.  Each z-plane stores its 15
+// population planes in turn (x fastest), so every load and store of a
+// wavefront is a contiguous 64-element run (shifted by at most one element
+// for c_x = +-1).
 //
 // Bandwidth bound: 15 loads + 15 stores per lattice update and ~430 FP64
 // add/mul (+5 divisions).  No MFMA: there is no dense contraction here.
@@ -23,44 +24,51 @@ namespace lbx {
 namespace {
 
 // Source offsets of the 15 populations of cell (x, y, z) on a periodic
-// lattice.  xs/ys/zs are indexed by 1 + c: the upstream coordinate of
+// lattice.  xs/ys/zo are indexed by 1 + c: the upstream coordinate of
 // x is x - c, so index 0 (c = -1) holds x+1, 1 holds x, 2 (c = +1) holds x-1.
 struct Upstream {
-  int xs[3], ys[3], zs[3];
+  int xs[3], ys[3];
+  long long zo[3];
 };
 
 __device__ __forceinline__ Upstream upstream(int x, int y, int z, const Lattice& L) {
   Upstream u;
   const int row = L.nx;
-  const int plane = L.nx * L.ny;
   u.xs[0] = (x == L.nx - 1) ? 0 : x + 1;
   u.xs[1] = x;
   u.xs[2] = (x == 0) ? L.nx - 1 : x - 1;
   u.ys[0] = ((y == L.ny - 1) ? 0 : y + 1) * row;
   u.ys[1] = y * row;
   u.ys[2] = ((y == 0) ? L.ny - 1 : y - 1) * row;
-  u.zs[0] = ((z == L.nz - 1) ? 0 : z + 1) * plane;
-  u.zs[1] = z * plane;
-  u.zs[2] = ((z == 0) ? L.nz - 1 : z - 1) * plane;
+  u.zo[0] = ((z == L.nz - 1) ? 0 : z + 1) * L.zs;
+  u.zo[1] = z * L.zs;
+  u.zo[2] = ((z == 0) ? L.nz - 1 : z - 1) * L.zs;
   return u;
 }
 
+// self: (z*zs, y*nx + x) of the cell itself
 template <typename S, bool STREAM>
-__device__ __forceinline__ void load_cell(const S* __restrict__ src, long long qs, int self,
-                                          const Upstream& u, S* f) {
+__device__ __forceinline__ void load_cell(const S* __restrict__ src, const Lattice& L, long long selfz,
+                                          int selfp, const Upstream& u, S* f) {
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    const int off = STREAM ? (u.zs[1 + kC[q][2]] + u.ys[1 + kC[q][1]] + u.xs[1 + kC[q][0]]) : self;
-    f[q] = src[q * qs + off];
+    const long long off = STREAM ? (u.zo[1 + kC[q][2]] + u.ys[1 + kC[q][1]] + u.xs[1 + kC[q][0]])
+                                 : selfz + selfp;
+    f[q] = src[off + q * L.ps];
   }
 }
 
 template <typename S>
-__device__ __forceinline__ void store_cell(S* __restrict__ dst, long long qs, int self, const S* f) {
+__device__ __forceinline__ void store_cell(S* __restrict__ dst, const Lattice& L, long long selfz, int selfp,
+                                           const S* f) {
 #pragma unroll
-  for (int q = 0; q < NV; ++q) __builtin_nontemporal_store(f[q], dst + q * qs + self);
+  for (int q = 0; q < NV; ++q) __builtin_nontemporal_store(f[q], dst + selfz + q * L.ps + selfp);
 }
 
+// Resident workgroups per CU are capped at launch time by reserving
+// dynamic LDS (LaunchShape::lds_bytes); the kernels do not use it.  With 30
+// streams per cell, fewer resident wavefronts contend less for DRAM: 3
+// workgroups of 256 lanes per CU is ~6% faster than no cap in FP64.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ src,
                                                   double* __restrict__ dst, Lattice L, Relax r) {
@@ -68,16 +76,17 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ src
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
   if (x >= L.nx || y >= L.ny) return;
-  const int self = (z * L.ny + y) * L.nx + x;
+  const long long selfz = (long long)z * L.zs;
+  const int selfp = y * L.nx + x;
   double f[NV];
   if (KIND == STEP_COLLIDE) {
-    load_cell<double, false>(src, L.qs, self, Upstream{}, f);
+    load_cell<double, false>(src, L, selfz, selfp, Upstream{}, f);
   } else {
     const Upstream u = upstream(x, y, z, L);
-    load_cell<double, true>(src, L.qs, self, u, f);
+    load_cell<double, true>(src, L, selfz, selfp, u, f);
   }
   if (KIND != STEP_STREAM) collide_f64(f, r.omega_s, r.omega_b);
-  store_cell<double>(dst, L.qs, self, f);
+  store_cell<double>(dst, L, selfz, selfp, f);
 }
 
 template <int KIND>
@@ -87,17 +96,18 @@ __global__ __launch_bounds__(256) void k_step_f32s(const float* __restrict__ src
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
   if (x >= L.nx || y >= L.ny) return;
-  const int self = (z * L.ny + y) * L.nx + x;
+  const long long selfz = (long long)z * L.zs;
+  const int selfp = y * L.nx + x;
   float f[NV];
   if (KIND == STEP_COLLIDE) {
-    load_cell<float, false>(src, L.qs, self, Upstream{}, f);
+    load_cell<float, false>(src, L, selfz, selfp, Upstream{}, f);
   } else {
     const Upstream u = upstream(x, y, z, L);
-    load_cell<float, true>(src, L.qs, self, u, f);
+    load_cell<float, true>(src, L, selfz, selfp, u, f);
   }
   if (KIND != STEP_STREAM)
     collide_f32_shifted(f, float(r.rho0), float(r.omega_s), float(r.omega_b));
-  store_cell<float>(dst, L.qs, self, f);
+  store_cell<float>(dst, L, selfz, selfp, f);
 }
 
 dim3 grid_of(const Lattice& L, LaunchShape s) {
@@ -127,15 +137,16 @@ __global__ __launch_bounds__(256) void k_equilibrium(S* __restrict__ f, Lattice 
   if (i >= n) return;
   int x, y, z;
   region_cell(R, i, x, y, z);
-  const int self = (z * L.ny + y) * L.nx + x;
+  const long long selfz = (long long)z * L.zs;
+  const int selfp = y * L.nx + x;
   double fe[NV];
   equilibrium_f64(rho[i], u[i], u[n + i], u[2 * n + i], fe);
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     if constexpr (sizeof(S) == 8)
-      f[q * L.qs + self] = fe[q];
+      f[selfz + q * L.ps + selfp] = fe[q];
     else
-      f[q * L.qs + self] = float(fe[q] - kT.W[q] * rho0);
+      f[selfz + q * L.ps + selfp] = float(fe[q] - kT.W[q] * rho0);
   }
 }
 
@@ -147,13 +158,13 @@ __global__ __launch_bounds__(256) void k_pack(const S* __restrict__ f, Lattice L
   if (i >= n) return;
   int x, y, z;
   region_cell(R, i, x, y, z);
-  const int self = (z * L.ny + y) * L.nx + x;
+  const long long self = L.at(0, y * L.nx + x, z);
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     if constexpr (sizeof(S) == 8)
-      out[q * (long long)n + i] = f[q * L.qs + self];
+      out[q * (long long)n + i] = f[self + q * L.ps];
     else
-      out[q * (long long)n + i] = kT.W[q] * rho0 + double(f[q * L.qs + self]);
+      out[q * (long long)n + i] = kT.W[q] * rho0 + double(f[self + q * L.ps]);
   }
 }
 
@@ -165,26 +176,27 @@ __global__ __launch_bounds__(256) void k_unpack(S* __restrict__ f, Lattice L, Re
   if (i >= n) return;
   int x, y, z;
   region_cell(R, i, x, y, z);
-  const int self = (z * L.ny + y) * L.nx + x;
+  const long long self = L.at(0, y * L.nx + x, z);
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     if constexpr (sizeof(S) == 8)
-      f[q * L.qs + self] = in[q * (long long)n + i];
+      f[self + q * L.ps] = in[q * (long long)n + i];
     else
-      f[q * L.qs + self] = float(in[q * (long long)n + i] - kT.W[q] * rho0);
+      f[self + q * L.ps] = float(in[q * (long long)n + i] - kT.W[q] * rho0);
   }
 }
 
 // FP64 populations of one cell from either storage.
+// self = element of population 0 of the cell; populations are ps apart.
 template <typename S>
-__device__ __forceinline__ void cell_f64(const S* __restrict__ f, long long qs, int self,
+__device__ __forceinline__ void cell_f64(const S* __restrict__ f, long long ps, long long self,
                                          double rho0, double* g) {
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     if constexpr (sizeof(S) == 8)
-      g[q] = f[q * qs + self];
+      g[q] = f[self + q * ps];
     else
-      g[q] = kT.W[q] * rho0 + double(f[q * qs + self]);
+      g[q] = kT.W[q] * rho0 + double(f[self + q * ps]);
   }
 }
 
@@ -197,9 +209,8 @@ __global__ __launch_bounds__(256) void k_hydro(const S* __restrict__ f, Lattice 
   if (i >= n) return;
   int x, y, z;
   region_cell(R, i, x, y, z);
-  const int self = (z * L.ny + y) * L.nx + x;
   double g[NV];
-  cell_f64<S>(f, L.qs, self, rho0, g);
+  cell_f64<S>(f, L.ps, L.at(0, y * L.nx + x, z), rho0, g);
   double r, u0, u1, u2;
   hydro_f64(g, r, u0, u1, u2);
   rho[i] = r;
@@ -234,7 +245,7 @@ __global__ __launch_bounds__(256) void k_conservation(const S* __restrict__ f, L
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     double g[NV];
-    cell_f64<S>(f, L.qs, int(i), rho0, g);
+    cell_f64<S>(f, L.ps, L.at_cell(0, i), rho0, g);
     acc[0] += mode_of<0>(g, std::make_integer_sequence<int, NV>{});
     acc[1] += mode_of<1>(g, std::make_integer_sequence<int, NV>{});
     acc[2] += mode_of<2>(g, std::make_integer_sequence<int, NV>{});
@@ -269,12 +280,34 @@ __global__ __launch_bounds__(256) void k_copy(const copy_vec* __restrict__ a, co
 
 }  // namespace
 
-LaunchShape default_shape(const Lattice& L) {
+int lds_bytes_for_cap(int cap) {
+  static int per_cu = -1, per_block = -1;
+  if (per_cu < 0) {
+    int dev = 0;
+    per_cu = 64 * 1024;
+    per_block = 64 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      (void)hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+      (void)hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    }
+  }
+  if (cap <= 0) return 0;
+  const int bytes = per_cu / cap - 512;
+  return (bytes > 0 && bytes <= per_block) ? bytes : -1;
+}
+
+LaunchShape default_shape(const Lattice& L, int precision) {
   LaunchShape s;
   s.bx = L.nx < 256 ? L.nx : 256;
   s.by = 256 / s.bx;
   if (s.by > L.ny) s.by = L.ny;
   if (s.by < 1) s.by = 1;
+  s.cap = precision == 0 ? 3 : 4;
+  s.lds_bytes = lds_bytes_for_cap(s.cap);
+  if (s.lds_bytes < 0) {
+    s.cap = 0;
+    s.lds_bytes = 0;
+  }
   return s;
 }
 
@@ -282,9 +315,9 @@ hipError_t launch_step_f64(StepKind kind, const double* src, double* dst, const 
                            const Relax& r, LaunchShape s, hipStream_t st) {
   const dim3 grid = grid_of(L, s), block(s.bx, s.by, 1);
   switch (kind) {
-    case STEP_FUSED: k_step_f64<STEP_FUSED><<<grid, block, 0, st>>>(src, dst, L, r); break;
-    case STEP_COLLIDE: k_step_f64<STEP_COLLIDE><<<grid, block, 0, st>>>(src, dst, L, r); break;
-    case STEP_STREAM: k_step_f64<STEP_STREAM><<<grid, block, 0, st>>>(src, dst, L, r); break;
+    case STEP_FUSED: k_step_f64<STEP_FUSED><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
+    case STEP_COLLIDE: k_step_f64<STEP_COLLIDE><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
+    case STEP_STREAM: k_step_f64<STEP_STREAM><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
   }
   return hipGetLastError();
 }
@@ -293,9 +326,9 @@ hipError_t launch_step_f32s(StepKind kind, const float* src, float* dst, const L
                             const Relax& r, LaunchShape s, hipStream_t st) {
   const dim3 grid = grid_of(L, s), block(s.bx, s.by, 1);
   switch (kind) {
-    case STEP_FUSED: k_step_f32s<STEP_FUSED><<<grid, block, 0, st>>>(src, dst, L, r); break;
-    case STEP_COLLIDE: k_step_f32s<STEP_COLLIDE><<<grid, block, 0, st>>>(src, dst, L, r); break;
-    case STEP_STREAM: k_step_f32s<STEP_STREAM><<<grid, block, 0, st>>>(src, dst, L, r); break;
+    case STEP_FUSED: k_step_f32s<STEP_FUSED><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
+    case STEP_COLLIDE: k_step_f32s<STEP_COLLIDE><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
+    case STEP_STREAM: k_step_f32s<STEP_STREAM><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
   }
   return hipGetLastError();
 }

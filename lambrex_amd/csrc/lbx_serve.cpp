@@ -97,7 +97,7 @@ struct Request {
 struct Run {
   Request r;
   size_t off = 0, bytes = 0;
-  long long qs = 0;
+  lbx::Lattice lat{};    // the run's level-array geometry
   long long start = 0;  // global step of pass 0
   double t_admit = 0.0;
   bool launched_last = false;
@@ -161,8 +161,8 @@ long long cells_of(const lbx_request& q) { return (long long)q.dims[0] * q.dims[
 int admit(lbx_server* s) {
   while (!s->queue.empty() && (int)s->active.size() < s->max_active) {
     Request& q = s->queue.front();
-    const long long qs = (cells_of(q.req) + 63) / 64 * 64;
-    const size_t bytes = (size_t)2 * lbx::NV_TOTAL * qs * esize(s->precision);
+    const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2]);
+    const size_t bytes = (size_t)2 * L.elems() * esize(s->precision);
     size_t off = 0;
     if (s->free_slots.empty() || !s->arena.alloc(bytes, &off)) break;  // FIFO admission
     Run r;
@@ -172,17 +172,16 @@ int admit(lbx_server* s) {
     s->queue.pop_front();
     r.off = off;
     r.bytes = bytes;
-    r.qs = qs;
+    r.lat = L;
     r.start = s->step;
     r.t_admit = now_s();
     char* b0 = s->arena.base + off;
     const lbx_request& rq = r.r.req;
     if (rq.init_kind == LBX_INIT_PLANE_PULSE) {
-      SRV_HIP(s, lbx::launch_serve_init_pulse(s->precision, b0, rq.dims, qs, rq.axis, rq.plane,
-                                              rq.amplitude, s->rho0, s->stream));
+      SRV_HIP(s, lbx::launch_serve_init_pulse(s->precision, b0, L, rq.axis, rq.plane, rq.amplitude,
+                                              s->rho0, s->stream));
     } else {
       const long long n = cells_of(rq);
-      lbx::Lattice L{rq.dims[0], rq.dims[1], rq.dims[2], qs};
       lbx::Region R{{0, 0, 0}, {rq.dims[0], rq.dims[1], rq.dims[2]}};
       // staging holds rho then u for the upload
       SRV_HIP(s, hipMemcpyAsync(s->staging, r.r.rho.data(), n * sizeof(double), hipMemcpyHostToDevice,
@@ -204,9 +203,8 @@ int admit(lbx_server* s) {
 // into the run's slot, copy to pinned host memory, record the slot event.
 int queue_finish(lbx_server* s, Run& r) {
   const lbx_request& rq = r.r.req;
-  lbx::Lattice L{rq.dims[0], rq.dims[1], rq.dims[2], r.qs};
-  char* fin = s->arena.base + r.off +
-              (size_t)((rq.steps + 1) & 1) * lbx::NV_TOTAL * r.qs * esize(s->precision);
+  const lbx::Lattice& L = r.lat;
+  char* fin = s->arena.base + r.off + (size_t)((rq.steps + 1) & 1) * L.elems() * esize(s->precision);
   double* out = s->sums + 1024 + 4 * r.slot;
   SRV_HIP(s, lbx::launch_conservation(s->precision, fin, L, s->rho0, s->sums, 256, out, s->stream));
   SRV_HIP(s, hipMemcpyAsync(s->pinned + 4 * r.slot, out, 4 * sizeof(double), hipMemcpyDeviceToHost,
@@ -234,10 +232,9 @@ int collect(lbx_server* s, bool wait) {
     if (q != hipSuccess) return sfail(s, LBX_E_HIP, "run completion: %s", hipGetErrorString(q));
     const lbx_request& rq = r.r.req;
     const long long n = cells_of(rq);
-    lbx::Lattice L{rq.dims[0], rq.dims[1], rq.dims[2], r.qs};
+    const lbx::Lattice& L = r.lat;
     lbx::Region R{{0, 0, 0}, {rq.dims[0], rq.dims[1], rq.dims[2]}};
-    char* fin = s->arena.base + r.off +
-                (size_t)((rq.steps + 1) & 1) * lbx::NV_TOTAL * r.qs * esize(s->precision);
+    char* fin = s->arena.base + r.off + (size_t)((rq.steps + 1) & 1) * L.elems() * esize(s->precision);
     lbx_result res{};
     res.id = r.r.id;
     res.status = LBX_OK;
@@ -279,13 +276,13 @@ int launch_step(lbx_server* s) {
     if (r.launched_last) continue;
     ServeRun& d = b.run[b.nruns++];
     d.buf[0] = s->arena.base + r.off;
-    d.buf[1] = s->arena.base + r.off + (size_t)lbx::NV_TOTAL * r.qs * esize(s->precision);
+    d.buf[1] = s->arena.base + r.off + (size_t)r.lat.elems() * esize(s->precision);
     for (int k = 0; k < 3; ++k) d.n[k] = r.r.req.dims[k];
     d.rows = 256 / d.n[0];
     d.item_begin = items;
     d.steps = r.r.req.steps;
     d.start = r.start;
-    d.qs = r.qs;
+    d.ps = r.lat.ps;
     d.omega_s = 1.0 / (r.r.req.tau_s + 0.5);
     d.omega_b = 1.0 / (r.r.req.tau_b + 0.5);
     d.rho0 = s->rho0;
@@ -384,8 +381,8 @@ int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
   const long long n = cells_of(*req);
   if ((size_t)n * lbx::NV_TOTAL * sizeof(double) > s->staging_bytes)
     return sfail(s, LBX_E_ARG, "run of %lld cells exceeds the serving staging size", n);
-  const long long qs = (n + 63) / 64 * 64;
-  if ((size_t)2 * lbx::NV_TOTAL * qs * esize(s->precision) > s->arena.size)
+  const lbx::Lattice L = lbx::make_lattice(req->dims[0], req->dims[1], req->dims[2]);
+  if ((size_t)2 * L.elems() * esize(s->precision) > s->arena.size)
     return sfail(s, LBX_E_NOMEM, "run does not fit the serving arena");
   if (req->init_kind != LBX_INIT_PLANE_PULSE && req->init_kind != LBX_INIT_FIELDS)
     return sfail(s, LBX_E_ARG, "unknown init kind %d", req->init_kind);

@@ -25,22 +25,26 @@ template <typename S>
 __device__ __forceinline__ void serve_cell(const ServeRun& R, int kind, const S* __restrict__ src,
                                            S* __restrict__ dst, int x, int y, int z) {
   const int nx = R.n[0], ny = R.n[1], nz = R.n[2];
-  const int self = (z * ny + y) * nx + x;
-  int xs[3], ys[3], zs[3];
+  const long long zst = NV * R.ps;
+  const long long selfz = z * zst;
+  const int selfp = y * nx + x;
+  int xs[3], ys[3];
+  long long zs[3];
   xs[0] = (x == nx - 1) ? 0 : x + 1;
   xs[1] = x;
   xs[2] = (x == 0) ? nx - 1 : x - 1;
   ys[0] = ((y == ny - 1) ? 0 : y + 1) * nx;
   ys[1] = y * nx;
   ys[2] = ((y == 0) ? ny - 1 : y - 1) * nx;
-  zs[0] = ((z == nz - 1) ? 0 : z + 1) * nx * ny;
-  zs[1] = z * nx * ny;
-  zs[2] = ((z == 0) ? nz - 1 : z - 1) * nx * ny;
+  zs[0] = ((z == nz - 1) ? 0 : z + 1) * zst;
+  zs[1] = selfz;
+  zs[2] = ((z == 0) ? nz - 1 : z - 1) * zst;
   S f[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    const int off = (kind == STEP_COLLIDE) ? self : (zs[1 + kC[q][2]] + ys[1 + kC[q][1]] + xs[1 + kC[q][0]]);
-    f[q] = src[q * R.qs + off];
+    const long long off =
+        (kind == STEP_COLLIDE) ? selfz + selfp : (zs[1 + kC[q][2]] + ys[1 + kC[q][1]] + xs[1 + kC[q][0]]);
+    f[q] = src[off + q * R.ps];
   }
   if (kind != STEP_STREAM) {
     if constexpr (sizeof(S) == 8)
@@ -49,7 +53,7 @@ __device__ __forceinline__ void serve_cell(const ServeRun& R, int kind, const S*
       collide_f32_shifted(f, float(R.rho0), float(R.omega_s), float(R.omega_b));
   }
 #pragma unroll
-  for (int q = 0; q < NV; ++q) __builtin_nontemporal_store(f[q], dst + q * R.qs + self);
+  for (int q = 0; q < NV; ++q) __builtin_nontemporal_store(f[q], dst + selfz + q * R.ps + selfp);
 }
 
 template <typename S>
@@ -80,13 +84,14 @@ __global__ __launch_bounds__(256) void k_serve_step(ServeBatch B, long long step
 // `axis`, 1 elsewhere, u = 0), the reference's operation order, into the
 // run's buffer 0.
 template <typename S>
-__global__ __launch_bounds__(256) void k_serve_init_pulse(S* __restrict__ f, int nx, int ny, int nz,
-                                                         long long qs, int axis, int plane,
+__global__ __launch_bounds__(256) void k_serve_init_pulse(S* __restrict__ f, Lattice L, int axis, int plane,
                                                          double amplitude, double rho0) {
-  const long long n = (long long)nx * ny * nz;
+  const int nx = L.nx, ny = L.ny;
+  const long long n = (long long)nx * ny * L.nz;
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / ((long long)nx * ny));
+  const long long self = L.at(0, y * nx + x, z);
   const int c = axis == 0 ? x : (axis == 1 ? y : z);
   const double rho = (c == plane) ? 1.0 + amplitude : 1.0;
   double fe[NV];
@@ -94,9 +99,9 @@ __global__ __launch_bounds__(256) void k_serve_init_pulse(S* __restrict__ f, int
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     if constexpr (sizeof(S) == 8)
-      f[q * qs + i] = fe[q];
+      f[self + q * L.ps] = fe[q];
     else
-      f[q * qs + i] = float(fe[q] - kT.W[q] * rho0);
+      f[self + q * L.ps] = float(fe[q] - kT.W[q] * rho0);
   }
 }
 
@@ -113,16 +118,14 @@ hipError_t launch_serve_step(int precision, const ServeBatch& b, long long step,
   return hipGetLastError();
 }
 
-hipError_t launch_serve_init_pulse(int precision, void* f, const int* n, long long qs, int axis,
-                                   int plane, double amplitude, double rho0, hipStream_t st) {
-  const long long cells = (long long)n[0] * n[1] * n[2];
+hipError_t launch_serve_init_pulse(int precision, void* f, const Lattice& L, int axis, int plane,
+                                   double amplitude, double rho0, hipStream_t st) {
+  const long long cells = (long long)L.nx * L.ny * L.nz;
   const int blocks = int((cells + 255) / 256);
   if (precision == 0)
-    k_serve_init_pulse<double><<<blocks, 256, 0, st>>>((double*)f, n[0], n[1], n[2], qs, axis, plane,
-                                                       amplitude, rho0);
+    k_serve_init_pulse<double><<<blocks, 256, 0, st>>>((double*)f, L, axis, plane, amplitude, rho0);
   else
-    k_serve_init_pulse<float><<<blocks, 256, 0, st>>>((float*)f, n[0], n[1], n[2], qs, axis, plane,
-                                                      amplitude, rho0);
+    k_serve_init_pulse<float><<<blocks, 256, 0, st>>>((float*)f, L, axis, plane, amplitude, rho0);
   return hipGetLastError();
 }
 

@@ -76,9 +76,9 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
 #pragma unroll
       for (int q = 0; q < NV; ++q) f[q] = fnow[S.base + q * S.qs + li];
     } else {
-      const long long c = -1 - d;
+      const long long c = -1 - d;  // coarse element of population 0
 #pragma unroll
-      for (int q = 0; q < NV; ++q) f[q] = cnow[q * clat.qs + c];
+      for (int q = 0; q < NV; ++q) f[q] = cnow[c + q * clat.ps];
     }
     if (patch_outer(P, gx, gy, gz)) f[0] = 0.0;
   }
@@ -138,11 +138,11 @@ __global__ __launch_bounds__(256) void k_coarse_push(const double* __restrict__ 
   const int t = i / nx;
   const int y = B.lo[1] + t % ny;
   const int z = B.lo[2] + t / ny;
-  const int self = (z * L.ny + y) * L.nx + x;
+  const long long self = L.at(0, y * L.nx + x, z);
   double pre[NV], post[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) pre[q] = post[q] = cnow[q * L.qs + self];
-  if (mask[self]) {
+  for (int q = 0; q < NV; ++q) pre[q] = post[q] = cnow[self + q * L.ps];
+  if (mask[((long long)z * L.ny + y) * L.nx + x]) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) post[q] = 0.0;
   } else {
@@ -152,8 +152,8 @@ __global__ __launch_bounds__(256) void k_coarse_push(const double* __restrict__ 
   for (int q = 0; q < NV; ++q) {
     const int dx = x + kC[q][0], dy = y + kC[q][1], dz = z + kC[q][2];
     const bool same_box = in_valid(B, dx, dy, dz);
-    const int dest = (wrap(dz, L.nz) * L.ny + wrap(dy, L.ny)) * L.nx + wrap(dx, L.nx);
-    cnext[q * L.qs + dest] = same_box ? post[q] : pre[q];
+    const long long dest = L.at(q, wrap(dy, L.ny) * L.nx + wrap(dx, L.nx), wrap(dz, L.nz));
+    cnext[dest] = same_box ? post[q] : pre[q];
   }
 }
 
@@ -161,14 +161,14 @@ __global__ __launch_bounds__(256) void k_coarse_push(const double* __restrict__ 
 // the order the reference's loops would add them.
 __global__ __launch_bounds__(256) void k_sum_fine_to_coarse(
     const double* __restrict__ fnow, double* __restrict__ cnext, Lattice L,
-    const PatchDesc* __restrict__ patches, const int* __restrict__ cell,
+    const PatchDesc* __restrict__ patches, const long long* __restrict__ cell,
     const int* __restrict__ start, const int4* __restrict__ contrib, int ncell) {
   const int u = blockIdx.x * 256 + threadIdx.x;
   if (u >= ncell) return;
-  const int c = cell[u];
+  const long long c = cell[u];  // coarse element of population 0
   double acc[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = cnext[q * L.qs + c];
+  for (int q = 0; q < NV; ++q) acc[q] = cnext[c + q * L.ps];
   for (int j = start[u]; j < start[u + 1]; ++j) {
     const int4 k = contrib[j];  // box, unwrapped coarse position
     const PatchDesc P = patches[k.x];
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_sum_fine_to_coarse(
     }
   }
 #pragma unroll
-  for (int q = 0; q < NV; ++q) cnext[q * L.qs + c] = acc[q];
+  for (int q = 0; q < NV; ++q) cnext[c + q * L.ps] = acc[q];
 }
 
 // DistFnFillFromCoarse (src/AmrSim.cpp:393-411): valid fine cells take the
@@ -206,9 +206,9 @@ __global__ __launch_bounds__(256) void k_fine_from_coarse(double* __restrict__ f
   patch_coords(P, i, gx, gy, gz);
   if (patch_valid(P, gx, gy, gz)) {
     const int x = (P.lo[0] - 2 + gx) >> 1, y = (P.lo[1] - 2 + gy) >> 1, z = (P.lo[2] - 2 + gz) >> 1;
-    const long long c = ((long long)z * clat.ny + y) * clat.nx + x;
+    const long long c = clat.at(0, y * clat.nx + x, z);
 #pragma unroll
-    for (int q = 0; q < NV; ++q) fnow[P.base + q * P.qs + i] = cnow[q * clat.qs + c];
+    for (int q = 0; q < NV; ++q) fnow[P.base + q * P.qs + i] = cnow[c + q * clat.ps];
   } else {
 #pragma unroll
     for (int q = 0; q < NV; ++q) fnow[P.base + q * P.qs + i] = 0.0;
@@ -274,7 +274,7 @@ hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& 
 }
 
 hipError_t launch_sum_fine_to_coarse(const double* fnow, double* cnext, const Lattice& L,
-                                     const PatchDesc* patches, const int* cell, const int* start,
+                                     const PatchDesc* patches, const long long* cell, const int* start,
                                      const int4* contrib, int ncell, hipStream_t st) {
   if (ncell > 0)
     k_sum_fine_to_coarse<<<(ncell + 255) / 256, 256, 0, st>>>(fnow, cnext, L, patches, cell, start,

@@ -48,7 +48,7 @@ SIGNATURES = {
     "lbx_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "lbx_bench_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
     "lbx_set_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
-    "lbx_tuned_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
+    "lbx_tuned_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p,
                                        ctypes.POINTER(ctypes.c_float)]),
 }
 
@@ -159,10 +159,13 @@ class Context:
         self.check(self._lib.lbx_set_autotune(self.h, 1 if on else 0))
 
     def tuned_shape(self, nx, ny, precision=LBX_FP64):
-        bx, by, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
+        """((bx, by, workgroups_per_cu_cap), ms per proxy launch) that launches over
+        rows of nx x ny use: the autotuner's choice, or the default shape (ms 0)
+        if that geometry was never tuned."""
+        bx, by, occ, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_float()
         self.check(self._lib.lbx_tuned_shape(self.h, int(nx), int(ny), int(precision), ctypes.byref(bx),
-                                             ctypes.byref(by), ctypes.byref(ms)))
-        return (bx.value, by.value), ms.value
+                                             ctypes.byref(by), ctypes.byref(occ), ctypes.byref(ms)))
+        return (bx.value, by.value, occ.value), ms.value
 
     def close(self):
         if self.h:
