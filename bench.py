@@ -213,7 +213,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--serve-only", action="store_true")
-    ap.add_argument("--two-level-steps", type=int, default=100,
+    ap.add_argument("--two-level-steps", type=int, default=500,
                     help="coarse steps of the two-level measurement (0 skips it)")
     args = ap.parse_args()
 

@@ -136,8 +136,8 @@ int lbx_profile_enable(lbx_ctx* ctx, int enable);
 int lbx_profile_read(lbx_ctx* ctx, double* ms, int64_t* launches);
 
 /* Autotuning of the stream-collide kernel's launch shape: workgroup extent
- * (bx, by) and the cap on resident workgroups per CU (0 = none, else 2, 3,
- * 4 or 6; enforced by reserving LDS the kernel does not use).
+ * (bx, by; 256 or 512 lanes) and the cap on resident workgroups per CU
+ * (0 = none, else 1 to 6; enforced by reserving LDS the kernel does not use).
  * While enabled, the first launch for a new (nx, ny, precision) times every
  * candidate on a small proxy lattice with the same rows and keeps the
  * fastest for the context's lifetime; shapes never change results.  No

@@ -60,8 +60,11 @@ struct Level {
   long long* d_fill = nullptr;
   long long* d_sum_cell = nullptr;  // coarse element of population 0 per touched cell
   int* d_sum_start = nullptr;
-  int4* d_sum_contrib = nullptr;
+  int* d_sum_contrib = nullptr;  // index into the block averages, per contribution
   int sum_ncell = 0;
+  double* d_avg = nullptr;       // 15 planes of avg_stride 2x2x2 block averages
+  long long avg_stride = 0;
+  int max_blocks = 0;            // largest (g/2)^3 of a patch
 };
 
 // One captured two-level coarse step per (coarse parity, fine parity).
@@ -188,6 +191,7 @@ void free_level(Level& L) {
   if (L.d_sum_cell) (void)hipFree(L.d_sum_cell);
   if (L.d_sum_start) (void)hipFree(L.d_sum_start);
   if (L.d_sum_contrib) (void)hipFree(L.d_sum_contrib);
+  if (L.d_avg) (void)hipFree(L.d_avg);
   L = Level{};
 }
 
@@ -268,15 +272,19 @@ struct ProfileScope {
 // on the choice.
 std::vector<LaunchShape> shape_candidates(const Lattice& L) {
   std::vector<LaunchShape> c;
-  for (int cap : {0, 2, 3, 4, 6}) {
-    const int lds = lbx::lds_bytes_for_cap(cap);
-    if (lds < 0) continue;
-    for (int bx : {256, 128, 64, 32}) {
-      const int x = std::min(bx, L.nx);
-      LaunchShape s{x, std::max(1, std::min(256 / x, L.ny)), cap, lds};
-      bool seen = false;
-      for (const LaunchShape& o : c) seen |= (o.bx == s.bx && o.by == s.by && o.cap == s.cap);
-      if (!seen) c.push_back(s);
+  for (int lanes : {256, 512}) {
+    for (int cap : {0, 1, 2, 3, 4, 6}) {
+      if (cap == 1 && lanes == 256) continue;  // 4 wavefronts per CU
+      if (cap > 3 && lanes == 512) continue;
+      const int lds = lbx::lds_bytes_for_cap(cap);
+      if (lds < 0) continue;
+      for (int bx : {256, 128, 64, 32}) {
+        const int x = std::min(bx, L.nx);
+        LaunchShape s{x, std::max(1, std::min(lanes / x, L.ny)), cap, lds};
+        bool seen = false;
+        for (const LaunchShape& o : c) seen |= (o.bx == s.bx && o.by == s.by && o.cap == s.cap);
+        if (!seen) c.push_back(s);
+      }
     }
   }
   return c;
@@ -381,15 +389,16 @@ hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   const int np = (int)F.boxes.size();
   hipError_t e = lbx::launch_fine_fill_collide(fnow, fnext, cnow, C.lat, F.d_desc, np, F.max_cells,
                                                F.d_fill, rf, st);
-  if (e == hipSuccess) e = lbx::launch_fine_stream(0, fnext, ftmp, F.d_desc, np, F.max_cells, rf, st);
-  if (e == hipSuccess) e = lbx::launch_fine_stream(1, ftmp, fnext, F.d_desc, np, F.max_cells, rf, st);
+  if (e == hipSuccess) e = lbx::launch_fine_stream_collide(fnext, ftmp, F.d_desc, np, F.max_cells, rf, st);
+  if (e == hipSuccess)
+    e = lbx::launch_fine_stream_avg(ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, np, F.max_blocks, st);
   if (e == hipSuccess)
     e = lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_desc, (int)C.boxes.size(), C.max_cells, C.d_mask,
                                 rc, st);
   // the fine state after the step is fnext
   if (e == hipSuccess)
-    e = lbx::launch_sum_fine_to_coarse(fnext, cnext, C.lat, F.d_desc, F.d_sum_cell, F.d_sum_start,
-                                       F.d_sum_contrib, F.sum_ncell, st);
+    e = lbx::launch_sum_apply(F.d_avg, F.avg_stride, cnext, C.lat, F.d_sum_cell, F.d_sum_start,
+                              F.d_sum_contrib, F.sum_ncell, st);
   return e;
 }
 
@@ -505,7 +514,7 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
   free_level(F);
   F.boxes = boxes;
   F.patched = true;
-  long long base = 0, fill_base = 0;
+  long long base = 0, fill_base = 0, avg_base = 0;
   for (const Box& b : boxes) {
     PatchDesc d{};
     for (int k = 0; k < 3; ++k) {
@@ -519,12 +528,27 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
     d.qs = (cells + 63) / 64 * 64;
     d.base = base;
     d.fill_base = fill_base;
+    d.avg_base = avg_base;
+    const int blocks = (d.g[0] / 2) * (d.g[1] / 2) * (d.g[2] / 2);
     base += d.qs * lbx::NV_TOTAL;
     fill_base += cells;
+    avg_base += blocks;
     F.max_cells = std::max(F.max_cells, d.cells);
+    F.max_blocks = std::max(F.max_blocks, blocks);
     F.desc.push_back(d);
   }
+  if (avg_base >= (1ll << 31)) return fail(ctx, LBX_E_ARG, "fine level too large");
+  F.avg_stride = (avg_base + 63) / 64 * 64;
   const size_t bytes = (size_t)base * sizeof(double);
+  {
+    const size_t abytes = (size_t)F.avg_stride * lbx::NV_TOTAL * sizeof(double);
+    hipError_t e = hipMalloc(&F.d_avg, abytes);
+    if (e != hipSuccess) {
+      free_level(F);
+      return fail(ctx, LBX_E_NOMEM, "allocation of %zu bytes for level 1 failed: %s", abytes,
+                  hipGetErrorString(e));
+    }
+  }
   for (int i = 0; i < 3; ++i) {
     hipError_t e = hipMalloc(&F.buf[i], bytes);
     if (e == hipSuccess) e = hipMemsetAsync(F.buf[i], 0, bytes, ctx->stream);
@@ -574,30 +598,36 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
   // each fine box's coarsened box grown by one coarse cell, periodic image
   // in the coarse level; per coarse cell, contributions in box order and,
   // within a box, in (i, j, k) loop order.
+  // A contribution is the average of one 2x2x2 block of a grown patch,
+  // stored by k_fine_stream_avg at avg_base + block index.
   struct Contrib {
     long long cell;  // coarse element of population 0
-    int box, ci, cj, ck;
+    int avg;         // block-average index
   };
   std::vector<Contrib> list;
   for (size_t b = 0; b < boxes.size(); ++b) {
     const Box& B = boxes[b];
-    for (int ci = floor_div2(B.lo[0]) - 1; ci <= floor_div2(B.hi[0]) + 1; ++ci)
-      for (int cj = floor_div2(B.lo[1]) - 1; cj <= floor_div2(B.hi[1]) + 1; ++cj)
-        for (int ck = floor_div2(B.lo[2]) - 1; ck <= floor_div2(B.hi[2]) + 1; ++ck) {
+    const PatchDesc& P = F.desc[b];
+    const int c0 = P.g[0] / 2, c1 = P.g[1] / 2;
+    const int i0 = floor_div2(B.lo[0]) - 1, j0 = floor_div2(B.lo[1]) - 1, k0 = floor_div2(B.lo[2]) - 1;
+    for (int ci = i0; ci <= floor_div2(B.hi[0]) + 1; ++ci)
+      for (int cj = j0; cj <= floor_div2(B.hi[1]) + 1; ++cj)
+        for (int ck = k0; ck <= floor_div2(B.hi[2]) + 1; ++ck) {
           const long long c = C.lat.at(0, wrap(cj, C.lat.ny) * C.lat.nx + wrap(ci, C.lat.nx), wrap(ck, C.lat.nz));
-          list.push_back({c, (int)b, ci, cj, ck});
+          const long long blk = ((long long)(ck - k0) * c1 + (cj - j0)) * c0 + (ci - i0);
+          list.push_back({c, (int)(P.avg_base + blk)});
         }
   }
   std::stable_sort(list.begin(), list.end(), [](const Contrib& a, const Contrib& b) { return a.cell < b.cell; });
   std::vector<long long> cell;
   std::vector<int> start;
-  std::vector<int4> contrib;
+  std::vector<int> contrib;
   for (size_t j = 0; j < list.size(); ++j) {
     if (j == 0 || list[j].cell != list[j - 1].cell) {
       cell.push_back(list[j].cell);
       start.push_back((int)j);
     }
-    contrib.push_back(make_int4(list[j].box, list[j].ci, list[j].cj, list[j].ck));
+    contrib.push_back(list[j].avg);
   }
   start.push_back((int)list.size());
   F.sum_ncell = (int)cell.size();

