@@ -149,7 +149,7 @@ def serve(ctx, trace, max_active):
             "steps_launched": st["steps_launched"]}
 
 
-def two_level(ctx, steps, warmup):
+def two_level_geometry():
     """Two-level configuration: coarse 128^3 periodic (boxes of 32^3), fine
     level over coarse x in [48, 80) and all y, z (fine 64x256x256 in boxes of
     32^3), tau_coarse 0.8 (fine tau 1.1, src/AmrSim.cpp:313-314), plane pulse
@@ -161,6 +161,14 @@ def two_level(ctx, steps, warmup):
     fboxes = [((96 + a[0], a[1], a[2]), (96 + b[0], b[1], b[2])) for a, b in workloads.chop(ext, 32)]
     mask = [((2 * a[0], 2 * a[1], 2 * a[2]), (2 * b[0] + 1, 2 * b[1] + 1, 2 * b[2] + 1)) for a, b in cboxes]
     rho, u = workloads.plane_pulse(dims, 0.01, axis=0, plane=64)
+    lu_per_step = 128 ** 3 + 2 * 64 * 256 * 256
+    return dims, cboxes, fboxes, mask, rho, u, lu_per_step
+
+
+def two_level(ctx, steps, warmup):
+    """The two-level configuration on the GPU (one coarse step = one replay
+    of the captured HIP graph)."""
+    dims, cboxes, fboxes, mask, rho, u, lu_per_step = two_level_geometry()
     sim = lbx.Sim(ctx, dims)
     try:
         sim.define_level(0, cboxes)
@@ -178,11 +186,27 @@ def two_level(ctx, steps, warmup):
         t1 = time.perf_counter()
     finally:
         sim.close()
-    lu = (128 ** 3 + 2 * 64 * 256 * 256) * steps
+    lu = lu_per_step * steps
     return {"workload": "coarse 128^3 (64 boxes of 32^3) + fine 64x256x256 (128 boxes of 32^3), "
                         "ratio 2, 2 fine substeps per coarse step, FP64",
             "coarse_steps": steps, "mlups": lu / (t1 - t0) / 1e6, "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
             "graph": "one coarse step captured as a HIP graph"}
+
+
+def cpu_two_level(steps):
+    """The oracle's two-level Rohde cycle (serial C, per-FAB ghosted storage
+    like the reference) on the same two-level configuration."""
+    from oracle import oracle as orc
+    dims, cboxes, fboxes, mask, rho, u, lu_per_step = two_level_geometry()
+    s = orc.Sim(*dims, 0.8, boxes=[(list(a), list(b)) for a, b in cboxes])
+    s.init(np.ascontiguousarray(rho.transpose(2, 1, 0)).reshape(-1),
+           np.ascontiguousarray(u.transpose(3, 2, 1, 0)).reshape(-1))
+    s.add_fine([(list(a), list(b)) for a, b in fboxes], [(list(a), list(b)) for a, b in mask])
+    t0 = time.perf_counter()
+    s.iterate(steps)
+    t1 = time.perf_counter()
+    s.close()
+    return lu_per_step * steps / (t1 - t0) / 1e6, t1 - t0
 
 
 def cpu_model():
@@ -210,6 +234,8 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the copy kernel (counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-two-level-steps", type=int, default=1,
+                    help="coarse steps of the two-level CPU baseline (0 skips it)")
     ap.add_argument("--serve-requests", type=int, default=512,
                     help="requests of the serving measurement (0 skips it)")
     ap.add_argument("--serve-only", action="store_true")
@@ -351,6 +377,14 @@ def main():
                       f"({secs:.1f} s), serial C restatement of the reference; host CPU {cpu_model()}, "
                       f"{os.cpu_count()} logical cores present",
         }
+        if twolev and args.cpu_two_level_steps:
+            log(f"[bench] CPU baseline, two-level: oracle, 1 thread, {args.cpu_two_level_steps} coarse step(s)")
+            mlups2, secs2 = cpu_two_level(args.cpu_two_level_steps)
+            twolev["cpu_baseline"] = {
+                "value": round(mlups2, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+                "sample": f"{args.cpu_two_level_steps} coarse step(s) of the same two-level workload "
+                          f"({secs2:.1f} s), serial C restatement of the reference's Rohde cycle",
+            }
     print(json.dumps(out), flush=True)
 
 

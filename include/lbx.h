@@ -158,7 +158,8 @@ int lbx_bench_copy(lbx_ctx* ctx, size_t bytes, int reps, double* gbs);
 /* ---------------------------------------------------------------------
  * Batched serving of independent runs.  No reference counterpart: the
  * reference runs one AmrSim at a time.  Each request is a periodic
- * single-level run (x extent <= 256, at most 128^3 cells) with its own tau,
+ * single-level run (its FP64 populations must fit the server's 256 MB
+ * staging buffer: at most 2,236,962 cells; 128^3 fits) with its own tau,
  * initial state and step count.  Requests are admitted in arrival order
  * while the server's device arena has room; all admitted runs advance
  * together, one persistent-kernel launch per step over the patch list of

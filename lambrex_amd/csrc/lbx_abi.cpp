@@ -54,8 +54,10 @@ struct Level {
   std::vector<PatchDesc> desc;
   PatchDesc* d_desc = nullptr;
   int max_cells = 0;
-  // level 0 with a fine level: coarse cells masked in the coarse collide
+  // level 0 with a fine level: per-cell box-face and fine-mask flags of the
+  // coarse push (launch_coarse_push), and its launch shape
   unsigned char* d_mask = nullptr;
+  LaunchShape push_shape{};
   // level 1: ghost fill sources and the average-down contributions
   long long* d_fill = nullptr;
   long long* d_sum_cell = nullptr;  // coarse element of population 0 per touched cell
@@ -393,8 +395,7 @@ hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   if (e == hipSuccess)
     e = lbx::launch_fine_stream_avg(ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, np, F.max_blocks, st);
   if (e == hipSuccess)
-    e = lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_desc, (int)C.boxes.size(), C.max_cells, C.d_mask,
-                                rc, st);
+    e = lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
   // the fine state after the step is fnext
   if (e == hipSuccess)
     e = lbx::launch_sum_apply(F.d_avg, F.avg_stride, cnext, C.lat, F.d_sum_cell, F.d_sum_start,
@@ -492,6 +493,29 @@ int define_level0(lbx_sim* sim, const std::vector<Box>& boxes) {
   }
   L.defined = true;
   return LBX_OK;
+}
+
+// Per coarse cell (x fastest): bits 0-5 = moving by +x, -x, +y, -y, +z, -z
+// leaves the cell's level-0 box (the coarse push then streams the
+// pre-collision value, as the reference's pull from stale ghost copies
+// does); bit 7 = the cell lies under one of the mask boxes (fine index
+// space) and is zeroed instead of collided.
+std::vector<unsigned char> coarse_flags(const Level& C, const std::vector<Box>& mask) {
+  const Lattice& L = C.lat;
+  std::vector<unsigned char> fl((size_t)L.nx * L.ny * L.nz, 0);
+  auto at = [&](int x, int y, int z) -> unsigned char& { return fl[((size_t)z * L.ny + y) * L.nx + x]; };
+  for (const Box& b : C.boxes)
+    for (int z = b.lo[2]; z <= b.hi[2]; ++z)
+      for (int y = b.lo[1]; y <= b.hi[1]; ++y)
+        for (int x = b.lo[0]; x <= b.hi[0]; ++x)
+          at(x, y, z) = (unsigned char)((x == b.hi[0] ? 1 : 0) | (x == b.lo[0] ? 2 : 0) | (y == b.hi[1] ? 4 : 0) |
+                                        (y == b.lo[1] ? 8 : 0) | (z == b.hi[2] ? 16 : 0) | (z == b.lo[2] ? 32 : 0));
+  for (const Box& b : mask)
+    for (int z = std::max(0, floor_div2(b.lo[2])); z <= std::min(L.nz - 1, floor_div2(b.hi[2])); ++z)
+      for (int y = std::max(0, floor_div2(b.lo[1])); y <= std::min(L.ny - 1, floor_div2(b.hi[1])); ++y)
+        for (int x = std::max(0, floor_div2(b.lo[0])); x <= std::min(L.nx - 1, floor_div2(b.hi[0])); ++x)
+          at(x, y, z) |= 0x80;
+  return fl;
 }
 
 // Fine level: patches grown by 2; ghost-fill sources; average-down table.
@@ -637,10 +661,10 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
   if (e == hipSuccess) e = upload(&F.d_sum_start, start, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_contrib, contrib, ctx->stream);
   if (e == hipSuccess && !C.d_mask) {
-    const size_t ncell = (size_t)C.lat.nx * C.lat.ny * C.lat.nz;
-    e = hipMalloc(&C.d_mask, ncell);
-    if (e == hipSuccess) e = hipMemsetAsync(C.d_mask, 0, ncell, ctx->stream);
+    const std::vector<unsigned char> flags = coarse_flags(C, {});
+    e = upload(&C.d_mask, flags, ctx->stream);
   }
+  C.push_shape = lbx::default_shape(C.lat, LBX_FP64);
   if (e != hipSuccess) {
     free_level(F);
     return hip_fail(ctx, e, "level-1 tables");
@@ -781,16 +805,10 @@ int lbx_set_fine_mask(lbx_sim* sim, int n_boxes, const int32_t* lohi_fine) {
   if (!C.defined) return fail(ctx, LBX_E_STATE, "level 0 is not defined");
   if (n_boxes < 0 || (n_boxes > 0 && !lohi_fine)) return fail(ctx, LBX_E_ARG, "bad mask boxes");
   const size_t ncell = (size_t)C.lat.nx * C.lat.ny * C.lat.nz;
-  std::vector<unsigned char> mask(ncell, 0);
-  const std::vector<Box> boxes = read_boxes(n_boxes, lohi_fine);
-  for (const Box& b : boxes)
-    for (int z = std::max(0, floor_div2(b.lo[2])); z <= std::min(C.lat.nz - 1, floor_div2(b.hi[2])); ++z)
-      for (int y = std::max(0, floor_div2(b.lo[1])); y <= std::min(C.lat.ny - 1, floor_div2(b.hi[1])); ++y)
-        for (int x = std::max(0, floor_div2(b.lo[0])); x <= std::min(C.lat.nx - 1, floor_div2(b.hi[0])); ++x)
-          mask[((size_t)z * C.lat.ny + y) * C.lat.nx + x] = 1;
+  const std::vector<unsigned char> flags = coarse_flags(C, read_boxes(n_boxes, lohi_fine));
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   if (!C.d_mask) LBX_HIP(ctx, hipMalloc(&C.d_mask, ncell));
-  LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, mask.data(), ncell, hipMemcpyHostToDevice, ctx->stream));
+  LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, flags.data(), ncell, hipMemcpyHostToDevice, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }

@@ -125,9 +125,10 @@ hipError_t launch_fine_stream_collide(const double* in, double* out, const Patch
 // avg holds 15 planes of avg_stride block averages; max_blocks = max (g/2)^3.
 hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, long long avg_stride,
                                   const PatchDesc* patches, int npatch, int max_blocks, hipStream_t st);
-hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L,
-                              const PatchDesc* boxes, int nbox, int max_cells,
-                              const unsigned char* mask, const Relax& r, hipStream_t st);
+// flags: per coarse cell (x fastest), bits 0-5 = x+1, x-1, y+1, y-1, z+1,
+// z-1 leave the cell's box; bit 7 = masked by the fine level.
+hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L, const unsigned char* flags,
+                              const Relax& r, LaunchShape shape, hipStream_t st);
 hipError_t launch_sum_apply(const double* avg, long long avg_stride, double* cnext, const Lattice& L,
                             const long long* cell, const int* start, const int* contrib, int ncell,
                             hipStream_t st);
@@ -142,9 +143,8 @@ hipError_t launch_patch_io(int op, double* f, const PatchDesc* patches, int b, i
 // arguments (kServeMaxRuns * sizeof(ServeRun) stays under 4 KB).
 struct ServeRun {
   void* buf[2];      // state buffers (pass p reads buf[p & 1])
-  int n[3];          // lattice extents (n[0] <= 256)
-  int rows;          // x-rows per work item (256 / n[0])
-  int item_begin;    // first work item of this run in the batch
+  int n[3];          // lattice extents
+  int item_begin;    // first work item (256 cells, x fastest) of this run in the batch
   int steps;         // LB steps of the run (passes 0..steps)
   long long start;   // global step of pass 0
   long long ps;      // population-plane stride of the run's lattice (zs = 15 ps)

@@ -278,7 +278,6 @@ int launch_step(lbx_server* s) {
     d.buf[0] = s->arena.base + r.off;
     d.buf[1] = s->arena.base + r.off + (size_t)r.lat.elems() * esize(s->precision);
     for (int k = 0; k < 3; ++k) d.n[k] = r.r.req.dims[k];
-    d.rows = 256 / d.n[0];
     d.item_begin = items;
     d.steps = r.r.req.steps;
     d.start = r.start;
@@ -286,7 +285,7 @@ int launch_step(lbx_server* s) {
     d.omega_s = 1.0 / (r.r.req.tau_s + 0.5);
     d.omega_b = 1.0 / (r.r.req.tau_b + 0.5);
     d.rho0 = s->rho0;
-    items += d.n[2] * ((d.n[1] + d.rows - 1) / d.rows);
+    items += (int)((r.lat.nx * (long long)r.lat.ny * r.lat.nz + 255) / 256);
     if (s->step - r.start == r.r.req.steps) {
       r.launched_last = true;
       r.last_step = s->step;
@@ -327,7 +326,9 @@ int lbx_serve_create(lbx_ctx* ctx, int precision, double rho0, size_t arena_byte
   hipError_t e = hipSetDevice(s->device);
   int cus = 256;
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device);
-  s->grid = 8 * cus;
+  // persistent grid: as many resident 256-lane workgroups per CU as the
+  // stream-collide launch caps itself to (FP64 3, FP32 4)
+  s->grid = (precision == LBX_FP64 ? 3 : 4) * cus;
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e == hipSuccess && arena_bytes == 0) {
     size_t fr = 0, tot = 0;
@@ -376,7 +377,6 @@ int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
   if (!s || !req) return LBX_E_ARG;
   for (int d = 0; d < 3; ++d)
     if (req->dims[d] < 1) return sfail(s, LBX_E_ARG, "dims must be positive");
-  if (req->dims[0] > 256) return sfail(s, LBX_E_ARG, "run edge in x must be <= 256");
   if (req->steps < 1) return sfail(s, LBX_E_ARG, "steps must be >= 1");
   const long long n = cells_of(*req);
   if ((size_t)n * lbx::NV_TOTAL * sizeof(double) > s->staging_bytes)

@@ -3,8 +3,9 @@
 // Every admitted run is an independent periodic lattice (nx, ny, nz) in the
 // serving arena.  One launch advances every active run by one pass: a
 // persistent grid walks the patch list (the work items of all runs: one item
-// = one z-plane slab of `rows` x-rows of one run) with a static stride, so
-// every block ends after a bounded number of items.  Each run's pass kind
+// = 256 consecutive cells, x fastest, of one run, so every lane has a cell
+// whatever the run's shape) with a static stride, so every block ends after
+// a bounded number of items.  Each run's pass kind
 // follows from the global step: pass p = step - start is collide-only for
 // p = 0, fused pull + collide for 0 < p < steps, stream-only for p = steps
 // (the same S (C S)^(n-1) C association as lbx_iterate), reading buffer
@@ -44,7 +45,7 @@ __device__ __forceinline__ void serve_cell(const ServeRun& R, int kind, const S*
   for (int q = 0; q < NV; ++q) {
     const long long off =
         (kind == STEP_COLLIDE) ? selfz + selfp : (zs[1 + kC[q][2]] + ys[1 + kC[q][1]] + xs[1 + kC[q][0]]);
-    f[q] = src[off + q * R.ps];
+    f[q] = __builtin_nontemporal_load(src + off + q * R.ps);
   }
   if (kind != STEP_STREAM) {
     if constexpr (sizeof(S) == 8)
@@ -71,12 +72,13 @@ __global__ __launch_bounds__(256) void k_serve_step(ServeBatch B, long long step
     const int kind = (p == 0) ? STEP_COLLIDE : (p == R.steps ? STEP_STREAM : STEP_FUSED);
     const S* src = (const S*)R.buf[p & 1];
     S* dst = (S*)R.buf[(p + 1) & 1];
-    const int local = item - R.item_begin;
-    const int yblocks = (R.n[1] + R.rows - 1) / R.rows;
-    const int z = local / yblocks;
-    const int y = (local % yblocks) * R.rows + threadIdx.x / R.n[0];
-    const int x = threadIdx.x % R.n[0];
-    if (threadIdx.x < R.rows * R.n[0] && y < R.n[1]) serve_cell<S>(R, kind, src, dst, x, y, z);
+    const int plane = R.n[0] * R.n[1];
+    const long long c = (long long)(item - R.item_begin) * 256 + threadIdx.x;
+    if (c < (long long)plane * R.n[2]) {
+      const int z = int(c / plane), r = int(c - (long long)z * plane);
+      const int y = r / R.n[0], x = r - y * R.n[0];
+      serve_cell<S>(R, kind, src, dst, x, y, z);
+    }
   }
 }
 

@@ -55,6 +55,10 @@ __device__ __forceinline__ bool in_valid(const PatchDesc& P, int x, int y, int z
   return x >= P.lo[0] && x <= P.hi[0] && y >= P.lo[1] && y <= P.hi[1] && z >= P.lo[2] && z <= P.hi[2];
 }
 
+// Every lane first picks where its 15 populations come from (its own valid
+// cell, the fine valid cell holding its periodic image, or the coarse
+// parent) so the loads of valid and ghost lanes issue together; valid cells
+// are then collided.
 __global__ __launch_bounds__(256) void k_fine_fill_collide(
     const double* __restrict__ fnow, double* __restrict__ fnext, const double* __restrict__ cnow,
     Lattice clat, const PatchDesc* __restrict__ patches, const long long* __restrict__ fill,
@@ -64,25 +68,30 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
-  double f[NV];
-  if (patch_valid(P, gx, gy, gz)) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) f[q] = fnow[P.base + q * P.qs + i];
-    collide_f64(f, r.omega_s, r.omega_b);
+  const bool valid = patch_valid(P, gx, gy, gz);
+  const double* src;
+  long long stride;
+  if (valid) {
+    src = fnow + P.base + i;
+    stride = P.qs;
   } else {
     const long long d = fill[P.fill_base + i];
     if (d >= 0) {
-      const PatchDesc S = patches[d >> 32];
-      const int li = int(d & 0xffffffffll);
-#pragma unroll
-      for (int q = 0; q < NV; ++q) f[q] = fnow[S.base + q * S.qs + li];
+      const int s = int(d >> 32);
+      src = fnow + patches[s].base + (d & 0xffffffffll);
+      stride = patches[s].qs;
     } else {
-      const long long c = -1 - d;  // coarse element of population 0
-#pragma unroll
-      for (int q = 0; q < NV; ++q) f[q] = cnow[c + q * clat.ps];
+      src = cnow + (-1 - d);  // coarse element of population 0
+      stride = clat.ps;
     }
-    if (patch_outer(P, gx, gy, gz)) f[0] = 0.0;
   }
+  double f[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) f[q] = src[q * stride];
+  if (valid)
+    collide_f64(f, r.omega_s, r.omega_b);
+  else if (patch_outer(P, gx, gy, gz))
+    f[0] = 0.0;
 #pragma unroll
   for (int q = 0; q < NV; ++q) fnext[P.base + q * P.qs + i] = f[q];
 }
@@ -170,38 +179,43 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const double* __restric
   }
 }
 
-__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+// Coarse cell flags: bit 0/1 = the cell is on its box's high/low x face
+// (x+1 / x-1 leaves the box), bits 2/3 the same in y, 4/5 in z; bit 7 = the
+// cell lies under the fine mask.
+constexpr unsigned kMasked = 0x80u;
 
-// One workgroup row per coarse box: blockIdx.y = box.
-__global__ __launch_bounds__(256) void k_coarse_push(const double* __restrict__ cnow,
+__device__ constexpr unsigned leave_bits(int q) {
+  return (kC[q][0] > 0 ? 1u : 0u) | (kC[q][0] < 0 ? 2u : 0u) | (kC[q][1] > 0 ? 4u : 0u) |
+         (kC[q][1] < 0 ? 8u : 0u) | (kC[q][2] > 0 ? 16u : 0u) | (kC[q][2] < 0 ? 32u : 0u);
+}
+
+// Over the whole coarse level array, shaped like the stream-collide launch.
+__global__ __launch_bounds__(512) void k_coarse_push(const double* __restrict__ cnow,
                                                      double* __restrict__ cnext, Lattice L,
-                                                     const PatchDesc* __restrict__ boxes,
-                                                     const unsigned char* __restrict__ mask,
+                                                     const unsigned char* __restrict__ flags,
                                                      Relax r) {
-  const PatchDesc B = boxes[blockIdx.y];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= B.cells) return;
-  const int nx = B.hi[0] - B.lo[0] + 1, ny = B.hi[1] - B.lo[1] + 1;
-  const int x = B.lo[0] + i % nx;
-  const int t = i / nx;
-  const int y = B.lo[1] + t % ny;
-  const int z = B.lo[2] + t / ny;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= L.nx || y >= L.ny) return;
   const long long self = L.at(0, y * L.nx + x, z);
+  const unsigned fl = flags[((long long)z * L.ny + y) * L.nx + x];
   double pre[NV], post[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) pre[q] = post[q] = cnow[self + q * L.ps];
-  if (mask[((long long)z * L.ny + y) * L.nx + x]) {
+  for (int q = 0; q < NV; ++q) pre[q] = post[q] = __builtin_nontemporal_load(cnow + self + q * L.ps);
+  if (fl & kMasked) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) post[q] = 0.0;
   } else {
     collide_f64(post, r.omega_s, r.omega_b);
   }
+  const int xs[3] = {(x == 0) ? L.nx - 1 : x - 1, x, (x == L.nx - 1) ? 0 : x + 1};
+  const int ys[3] = {((y == 0) ? L.ny - 1 : y - 1) * L.nx, y * L.nx, ((y == L.ny - 1) ? 0 : y + 1) * L.nx};
+  const int zs[3] = {(z == 0) ? L.nz - 1 : z - 1, z, (z == L.nz - 1) ? 0 : z + 1};
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    const int dx = x + kC[q][0], dy = y + kC[q][1], dz = z + kC[q][2];
-    const bool same_box = in_valid(B, dx, dy, dz);
-    const long long dest = L.at(q, wrap(dy, L.ny) * L.nx + wrap(dx, L.nx), wrap(dz, L.nz));
-    cnext[dest] = same_box ? post[q] : pre[q];
+    const long long dest = L.at(q, ys[1 + kC[q][1]] + xs[1 + kC[q][0]], zs[1 + kC[q][2]]);
+    __builtin_nontemporal_store((fl & leave_bits(q)) ? pre[q] : post[q], cnext + dest);
   }
 }
 
@@ -305,10 +319,10 @@ hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, lo
   return hipGetLastError();
 }
 
-hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L,
-                              const PatchDesc* boxes, int nbox, int max_cells,
-                              const unsigned char* mask, const Relax& r, hipStream_t st) {
-  k_coarse_push<<<dim3((max_cells + 255) / 256, nbox), 256, 0, st>>>(cnow, cnext, L, boxes, mask, r);
+hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L, const unsigned char* flags,
+                              const Relax& r, LaunchShape sh, hipStream_t st) {
+  const dim3 grid((L.nx + sh.bx - 1) / sh.bx, (L.ny + sh.by - 1) / sh.by, L.nz);
+  k_coarse_push<<<grid, dim3(sh.bx, sh.by, 1), sh.lds_bytes, st>>>(cnow, cnext, L, flags, r);
   return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@ MIX = [  # dims, steps, tau, amplitude
     ((64, 64, 64), 5, 0.9, 5e-3),
     ((20, 16, 12), 1, 1.5, 2e-3),
     ((96, 8, 4), 12, 0.6, 1e-3),
+    ((300, 5, 3), 7, 0.7, 1e-3),  # rows longer than a work item; last item partial
 ]
 
 
@@ -97,7 +98,7 @@ def test_admission_respects_the_arena(ctx):
 def test_bad_requests_are_refused(ctx):
     srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=16 << 20)
     with pytest.raises(lbx.LbxError) as e:
-        srv.submit((300, 4, 4), 5, 0.8)
+        srv.submit((160, 128, 128), 5, 0.8)  # FP64 populations exceed the 256 MB staging buffer
     assert e.value.code == lbx.LBX_E_ARG
     with pytest.raises(lbx.LbxError) as e:
         srv.submit((8, 8, 8), 0, 0.8)
