@@ -378,6 +378,8 @@ int launch_step(lbx_sim* sim, Level& L, lbx::StepKind kind) {
 
 // The kernels of one coarse step of the two-level cycle
 // (src/AmrSim.cpp:430-469) for the current buffer parities.
+constexpr int kFinePatchGroup = 16;  // patches per fine launch group (0 = all)
+
 hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   Level& C = sim->lev[0];
   Level& F = sim->lev[1];
@@ -389,11 +391,20 @@ hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   double* fnext = (double*)F.buf[1 - F.cur];
   double* ftmp = (double*)F.buf[2];
   const int np = (int)F.boxes.size();
-  hipError_t e = lbx::launch_fine_fill_collide(fnow, fnext, cnow, C.lat, F.d_desc, np, F.max_cells,
-                                               F.d_fill, rf, st);
-  if (e == hipSuccess) e = lbx::launch_fine_stream_collide(fnext, ftmp, F.d_desc, np, F.max_cells, rf, st);
-  if (e == hipSuccess)
-    e = lbx::launch_fine_stream_avg(ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, np, F.max_blocks, st);
+  // The fine patches' two substeps depend on no other patch's substeps (the
+  // ghosts are filled once, from fnow and cnow), so they run in groups of
+  // patches whose intermediate arrays stay in the MALL.
+  const int group = kFinePatchGroup > 0 ? kFinePatchGroup : np;
+  hipError_t e = hipSuccess;
+  for (int first = 0; first < np && e == hipSuccess; first += group) {
+    const int n = std::min(group, np - first);
+    e = lbx::launch_fine_fill_collide(fnow, fnext, cnow, C.lat, F.d_desc, first, n, F.max_cells, F.d_fill, rf,
+                                      st);
+    if (e == hipSuccess)
+      e = lbx::launch_fine_stream_collide(fnext, ftmp, F.d_desc, first, n, F.max_cells, rf, st);
+    if (e == hipSuccess)
+      e = lbx::launch_fine_stream_avg(ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n, F.max_blocks, st);
+  }
   if (e == hipSuccess)
     e = lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
   // the fine state after the step is fnext

@@ -115,16 +115,18 @@ struct PatchDesc {
   long long avg_base;   // offset of this patch's 2x2x2 block averages ((g/2)^3 per population)
 };
 
-// Two-level kernels (lbx_twolevel.hip), FP64.
+// Two-level kernels (lbx_twolevel.hip), FP64.  The fine kernels run over
+// patches [first, first + npatch).
 hipError_t launch_fine_fill_collide(const double* fnow, double* fnext, const double* cnow,
-                                    const Lattice& clat, const PatchDesc* patches, int npatch,
+                                    const Lattice& clat, const PatchDesc* patches, int first, int npatch,
                                     int max_cells, const long long* fill, const Relax& r,
                                     hipStream_t st);
-hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int npatch,
-                                     int max_cells, const Relax& r, hipStream_t st);
+hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int first,
+                                     int npatch, int max_cells, const Relax& r, hipStream_t st);
 // avg holds 15 planes of avg_stride block averages; max_blocks = max (g/2)^3.
 hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, long long avg_stride,
-                                  const PatchDesc* patches, int npatch, int max_blocks, hipStream_t st);
+                                  const PatchDesc* patches, int first, int npatch, int max_blocks,
+                                  hipStream_t st);
 // flags: per coarse cell (x fastest), bits 0-5 = x+1, x-1, y+1, y-1, z+1,
 // z-1 leave the cell's box; bit 7 = masked by the fine level.
 hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L, const unsigned char* flags,

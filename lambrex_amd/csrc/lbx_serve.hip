@@ -45,7 +45,7 @@ __device__ __forceinline__ void serve_cell(const ServeRun& R, int kind, const S*
   for (int q = 0; q < NV; ++q) {
     const long long off =
         (kind == STEP_COLLIDE) ? selfz + selfp : (zs[1 + kC[q][2]] + ys[1 + kC[q][1]] + xs[1 + kC[q][0]]);
-    f[q] = __builtin_nontemporal_load(src + off + q * R.ps);
+    f[q] = src[off + q * R.ps];  // cached: small runs stay in the 256 MB MALL between steps
   }
   if (kind != STEP_STREAM) {
     if constexpr (sizeof(S) == 8)

@@ -61,9 +61,9 @@ __device__ __forceinline__ bool in_valid(const PatchDesc& P, int x, int y, int z
 // are then collided.
 __global__ __launch_bounds__(256) void k_fine_fill_collide(
     const double* __restrict__ fnow, double* __restrict__ fnext, const double* __restrict__ cnow,
-    Lattice clat, const PatchDesc* __restrict__ patches, const long long* __restrict__ fill,
+    Lattice clat, const PatchDesc* __restrict__ patches, int first, const long long* __restrict__ fill,
     Relax r) {
-  const PatchDesc P = patches[blockIdx.y];
+  const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
@@ -116,8 +116,8 @@ __device__ __forceinline__ bool fine_pull(const PatchDesc& P, const double* __re
 __global__ __launch_bounds__(256) void k_fine_stream_collide(const double* __restrict__ in,
                                                              double* __restrict__ out,
                                                              const PatchDesc* __restrict__ patches,
-                                                             Relax r) {
-  const PatchDesc P = patches[blockIdx.y];
+                                                             int first, Relax r) {
+  const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
@@ -141,15 +141,15 @@ constexpr int kAvgBlocks = 32;
 __global__ __launch_bounds__(256) void k_fine_stream_avg(const double* __restrict__ in,
                                                          double* __restrict__ out,
                                                          double* __restrict__ avg, long long avg_stride,
-                                                         const PatchDesc* __restrict__ patches) {
+                                                         const PatchDesc* __restrict__ patches, int first) {
   __shared__ double part[NV][8][kAvgBlocks];
-  const PatchDesc P = patches[blockIdx.y];
+  const PatchDesc P = patches[first + blockIdx.y];
   const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
   const int nblk = c0 * c1 * c2;
-  const int first = blockIdx.x * kAvgBlocks;
-  if (first >= nblk) return;  // uniform over the workgroup
+  const int b0 = blockIdx.x * kAvgBlocks;  // first block of this workgroup
+  if (b0 >= nblk) return;                  // uniform over the workgroup
   const int child = threadIdx.x / kAvgBlocks, lb = threadIdx.x % kAvgBlocks;
-  const int u = first + lb;
+  const int u = b0 + lb;
   if (u < nblk) {
     const int di = child & 1, dj = (child >> 1) & 1, dk = child >> 2;
     const int cx = u % c0, t = u / c0, cy = t % c1, cz = t / c1;
@@ -171,11 +171,11 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const double* __restric
   __syncthreads();
   for (int task = threadIdx.x; task < NV * kAvgBlocks; task += 256) {
     const int q = task / kAvgBlocks, b = task % kAvgBlocks;
-    if (first + b >= nblk) continue;
+    if (b0 + b >= nblk) continue;
     double sum = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) sum += part[q][c][b];
-    avg[q * avg_stride + P.avg_base + first + b] = sum * 0.125;
+    avg[q * avg_stride + P.avg_base + b0 + b] = sum * 0.125;
   }
 }
 
@@ -298,24 +298,25 @@ __global__ __launch_bounds__(256) void k_patch_io(double* __restrict__ f, const 
 }  // namespace
 
 hipError_t launch_fine_fill_collide(const double* fnow, double* fnext, const double* cnow,
-                                    const Lattice& clat, const PatchDesc* patches, int npatch,
+                                    const Lattice& clat, const PatchDesc* patches, int first, int npatch,
                                     int max_cells, const long long* fill, const Relax& r,
                                     hipStream_t st) {
   k_fine_fill_collide<<<dim3((max_cells + 255) / 256, npatch), 256, 0, st>>>(fnow, fnext, cnow, clat,
-                                                                            patches, fill, r);
+                                                                            patches, first, fill, r);
   return hipGetLastError();
 }
 
-hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int npatch,
-                                     int max_cells, const Relax& r, hipStream_t st) {
-  k_fine_stream_collide<<<dim3((max_cells + 255) / 256, npatch), 256, 0, st>>>(in, out, patches, r);
+hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int first,
+                                     int npatch, int max_cells, const Relax& r, hipStream_t st) {
+  k_fine_stream_collide<<<dim3((max_cells + 255) / 256, npatch), 256, 0, st>>>(in, out, patches, first, r);
   return hipGetLastError();
 }
 
 hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, long long avg_stride,
-                                  const PatchDesc* patches, int npatch, int max_blocks, hipStream_t st) {
-  k_fine_stream_avg<<<dim3((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch), 256, 0, st>>>(in, out, avg,
-                                                                                             avg_stride, patches);
+                                  const PatchDesc* patches, int first, int npatch, int max_blocks,
+                                  hipStream_t st) {
+  k_fine_stream_avg<<<dim3((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch), 256, 0, st>>>(
+      in, out, avg, avg_stride, patches, first);
   return hipGetLastError();
 }
 
