@@ -104,6 +104,18 @@ def test_random_two_level_bitwise_uneven_boxes(ctx):
     assert abs(cons1[0] - ho1[0].sum()) <= 1e-12 * abs(cons1[0])
 
 
+def test_many_small_fine_boxes_bitwise(ctx):
+    """32 fine boxes of 4^3: the fine kernels run in several launch groups
+    of patches, and most ghosts come from other fine boxes."""
+    dims = (16, 8, 8)
+    rho, u = workloads.random_state(dims, seed=29, rho_sigma=1e-2, u_amp=0.02)
+    (fo0, fo1, _), (fg0, fg1, _, _), fboxes = run_both(ctx, dims, 8, 0.75, rho, u, (8, 0, 0), (15, 15, 15), 4,
+                                                       "fine", 3)
+    assert len(fboxes) == 32
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
+
+
 def test_fine_box_wrapping_onto_itself(ctx):
     """One fine box spanning the whole domain in y and z: its ghosts are its
     own periodic images and its average-down wraps onto itself."""
