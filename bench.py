@@ -37,8 +37,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup):
+def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup, keep_fields=False):
+    """Iterate(steps) after Iterate(warmup), timed; with keep_fields the final
+    density and velocity come back as result["fields"]."""
     boxes = workloads.chop(dims, mgs)
+    fields = None
     sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
     try:
         sim.define_level(0, boxes)
@@ -56,6 +59,8 @@ def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup):
         kern_ms, launches = ctx.profile_read()
         ctx.profile_enable(False)
         c1 = sim.conservation()
+        if keep_fields:
+            fields = sim.get_hydro(0)
     finally:
         sim.close()
     cells = dims[0] * dims[1] * dims[2]
@@ -70,6 +75,7 @@ def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup):
         "kernel_launches": launches,
         "kernel_gbs": achieved,
         "mass_drift_rel": abs(c1[0] - c0[0]) / abs(c0[0]),
+        "fields": fields,
     }
 
 
@@ -115,6 +121,27 @@ def serving_trace(n_requests, seed, rate_hz):
             for e, t, a, s, at in zip(edges, taus, amps, steps, arrivals)]
 
 
+def serving_precision_sample(ctx, trace, n):
+    """The first n requests of the trace through an FP64 and an FP32-shifted
+    server: max relative density error, and max |du| relative to each run's
+    pulse amplitude (the velocity scale the pulse starts; a run's own final
+    max |u| can have decayed to round-off after up to 2000 steps)."""
+    outs = {}
+    for prec in (lbx.LBX_FP64, lbx.LBX_FP32_SHIFTED):
+        srv = lbx.Server(ctx, precision=prec, arena_bytes=8 << 30, max_active=44)
+        got = [srv.submit((r["edge"],) * 3, r["steps"], r["tau"], amplitude=r["amp"], axis=0, want_rho=True)[1]
+               for r in trace[:n]]
+        srv.drain()
+        srv.poll()
+        srv.close()
+        outs[prec] = got
+    e_rho = max(float(np.max(np.abs(b["rho"] - a["rho"]) / a["rho"]))
+                for a, b in zip(outs[lbx.LBX_FP64], outs[lbx.LBX_FP32_SHIFTED]))
+    e_u = max(float(np.max(np.abs(b["u"] - a["u"]))) / r["amp"]
+              for a, b, r in zip(outs[lbx.LBX_FP64], outs[lbx.LBX_FP32_SHIFTED], trace[:n]))
+    return e_rho, e_u
+
+
 def serve(ctx, trace, max_active):
     """Replay the trace in real time against one server; returns aggregate
     MLUPS over the makespan and completion latencies (finish - arrival)."""
@@ -153,23 +180,27 @@ def two_level_geometry():
     """Two-level configuration: coarse 128^3 periodic (boxes of 32^3), fine
     level over coarse x in [48, 80) and all y, z (fine 64x256x256 in boxes of
     32^3), tau_coarse 0.8 (fine tau 1.1, src/AmrSim.cpp:313-314), plane pulse
-    rho = 1.01 at coarse x = 64, the reference's fine mask.  A step is one
-    coarse step: 128^3 + 2 x 64x256x256 lattice updates."""
+    rho = 1.01 at coarse x = 64.  The coarse cells under the fine level are
+    masked (the fine-BoxArray mask; the reference's own MakeFineMask masks the
+    coarse low octant instead, src/AmrSim.cpp:419-428, which zeroes coarse
+    cells nowhere near the fine level every step).  A step is one coarse
+    step: 128^3 + 2 x 64x256x256 lattice updates."""
     dims = (128, 128, 128)
     cboxes = workloads.chop(dims, 32)
     ext = (64, 256, 256)
     fboxes = [((96 + a[0], a[1], a[2]), (96 + b[0], b[1], b[2])) for a, b in workloads.chop(ext, 32)]
-    mask = [((2 * a[0], 2 * a[1], 2 * a[2]), (2 * b[0] + 1, 2 * b[1] + 1, 2 * b[2] + 1)) for a, b in cboxes]
+    mask = list(fboxes)
     rho, u = workloads.plane_pulse(dims, 0.01, axis=0, plane=64)
     lu_per_step = 128 ** 3 + 2 * 64 * 256 * 256
     return dims, cboxes, fboxes, mask, rho, u, lu_per_step
 
 
-def two_level(ctx, steps, warmup):
+def two_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
     """The two-level configuration on the GPU (one coarse step = one replay
-    of the captured HIP graph)."""
+    of the captured HIP graph).  Also returns the final density and velocity
+    of both levels (for the FP32 vs FP64 comparison)."""
     dims, cboxes, fboxes, mask, rho, u, lu_per_step = two_level_geometry()
-    sim = lbx.Sim(ctx, dims)
+    sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
     try:
         sim.define_level(0, cboxes)
         sim.set_tau(0, 0.8)
@@ -184,13 +215,26 @@ def two_level(ctx, steps, warmup):
         sim.iterate(steps)
         ctx.sync()
         t1 = time.perf_counter()
+        fields = sim.get_hydro(0) + sim.get_hydro_bbox(1)[:2]
     finally:
         sim.close()
     lu = lu_per_step * steps
+    name = "FP64" if precision == lbx.LBX_FP64 else "FP32-shifted"
     return {"workload": "coarse 128^3 (64 boxes of 32^3) + fine 64x256x256 (128 boxes of 32^3), "
-                        "ratio 2, 2 fine substeps per coarse step, FP64",
-            "coarse_steps": steps, "mlups": lu / (t1 - t0) / 1e6, "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
-            "graph": "one coarse step captured as a HIP graph"}
+                        f"ratio 2, 2 fine substeps per coarse step, {name}",
+            "coarse_steps": steps, "warmup_steps": warmup, "mlups": lu / (t1 - t0) / 1e6,
+            "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
+            "graph": "one coarse step captured as a HIP graph"}, fields
+
+
+def precision_errors(ref, test):
+    """Max relative density error and max |du| / max |u|, over both levels."""
+    r0, u0, r1, u1 = ref
+    s0, v0, s1, v1 = test
+    e_rho = max(float(np.max(np.abs(s0 - r0) / r0)), float(np.max(np.abs(s1 - r1) / r1)))
+    e_u = max(float(np.max(np.abs(v0 - u0)) / np.max(np.abs(u0))),
+              float(np.max(np.abs(v1 - u1)) / np.max(np.abs(u1))))
+    return e_rho, e_u
 
 
 def cpu_two_level(steps):
@@ -279,15 +323,26 @@ def main():
             copy_gbs = ctx.bench_copy(COPY_BYTES, reps=10)
             log(f"[bench] copy ceiling: {copy_gbs:.0f} GB/s (2 GiB read + 2 GiB written per pass)")
         log(f"[bench] FP64 {n}^3 boxes of {args.max_grid_size}^3: warmup {args.warmup}, steps {args.steps}")
-        r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup)
+        r64 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.steps, args.warmup,
+                      keep_fields=not args.no_fp32)
         log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
             f"({r64['kernel_gbs']:.0f} GB/s)")
         shape64 = ctx.tuned_shape(n, n, lbx.LBX_FP64)[0]
         twolev = None
         if args.two_level_steps:
             log(f"[bench] two-level: {args.two_level_steps} coarse steps")
-            twolev = two_level(ctx, args.two_level_steps, 5)
+            twolev, f64_fields = two_level(ctx, args.two_level_steps, 5)
             log(f"[bench] two-level: {twolev['mlups']:.0f} MLUPS, {twolev['ms_per_coarse_step']:.3f} ms per coarse step")
+            if not args.no_fp32:
+                tl32, f32_fields = two_level(ctx, args.two_level_steps, 5, lbx.LBX_FP32_SHIFTED)
+                e_rho, e_u = precision_errors(f64_fields, f32_fields)
+                tl32["max_rel_rho_err_vs_fp64"] = e_rho
+                tl32["max_du_over_max_u_vs_fp64"] = e_u
+                tl32["speedup_vs_fp64"] = round(tl32["mlups"] / twolev["mlups"], 3)
+                twolev["fp32_shifted"] = tl32
+                log(f"[bench] two-level FP32-shifted: {tl32['mlups']:.0f} MLUPS, max rel rho err {e_rho:.2e}, "
+                    f"max |du|/max|u| {e_u:.2e}")
+            del f64_fields
         serving = None
         if args.serve_requests:
             trace = serving_trace(args.serve_requests, seed=7, rate_hz=200.0)
@@ -299,14 +354,24 @@ def main():
             seq = serve(ctx, trace, max_active=1)
             log(f"[bench] sequential: {seq['mlups']:.0f} MLUPS, p50 {seq['p50_ms']:.0f} ms, "
                 f"p99 {seq['p99_ms']:.0f} ms, makespan {seq['makespan_s']:.1f} s")
+            e_rho, e_u = serving_precision_sample(ctx, trace, 16)
+            log(f"[bench] serving FP32 vs FP64 on the first 16 requests: max rel rho err {e_rho:.2e}, "
+                f"max |du|/amplitude {e_u:.2e}")
             serving = {"requests": len(trace), "arrival": "Poisson 200/s (seed 7)", "precision": "FP32-shifted",
                        "batched": batched, "sequential": seq,
-                       "speedup_aggregate_mlups": round(batched["mlups"] / seq["mlups"], 3)}
+                       "speedup_aggregate_mlups": round(batched["mlups"] / seq["mlups"], 3),
+                       "fp32_vs_fp64_first_16": {"max_rel_rho_err": e_rho, "max_du_over_amplitude": e_u}}
         r32 = None
         if not args.no_fp32:
             log("[bench] FP32-shifted, same workload")
             r32 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP32_SHIFTED,
-                          args.steps, args.warmup)
+                          args.steps, args.warmup, keep_fields=True)
+            (r0, u0), (s0, v0) = r64.pop("fields"), r32.pop("fields")
+            r32["max_rel_rho_err_vs_fp64"] = float(np.max(np.abs(s0 - r0) / r0))
+            r32["max_du_over_max_u_vs_fp64"] = float(np.max(np.abs(v0 - u0)) / np.max(np.abs(u0)))
+            del r0, u0, s0, v0
+            log(f"[bench] FP32-shifted vs FP64 after {args.steps} steps: max rel rho err "
+                f"{r32['max_rel_rho_err_vs_fp64']:.2e}, max |du|/max|u| {r32['max_du_over_max_u_vs_fp64']:.2e}")
             shape32 = ctx.tuned_shape(n, n, lbx.LBX_FP32_SHIFTED)[0]
             log(f"[bench] FP32-shifted: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_avg_ms']:.4f} ms "
                 f"({r32['kernel_gbs']:.0f} GB/s)")
@@ -360,6 +425,8 @@ def main():
                          "kernel_avg_ms": round(r32["kernel_avg_ms"], 5),
                          "algorithmic_bytes_per_launch": BYTES_PER_LU[lbx.LBX_FP32_SHIFTED] * cells},
             "workgroup": {"bx": shape32[0], "by": shape32[1], "max_workgroups_per_cu": shape32[2]},
+            "max_rel_rho_err_vs_fp64": r32["max_rel_rho_err_vs_fp64"],
+            "max_du_over_max_u_vs_fp64": r32["max_du_over_max_u_vs_fp64"],
         }
     if twolev:
         out["two_level"] = twolev

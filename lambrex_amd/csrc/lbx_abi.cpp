@@ -3,8 +3,8 @@
 // Owns device memory and the per-context HIP stream; never aborts: every
 // failure becomes a negative status plus a message in lbx_last_error().
 //
-// Level 0 is a periodic level array (its boxes are index ranges into one
-// SoA array).  Level 1, when defined, is a set of patches, one per fine box,
+// Level 0 is a periodic level array (z-plane major; its boxes are index
+// ranges into it).  Level 1, when defined, is a set of patches, one per fine box,
 // each grown by 2 ghost layers; the tables that drive its ghost fill and its
 // average-down are built here on the host once per level definition.
 #include <hip/hip_runtime.h>
@@ -383,13 +383,14 @@ constexpr int kFinePatchGroup = 16;  // patches per fine launch group (0 = all)
 hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   Level& C = sim->lev[0];
   Level& F = sim->lev[1];
+  const int prec = sim->precision;
   const Relax rc{C.omega_s, C.omega_b, sim->rho0};
   const Relax rf{F.omega_s, F.omega_b, sim->rho0};
-  const double* cnow = (const double*)C.buf[C.cur];
-  double* cnext = (double*)C.buf[1 - C.cur];
-  double* fnow = (double*)F.buf[F.cur];
-  double* fnext = (double*)F.buf[1 - F.cur];
-  double* ftmp = (double*)F.buf[2];
+  const void* cnow = C.buf[C.cur];
+  void* cnext = C.buf[1 - C.cur];
+  void* fnow = F.buf[F.cur];
+  void* fnext = F.buf[1 - F.cur];
+  void* ftmp = F.buf[2];
   const int np = (int)F.boxes.size();
   // The fine patches' two substeps depend on no other patch's substeps (the
   // ghosts are filled once, from fnow and cnow), so they run in groups of
@@ -398,19 +399,19 @@ hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   hipError_t e = hipSuccess;
   for (int first = 0; first < np && e == hipSuccess; first += group) {
     const int n = std::min(group, np - first);
-    e = lbx::launch_fine_fill_collide(fnow, fnext, cnow, C.lat, F.d_desc, first, n, F.max_cells, F.d_fill, rf,
-                                      st);
+    e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, F.d_desc, first, n, F.max_cells, F.d_fill,
+                                      rf, st);
     if (e == hipSuccess)
-      e = lbx::launch_fine_stream_collide(fnext, ftmp, F.d_desc, first, n, F.max_cells, rf, st);
+      e = lbx::launch_fine_stream_collide(prec, fnext, ftmp, F.d_desc, first, n, F.max_cells, rf, st);
     if (e == hipSuccess)
-      e = lbx::launch_fine_stream_avg(ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n, F.max_blocks, st);
+      e = lbx::launch_fine_stream_avg(prec, ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n, F.max_blocks,
+                                      sim->rho0, st);
   }
-  if (e == hipSuccess)
-    e = lbx::launch_coarse_push(cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
+  if (e == hipSuccess) e = lbx::launch_coarse_push(prec, cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
   // the fine state after the step is fnext
   if (e == hipSuccess)
-    e = lbx::launch_sum_apply(F.d_avg, F.avg_stride, cnext, C.lat, F.d_sum_cell, F.d_sum_start,
-                              F.d_sum_contrib, F.sum_ncell, st);
+    e = lbx::launch_sum_apply(prec, F.d_avg, F.avg_stride, cnext, C.lat, F.d_sum_cell, F.d_sum_start,
+                              F.d_sum_contrib, F.sum_ncell, sim->rho0, st);
   return e;
 }
 
@@ -534,8 +535,6 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
   lbx_ctx* ctx = sim->ctx;
   Level& C = sim->lev[0];
   if (!C.defined) return fail(ctx, LBX_E_STATE, "define level 0 before level 1");
-  if (sim->precision != LBX_FP64)
-    return fail(ctx, LBX_E_UNSUPPORTED, "two-level runs use FP64 storage");
   const int fdom[3] = {2 * sim->dims[0], 2 * sim->dims[1], 2 * sim->dims[2]};
   for (size_t b = 0; b < boxes.size(); ++b) {
     if (!box_ok(boxes[b], fdom))
@@ -574,7 +573,7 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
   }
   if (avg_base >= (1ll << 31)) return fail(ctx, LBX_E_ARG, "fine level too large");
   F.avg_stride = (avg_base + 63) / 64 * 64;
-  const size_t bytes = (size_t)base * sizeof(double);
+  const size_t bytes = (size_t)base * elem_size(sim->precision);
   {
     const size_t abytes = (size_t)F.avg_stride * lbx::NV_TOTAL * sizeof(double);
     hipError_t e = hipMalloc(&F.d_avg, abytes);
@@ -675,7 +674,7 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
     const std::vector<unsigned char> flags = coarse_flags(C, {});
     e = upload(&C.d_mask, flags, ctx->stream);
   }
-  C.push_shape = lbx::default_shape(C.lat, LBX_FP64);
+  C.push_shape = lbx::default_shape(C.lat, sim->precision);
   if (e != hipSuccess) {
     free_level(F);
     return hip_fail(ctx, e, "level-1 tables");
@@ -832,8 +831,8 @@ int lbx_fill_from_coarse(lbx_sim* sim, int level) {
   Level& F = sim->lev[1];
   if (!C.defined || !F.defined) return fail(ctx, LBX_E_STATE, "levels 0 and 1 must be defined");
   LBX_HIP(ctx, hipSetDevice(ctx->device));
-  LBX_HIP(ctx, lbx::launch_fine_from_coarse((double*)F.buf[F.cur], (const double*)C.buf[C.cur], C.lat,
-                                            F.d_desc, (int)F.boxes.size(), F.max_cells, ctx->stream));
+  LBX_HIP(ctx, lbx::launch_fine_from_coarse(sim->precision, F.buf[F.cur], C.buf[C.cur], C.lat, F.d_desc,
+                                            (int)F.boxes.size(), F.max_cells, sim->rho0, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }
@@ -872,8 +871,8 @@ int lbx_upload_f(lbx_sim* sim, int level, int box, const double* f) {
   LBX_HIP(ctx, hipMemcpyAsync(t.p, f, n * lbx::NV_TOTAL * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
   if (L.patched)
-    LBX_HIP(ctx, lbx::launch_patch_io(1, (double*)L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
-                                      nullptr, ctx->stream));
+    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 1, L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
+                                      nullptr, sim->rho0, ctx->stream));
   else
     LBX_HIP(ctx, lbx::launch_unpack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
                                       (const double*)t.p, sim->rho0, ctx->stream));
@@ -891,8 +890,8 @@ int lbx_download_f(lbx_sim* sim, int level, int box, double* f) {
   DeviceTemp t;
   LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
   if (L.patched)
-    LBX_HIP(ctx, lbx::launch_patch_io(0, (double*)L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
-                                      nullptr, ctx->stream));
+    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 0, L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
+                                      nullptr, sim->rho0, ctx->stream));
   else
     LBX_HIP(ctx, lbx::launch_pack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
                                     (double*)t.p, sim->rho0, ctx->stream));
@@ -941,8 +940,8 @@ int lbx_hydro(lbx_sim* sim, int level, int box, double* rho, double* u) {
   double* drho = (double*)t.p;
   double* du = drho + n;
   if (L.patched)
-    LBX_HIP(ctx, lbx::launch_patch_io(2, (double*)L.buf[L.cur], L.d_desc, box, (int)n, drho, du,
-                                      ctx->stream));
+    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 2, L.buf[L.cur], L.d_desc, box, (int)n, drho, du,
+                                      sim->rho0, ctx->stream));
   else
     LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]), drho,
                                    du, sim->rho0, ctx->stream));
@@ -975,8 +974,8 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]) {
     const long long n = L.boxes[b].cells();
     DeviceTemp t;
     LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
-    LBX_HIP(ctx, lbx::launch_patch_io(0, (double*)L.buf[L.cur], L.d_desc, (int)b, (int)n, (double*)t.p,
-                                      nullptr, ctx->stream));
+    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 0, L.buf[L.cur], L.d_desc, (int)b, (int)n, (double*)t.p,
+                                      nullptr, sim->rho0, ctx->stream));
     const Lattice packed{(int)n, 1, 1, n, lbx::NV_TOTAL * n};  // population-major, stride n
     LBX_HIP(ctx, lbx::launch_conservation(LBX_FP64, t.p, packed, 0.0, partials, kPartials, res,
                                           ctx->stream));

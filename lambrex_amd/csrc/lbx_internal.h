@@ -115,31 +115,32 @@ struct PatchDesc {
   long long avg_base;   // offset of this patch's 2x2x2 block averages ((g/2)^3 per population)
 };
 
-// Two-level kernels (lbx_twolevel.hip), FP64.  The fine kernels run over
+// Two-level kernels (lbx_twolevel.hip); precision 0 = FP64 storage, else
+// FP32-shifted (rho0 in Relax or the argument).  The fine kernels run over
 // patches [first, first + npatch).
-hipError_t launch_fine_fill_collide(const double* fnow, double* fnext, const double* cnow,
+hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
                                     const Lattice& clat, const PatchDesc* patches, int first, int npatch,
-                                    int max_cells, const long long* fill, const Relax& r,
-                                    hipStream_t st);
-hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int first,
-                                     int npatch, int max_cells, const Relax& r, hipStream_t st);
-// avg holds 15 planes of avg_stride block averages; max_blocks = max (g/2)^3.
-hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, long long avg_stride,
-                                  const PatchDesc* patches, int first, int npatch, int max_blocks,
+                                    int max_cells, const long long* fill, const Relax& r, hipStream_t st);
+hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, const PatchDesc* patches,
+                                     int first, int npatch, int max_cells, const Relax& r, hipStream_t st);
+// avg holds 15 FP64 planes of avg_stride block averages; max_blocks = max (g/2)^3.
+hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
+                                  const PatchDesc* patches, int first, int npatch, int max_blocks, double rho0,
                                   hipStream_t st);
 // flags: per coarse cell (x fastest), bits 0-5 = x+1, x-1, y+1, y-1, z+1,
 // z-1 leave the cell's box; bit 7 = masked by the fine level.
-hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L, const unsigned char* flags,
-                              const Relax& r, LaunchShape shape, hipStream_t st);
-hipError_t launch_sum_apply(const double* avg, long long avg_stride, double* cnext, const Lattice& L,
-                            const long long* cell, const int* start, const int* contrib, int ncell,
+hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, const Lattice& L,
+                              const unsigned char* flags, const Relax& r, LaunchShape shape, hipStream_t st);
+hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stride, void* cnext, const Lattice& L,
+                            const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
                             hipStream_t st);
-hipError_t launch_fine_from_coarse(double* fnow, const double* cnow, const Lattice& clat,
-                                   const PatchDesc* patches, int npatch, int max_cells,
+hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, const Lattice& clat,
+                                   const PatchDesc* patches, int npatch, int max_cells, double rho0,
                                    hipStream_t st);
-// op 0: pack valid cells of patch b into a[15][n]; 1: unpack; 2: hydro into a (rho), u.
-hipError_t launch_patch_io(int op, double* f, const PatchDesc* patches, int b, int valid_cells,
-                           double* a, double* u, hipStream_t st);
+// op 0: pack valid cells of patch b into FP64 f a[15][n]; 1: unpack; 2: hydro
+// into a (rho), u.
+hipError_t launch_patch_io(int precision, int op, void* f, const PatchDesc* patches, int b, int valid_cells,
+                           double* a, double* u, double rho0, hipStream_t st);
 
 // Serving: one active run of the batch, passed by value in the kernel
 // arguments (kServeMaxRuns * sizeof(ServeRun) stays under 4 KB).

@@ -1,5 +1,5 @@
 // lbx_twolevel.hip -- device kernels of the two-level Rohde cycle
-// (src/AmrSim.cpp:430-631) on gfx950, FP64.
+// (src/AmrSim.cpp:430-631) on gfx950, FP64 or FP32-shifted storage.
 //
 // Storage.  The coarse level is a periodic level array (lbx_lattice.hip).
 // Each fine box is a patch of its own, grown by 2 ghost layers like the
@@ -25,6 +25,11 @@
 //   k_sum_apply           SumFromFine(0): ADD those block averages (fine valid
 //                         + ghosts) into coarse next, contributions to one
 //                         coarse cell in a fixed order
+//
+// Storage type S is double (FP64, bit-exact with the oracle) or float (the
+// shifted deviation d = f - w rho0 of the FP32 path).  Either way the block
+// sums and the coarse additions of SumFromFine are done in FP64 on
+// reconstructed f, and a population set to zero means f = 0 (d = -w rho0).
 #include <hip/hip_runtime.h>
 
 #include "d3q15.h"
@@ -34,6 +39,37 @@
 
 namespace lbx {
 namespace {
+
+// f of a stored value, and the stored value of f, for population q.
+template <typename S>
+__device__ __forceinline__ double to_f64(S v, int q, double rho0) {
+  if constexpr (sizeof(S) == 8)
+    return v;
+  else
+    return kT.W[q] * rho0 + double(v);
+}
+
+template <typename S>
+__device__ __forceinline__ S from_f64(double f, int q, double rho0) {
+  if constexpr (sizeof(S) == 8)
+    return f;
+  else
+    return float(f - kT.W[q] * rho0);
+}
+
+// The stored value of f_q = 0.
+template <typename S>
+__device__ __forceinline__ S zero_pop(int q, double rho0) {
+  return from_f64<S>(0.0, q, rho0);
+}
+
+template <typename S>
+__device__ __forceinline__ void collide_s(S* f, const Relax& r) {
+  if constexpr (sizeof(S) == 8)
+    collide_f64(f, r.omega_s, r.omega_b);
+  else
+    collide_f32_shifted(f, float(r.rho0), float(r.omega_s), float(r.omega_b));
+}
 
 __device__ __forceinline__ void patch_coords(const PatchDesc& P, int i, int& gx, int& gy, int& gz) {
   gx = i % P.g[0];
@@ -59,17 +95,17 @@ __device__ __forceinline__ bool in_valid(const PatchDesc& P, int x, int y, int z
 // cell, the fine valid cell holding its periodic image, or the coarse
 // parent) so the loads of valid and ghost lanes issue together; valid cells
 // are then collided.
+template <typename S>
 __global__ __launch_bounds__(256) void k_fine_fill_collide(
-    const double* __restrict__ fnow, double* __restrict__ fnext, const double* __restrict__ cnow,
-    Lattice clat, const PatchDesc* __restrict__ patches, int first, const long long* __restrict__ fill,
-    Relax r) {
+    const S* __restrict__ fnow, S* __restrict__ fnext, const S* __restrict__ cnow, Lattice clat,
+    const PatchDesc* __restrict__ patches, int first, const long long* __restrict__ fill, Relax r) {
   const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
   const bool valid = patch_valid(P, gx, gy, gz);
-  const double* src;
+  const S* src;
   long long stride;
   if (valid) {
     src = fnow + P.base + i;
@@ -85,24 +121,25 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
       stride = clat.ps;
     }
   }
-  double f[NV];
+  S f[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) f[q] = src[q * stride];
   if (valid)
-    collide_f64(f, r.omega_s, r.omega_b);
+    collide_s(f, r);
   else if (patch_outer(P, gx, gy, gz))
-    f[0] = 0.0;
+    f[0] = zero_pop<S>(0, r.rho0);
 #pragma unroll
   for (int q = 0; q < NV; ++q) fnext[P.base + q * P.qs + i] = f[q];
 }
 
 // Pull of grown cell i over valid + 1 ghost layer; the outer ghost layer of
 // the fresh array is defined as zero.  Returns whether i is a valid cell.
-__device__ __forceinline__ bool fine_pull(const PatchDesc& P, const double* __restrict__ in, int i, int gx,
-                                          int gy, int gz, double* f) {
+template <typename S>
+__device__ __forceinline__ bool fine_pull(const PatchDesc& P, const S* __restrict__ in, int i, int gx, int gy,
+                                          int gz, double rho0, S* f) {
   if (patch_outer(P, gx, gy, gz)) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) f[q] = 0.0;
+    for (int q = 0; q < NV; ++q) f[q] = zero_pop<S>(q, rho0);
     return false;
   }
   const int sx = 1, sy = P.g[0], sz = P.g[0] * P.g[1];
@@ -113,17 +150,17 @@ __device__ __forceinline__ bool fine_pull(const PatchDesc& P, const double* __re
 
 // First fine substep's stream + the second substep's collision: pull, then
 // collide valid cells.
-__global__ __launch_bounds__(256) void k_fine_stream_collide(const double* __restrict__ in,
-                                                             double* __restrict__ out,
-                                                             const PatchDesc* __restrict__ patches,
-                                                             int first, Relax r) {
+template <typename S>
+__global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict__ in, S* __restrict__ out,
+                                                             const PatchDesc* __restrict__ patches, int first,
+                                                             Relax r) {
   const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
-  double f[NV];
-  if (fine_pull(P, in, i, gx, gy, gz, f)) collide_f64(f, r.omega_s, r.omega_b);
+  S f[NV];
+  if (fine_pull(P, in, i, gx, gy, gz, r.rho0, f)) collide_s(f, r);
 #pragma unroll
   for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
 }
@@ -132,16 +169,17 @@ __global__ __launch_bounds__(256) void k_fine_stream_collide(const double* __res
 // population m only if pos - 2 c_m lies in the valid box), fused with the
 // first half of SumFromFine: the 1/8-weighted sum of each 2x2x2 block of the
 // grown patch (= one coarse cell of the coarsened box grown by 1), children
-// added in (k, j, i) order, to avg[q * avg_stride + avg_base + block].
+// added in (k, j, i) order in FP64, to avg[q * avg_stride + avg_base + block].
 // A workgroup takes 32 consecutive blocks; thread t computes child t / 32 of
 // block t % 32, so each wavefront reads whole x-pairs of its blocks.  The
 // children meet in LDS for the ordered sums.
 constexpr int kAvgBlocks = 32;
 
-__global__ __launch_bounds__(256) void k_fine_stream_avg(const double* __restrict__ in,
-                                                         double* __restrict__ out,
+template <typename S>
+__global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ in, S* __restrict__ out,
                                                          double* __restrict__ avg, long long avg_stride,
-                                                         const PatchDesc* __restrict__ patches, int first) {
+                                                         const PatchDesc* __restrict__ patches, int first,
+                                                         double rho0) {
   __shared__ double part[NV][8][kAvgBlocks];
   const PatchDesc P = patches[first + blockIdx.y];
   const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
@@ -155,17 +193,17 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const double* __restric
     const int cx = u % c0, t = u / c0, cy = t % c1, cz = t / c1;
     const int gx = 2 * cx + di, gy = 2 * cy + dj, gz = 2 * cz + dk;
     const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
-    double f[NV];
-    if (!fine_pull(P, in, i, gx, gy, gz, f) && !patch_outer(P, gx, gy, gz)) {
+    S f[NV];
+    if (!fine_pull(P, in, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz)) {
       const int x = P.lo[0] - 2 + gx, y = P.lo[1] - 2 + gy, z = P.lo[2] - 2 + gz;
 #pragma unroll
       for (int q = 0; q < NV; ++q)
-        if (!in_valid(P, x - 2 * kC[q][0], y - 2 * kC[q][1], z - 2 * kC[q][2])) f[q] = 0.0;
+        if (!in_valid(P, x - 2 * kC[q][0], y - 2 * kC[q][1], z - 2 * kC[q][2])) f[q] = zero_pop<S>(q, rho0);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       out[P.base + q * P.qs + i] = f[q];
-      part[q][child][lb] = f[q];
+      part[q][child][lb] = to_f64(f[q], q, rho0);
     }
   }
   __syncthreads();
@@ -190,9 +228,9 @@ __device__ constexpr unsigned leave_bits(int q) {
 }
 
 // Over the whole coarse level array, shaped like the stream-collide launch.
-__global__ __launch_bounds__(512) void k_coarse_push(const double* __restrict__ cnow,
-                                                     double* __restrict__ cnext, Lattice L,
-                                                     const unsigned char* __restrict__ flags,
+template <typename S>
+__global__ __launch_bounds__(512) void k_coarse_push(const S* __restrict__ cnow, S* __restrict__ cnext,
+                                                     Lattice L, const unsigned char* __restrict__ flags,
                                                      Relax r) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -200,14 +238,14 @@ __global__ __launch_bounds__(512) void k_coarse_push(const double* __restrict__ 
   if (x >= L.nx || y >= L.ny) return;
   const long long self = L.at(0, y * L.nx + x, z);
   const unsigned fl = flags[((long long)z * L.ny + y) * L.nx + x];
-  double pre[NV], post[NV];
+  S pre[NV], post[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) pre[q] = post[q] = __builtin_nontemporal_load(cnow + self + q * L.ps);
   if (fl & kMasked) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) post[q] = 0.0;
+    for (int q = 0; q < NV; ++q) post[q] = zero_pop<S>(q, r.rho0);
   } else {
-    collide_f64(post, r.omega_s, r.omega_b);
+    collide_s(post, r);
   }
   const int xs[3] = {(x == 0) ? L.nx - 1 : x - 1, x, (x == L.nx - 1) ? 0 : x + 1};
   const int ys[3] = {((y == 0) ? L.ny - 1 : y - 1) * L.nx, y * L.nx, ((y == L.ny - 1) ? 0 : y + 1) * L.nx};
@@ -221,34 +259,35 @@ __global__ __launch_bounds__(512) void k_coarse_push(const double* __restrict__ 
 
 // Second half of SumFromFine: thread per touched coarse cell; its
 // contributions [start[u], start[u+1]) (indices into avg, in the order the
-// reference's loops add them) are added to the streamed coarse value.
+// reference's loops add them) are added in FP64 to the streamed coarse value.
+template <typename S>
 __global__ __launch_bounds__(256) void k_sum_apply(const double* __restrict__ avg, long long avg_stride,
-                                                   double* __restrict__ cnext, Lattice L,
+                                                   S* __restrict__ cnext, Lattice L,
                                                    const long long* __restrict__ cell,
                                                    const int* __restrict__ start,
-                                                   const int* __restrict__ contrib, int ncell) {
+                                                   const int* __restrict__ contrib, int ncell, double rho0) {
   const int u = blockIdx.x * 256 + threadIdx.x;
   if (u >= ncell) return;
   const long long c = cell[u];  // coarse element of population 0
   double acc[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = cnext[c + q * L.ps];
+  for (int q = 0; q < NV; ++q) acc[q] = to_f64(cnext[c + q * L.ps], q, rho0);
   for (int j = start[u]; j < start[u + 1]; ++j) {
     const long long a = contrib[j];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] += avg[q * avg_stride + a];
   }
 #pragma unroll
-  for (int q = 0; q < NV; ++q) cnext[c + q * L.ps] = acc[q];
+  for (int q = 0; q < NV; ++q) cnext[c + q * L.ps] = from_f64<S>(acc[q], q, rho0);
 }
 
 // DistFnFillFromCoarse (src/AmrSim.cpp:393-411): valid fine cells take the
 // coarse value of their parent; ghosts are zeroed (they are rewritten before
 // they are read).
-__global__ __launch_bounds__(256) void k_fine_from_coarse(double* __restrict__ fnow,
-                                                          const double* __restrict__ cnow,
-                                                          Lattice clat,
-                                                          const PatchDesc* __restrict__ patches) {
+template <typename S>
+__global__ __launch_bounds__(256) void k_fine_from_coarse(S* __restrict__ fnow, const S* __restrict__ cnow,
+                                                          Lattice clat, const PatchDesc* __restrict__ patches,
+                                                          double rho0) {
   const PatchDesc P = patches[blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
@@ -261,14 +300,15 @@ __global__ __launch_bounds__(256) void k_fine_from_coarse(double* __restrict__ f
     for (int q = 0; q < NV; ++q) fnow[P.base + q * P.qs + i] = cnow[c + q * clat.ps];
   } else {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) fnow[P.base + q * P.qs + i] = 0.0;
+    for (int q = 0; q < NV; ++q) fnow[P.base + q * P.qs + i] = zero_pop<S>(q, rho0);
   }
 }
 
-// Valid cells of one patch <-> packed FP64 [15][nz][ny][nx]; hydro of one patch.
-template <int OP>  // 0 pack, 1 unpack, 2 hydro
-__global__ __launch_bounds__(256) void k_patch_io(double* __restrict__ f, const PatchDesc* __restrict__ patches,
-                                                  int b, double* __restrict__ a, double* __restrict__ u) {
+// Valid cells of one patch <-> packed FP64 f [15][nz][ny][nx]; hydro of one
+// patch.
+template <int OP, typename S>  // 0 pack, 1 unpack, 2 hydro
+__global__ __launch_bounds__(256) void k_patch_io(S* __restrict__ f, const PatchDesc* __restrict__ patches, int b,
+                                                  double* __restrict__ a, double* __restrict__ u, double rho0) {
   const PatchDesc P = patches[b];
   const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1, nz = P.hi[2] - P.lo[2] + 1;
   const int n = nx * ny * nz;
@@ -278,14 +318,14 @@ __global__ __launch_bounds__(256) void k_patch_io(double* __restrict__ f, const 
   const long long li = ((long long)(z + 2) * P.g[1] + (y + 2)) * P.g[0] + (x + 2);
   if (OP == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) a[(long long)q * n + i] = f[P.base + q * P.qs + li];
+    for (int q = 0; q < NV; ++q) a[(long long)q * n + i] = to_f64(f[P.base + q * P.qs + li], q, rho0);
   } else if (OP == 1) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) f[P.base + q * P.qs + li] = a[(long long)q * n + i];
+    for (int q = 0; q < NV; ++q) f[P.base + q * P.qs + li] = from_f64<S>(a[(long long)q * n + i], q, rho0);
   } else {
     double g[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) g[q] = f[P.base + q * P.qs + li];
+    for (int q = 0; q < NV; ++q) g[q] = to_f64(f[P.base + q * P.qs + li], q, rho0);
     double rr, u0, u1, u2;
     hydro_f64(g, rr, u0, u1, u2);
     a[i] = rr;
@@ -295,63 +335,97 @@ __global__ __launch_bounds__(256) void k_patch_io(double* __restrict__ f, const 
   }
 }
 
-}  // namespace
-
-hipError_t launch_fine_fill_collide(const double* fnow, double* fnext, const double* cnow,
-                                    const Lattice& clat, const PatchDesc* patches, int first, int npatch,
-                                    int max_cells, const long long* fill, const Relax& r,
-                                    hipStream_t st) {
-  k_fine_fill_collide<<<dim3((max_cells + 255) / 256, npatch), 256, 0, st>>>(fnow, fnext, cnow, clat,
-                                                                            patches, first, fill, r);
-  return hipGetLastError();
-}
-
-hipError_t launch_fine_stream_collide(const double* in, double* out, const PatchDesc* patches, int first,
-                                     int npatch, int max_cells, const Relax& r, hipStream_t st) {
-  k_fine_stream_collide<<<dim3((max_cells + 255) / 256, npatch), 256, 0, st>>>(in, out, patches, first, r);
-  return hipGetLastError();
-}
-
-hipError_t launch_fine_stream_avg(const double* in, double* out, double* avg, long long avg_stride,
-                                  const PatchDesc* patches, int first, int npatch, int max_blocks,
-                                  hipStream_t st) {
-  k_fine_stream_avg<<<dim3((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch), 256, 0, st>>>(
-      in, out, avg, avg_stride, patches, first);
-  return hipGetLastError();
-}
-
-hipError_t launch_coarse_push(const double* cnow, double* cnext, const Lattice& L, const unsigned char* flags,
-                              const Relax& r, LaunchShape sh, hipStream_t st) {
-  const dim3 grid((L.nx + sh.bx - 1) / sh.bx, (L.ny + sh.by - 1) / sh.by, L.nz);
-  k_coarse_push<<<grid, dim3(sh.bx, sh.by, 1), sh.lds_bytes, st>>>(cnow, cnext, L, flags, r);
-  return hipGetLastError();
-}
-
-hipError_t launch_sum_apply(const double* avg, long long avg_stride, double* cnext, const Lattice& L,
-                            const long long* cell, const int* start, const int* contrib, int ncell,
-                            hipStream_t st) {
-  if (ncell > 0)
-    k_sum_apply<<<(ncell + 255) / 256, 256, 0, st>>>(avg, avg_stride, cnext, L, cell, start, contrib, ncell);
-  return hipGetLastError();
-}
-
-hipError_t launch_fine_from_coarse(double* fnow, const double* cnow, const Lattice& clat,
-                                   const PatchDesc* patches, int npatch, int max_cells,
-                                   hipStream_t st) {
-  k_fine_from_coarse<<<dim3((max_cells + 255) / 256, npatch), 256, 0, st>>>(fnow, cnow, clat, patches);
-  return hipGetLastError();
-}
-
-hipError_t launch_patch_io(int op, double* f, const PatchDesc* patches, int b, int valid_cells,
-                           double* a, double* u, hipStream_t st) {
+template <typename S>
+hipError_t patch_io(int op, S* f, const PatchDesc* patches, int b, int valid_cells, double* a, double* u,
+                    double rho0, hipStream_t st) {
   const int blocks = (valid_cells + 255) / 256;
   if (op == 0)
-    k_patch_io<0><<<blocks, 256, 0, st>>>(f, patches, b, a, u);
+    k_patch_io<0, S><<<blocks, 256, 0, st>>>(f, patches, b, a, u, rho0);
   else if (op == 1)
-    k_patch_io<1><<<blocks, 256, 0, st>>>(f, patches, b, a, u);
+    k_patch_io<1, S><<<blocks, 256, 0, st>>>(f, patches, b, a, u, rho0);
   else
-    k_patch_io<2><<<blocks, 256, 0, st>>>(f, patches, b, a, u);
+    k_patch_io<2, S><<<blocks, 256, 0, st>>>(f, patches, b, a, u, rho0);
   return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
+                                    const Lattice& clat, const PatchDesc* patches, int first, int npatch,
+                                    int max_cells, const long long* fill, const Relax& r, hipStream_t st) {
+  const dim3 grid((max_cells + 255) / 256, npatch);
+  if (precision == 0)
+    k_fine_fill_collide<double><<<grid, 256, 0, st>>>((const double*)fnow, (double*)fnext, (const double*)cnow,
+                                                      clat, patches, first, fill, r);
+  else
+    k_fine_fill_collide<float><<<grid, 256, 0, st>>>((const float*)fnow, (float*)fnext, (const float*)cnow, clat,
+                                                     patches, first, fill, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, const PatchDesc* patches,
+                                     int first, int npatch, int max_cells, const Relax& r, hipStream_t st) {
+  const dim3 grid((max_cells + 255) / 256, npatch);
+  if (precision == 0)
+    k_fine_stream_collide<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, r);
+  else
+    k_fine_stream_collide<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
+                                  const PatchDesc* patches, int first, int npatch, int max_blocks, double rho0,
+                                  hipStream_t st) {
+  const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
+  if (precision == 0)
+    k_fine_stream_avg<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride, patches,
+                                                    first, rho0);
+  else
+    k_fine_stream_avg<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride, patches,
+                                                   first, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, const Lattice& L,
+                              const unsigned char* flags, const Relax& r, LaunchShape sh, hipStream_t st) {
+  const dim3 grid((L.nx + sh.bx - 1) / sh.bx, (L.ny + sh.by - 1) / sh.by, L.nz);
+  const dim3 block(sh.bx, sh.by, 1);
+  if (precision == 0)
+    k_coarse_push<double><<<grid, block, sh.lds_bytes, st>>>((const double*)cnow, (double*)cnext, L, flags, r);
+  else
+    k_coarse_push<float><<<grid, block, sh.lds_bytes, st>>>((const float*)cnow, (float*)cnext, L, flags, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stride, void* cnext, const Lattice& L,
+                            const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
+                            hipStream_t st) {
+  if (ncell <= 0) return hipSuccess;
+  const int blocks = (ncell + 255) / 256;
+  if (precision == 0)
+    k_sum_apply<double><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, cell, start, contrib, ncell,
+                                                rho0);
+  else
+    k_sum_apply<float><<<blocks, 256, 0, st>>>(avg, avg_stride, (float*)cnext, L, cell, start, contrib, ncell,
+                                               rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, const Lattice& clat,
+                                   const PatchDesc* patches, int npatch, int max_cells, double rho0,
+                                   hipStream_t st) {
+  const dim3 grid((max_cells + 255) / 256, npatch);
+  if (precision == 0)
+    k_fine_from_coarse<double><<<grid, 256, 0, st>>>((double*)fnow, (const double*)cnow, clat, patches, rho0);
+  else
+    k_fine_from_coarse<float><<<grid, 256, 0, st>>>((float*)fnow, (const float*)cnow, clat, patches, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_io(int precision, int op, void* f, const PatchDesc* patches, int b, int valid_cells,
+                           double* a, double* u, double rho0, hipStream_t st) {
+  if (precision == 0) return patch_io(op, (double*)f, patches, b, valid_cells, a, u, rho0, st);
+  return patch_io(op, (float*)f, patches, b, valid_cells, a, u, rho0, st);
 }
 
 }  // namespace lbx

@@ -285,6 +285,20 @@ class Sim:
                 blo[0] - lo[0]:bhi[0] - lo[0] + 1] = self.download_f(level, b)
         return out, lo
 
+    def get_hydro_bbox(self, level):
+        """rho [nz][ny][nx] and u [3][nz][ny][nx] over the bounding box of the
+        level's boxes (cells outside every box are zero), and the box lo."""
+        bl = self.boxes[level]
+        lo = [min(b[0][d] for b in bl) for d in range(3)]
+        hi = [max(b[1][d] for b in bl) for d in range(3)]
+        shape = (hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+        rho, u = np.zeros(shape), np.zeros((3,) + shape)
+        for b, (blo, bhi) in enumerate(bl):
+            sl = (slice(blo[2] - lo[2], bhi[2] - lo[2] + 1), slice(blo[1] - lo[1], bhi[1] - lo[1] + 1),
+                  slice(blo[0] - lo[0], bhi[0] - lo[0] + 1))
+            rho[sl], u[(slice(None),) + sl] = self.hydro(level, b)
+        return rho, u, lo
+
     def set_f(self, f, level=0):
         for b, (lo, hi) in enumerate(self.boxes[level]):
             self.upload_f(level, b, f[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])

@@ -163,14 +163,17 @@ def level_valid_array(L, bbox_lo, bbox_hi):
     return out
 
 
-@pytest.mark.parametrize("mask_kind", ["coarse", "fine"])
+@pytest.mark.parametrize("mask_kind", ["reference", "fine", "all"])
 def test_two_level_cycle_matches_independent_restatement(mask_kind):
     dims = (8, 4, 4)
     tau = 0.8
     cboxes = [(tuple(a), tuple(b)) for a, b in workloads.chop(dims, 4)]
     fboxes = [((4, 0, 0), (7, 3, 7)), ((4, 4, 0), (7, 7, 7)), ((8, 0, 0), (11, 7, 3)), ((8, 0, 4), (11, 7, 7))]
-    mboxes = [(tuple(2 * x for x in a), tuple(2 * x + 1 for x in b)) for a, b in cboxes] \
-        if mask_kind == "coarse" else fboxes
+    # "reference": the coarse BoxArray passed as the fine one, as MakeFineMask
+    # does (src/AmrSim.cpp:419-428); "fine": the fine boxes; "all": every
+    # coarse cell
+    mboxes = {"reference": list(cboxes), "fine": fboxes,
+              "all": [(tuple(2 * x for x in a), tuple(2 * x + 1 for x in b)) for a, b in cboxes]}[mask_kind]
     rho, u = workloads.random_state(dims, seed=5, rho_sigma=1e-2, u_amp=0.02)
     o = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in cboxes])
     o.init(np.ascontiguousarray(rho.transpose(2, 1, 0)).reshape(-1),
