@@ -40,7 +40,8 @@ extern "C" {
 #define LBX_E_UNSUPPORTED (-5)
 
 #define LBX_FP64 0          /* populations stored and collided in FP64 */
-#define LBX_FP32_SHIFTED 1  /* stored as f_i - w_i*rho0 in FP32, collided in FP32 */
+#define LBX_FP32_SHIFTED 1  /* stored as f_i - w_i*rho0 in FP32, collided in FP32; the
+                               two-level average-down sums and all reductions in FP64 */
 
 typedef struct lbx_ctx lbx_ctx;
 typedef struct lbx_sim lbx_sim;
