@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <map>
 #include <tuple>
@@ -746,16 +747,18 @@ int lbx_sim_create(lbx_ctx* ctx, const int32_t dims[3], const int32_t periodic[3
                    double rho0, lbx_sim** out) {
   if (!ctx || !dims || !periodic || !out) return fail(ctx, LBX_E_ARG, "null argument");
   *out = nullptr;
+  long long cells = 1;
   for (int d = 0; d < 3; ++d) {
-    if (dims[d] < 1) return fail(ctx, LBX_E_ARG, "domain extent %d in direction %d", dims[d], d);
+    if (dims[d] < 1 || dims[d] > (1 << 28))
+      return fail(ctx, LBX_E_ARG, "domain extent %d in direction %d", dims[d], d);
     if (!periodic[d])
       return fail(ctx, LBX_E_UNSUPPORTED, "Currently only periodic boundary conditions allowed.");
+    cells *= dims[d];  // < 2^31 after every factor, so no overflow
+    if (cells >= (1ll << 31)) return fail(ctx, LBX_E_ARG, "domain exceeds 2^31 cells");
   }
-  if ((long long)dims[0] * dims[1] * dims[2] >= (1ll << 31))
-    return fail(ctx, LBX_E_ARG, "domain of %lld cells exceeds 2^31",
-                (long long)dims[0] * dims[1] * dims[2]);
   if (precision != LBX_FP64 && precision != LBX_FP32_SHIFTED)
     return fail(ctx, LBX_E_ARG, "unknown precision %d", precision);
+  if (!std::isfinite(rho0)) return fail(ctx, LBX_E_ARG, "rho0 must be finite");
   lbx_sim* sim = new (std::nothrow) lbx_sim;
   if (!sim) return fail(ctx, LBX_E_NOMEM, "out of host memory");
   sim->ctx = ctx;
@@ -801,6 +804,8 @@ int lbx_level_clear(lbx_sim* sim, int level) {
 int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b) {
   if (int rc = check_sim(sim)) return rc;
   if (level < 0 || level > 1) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
+  if (!std::isfinite(omega_s) || !std::isfinite(omega_b))
+    return fail(sim->ctx, LBX_E_ARG, "relaxation rates must be finite");
   sim->lev[level].omega_s = omega_s;
   sim->lev[level].omega_b = omega_b;
   sim->lev[level].relax_set = true;

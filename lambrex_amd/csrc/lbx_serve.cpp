@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <deque>
@@ -375,12 +376,16 @@ const char* lbx_serve_last_error(const lbx_server* s) { return s ? s->err.c_str(
 
 int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
   if (!s || !req) return LBX_E_ARG;
-  for (int d = 0; d < 3; ++d)
+  const long long max_cells = (long long)(s->staging_bytes / (lbx::NV_TOTAL * sizeof(double)));
+  long long n = 1;
+  for (int d = 0; d < 3; ++d) {
     if (req->dims[d] < 1) return sfail(s, LBX_E_ARG, "dims must be positive");
+    n *= req->dims[d];  // <= max_cells after every factor, so no overflow
+    if (n > max_cells) return sfail(s, LBX_E_ARG, "run exceeds the serving staging size (%lld cells)", max_cells);
+  }
   if (req->steps < 1) return sfail(s, LBX_E_ARG, "steps must be >= 1");
-  const long long n = cells_of(*req);
-  if ((size_t)n * lbx::NV_TOTAL * sizeof(double) > s->staging_bytes)
-    return sfail(s, LBX_E_ARG, "run of %lld cells exceeds the serving staging size", n);
+  if (!(req->tau_s > -0.5) || !(req->tau_b > -0.5) || !std::isfinite(req->tau_s) || !std::isfinite(req->tau_b))
+    return sfail(s, LBX_E_ARG, "tau must be finite and > -1/2");
   const lbx::Lattice L = lbx::make_lattice(req->dims[0], req->dims[1], req->dims[2]);
   if ((size_t)2 * L.elems() * esize(s->precision) > s->arena.size)
     return sfail(s, LBX_E_NOMEM, "run does not fit the serving arena");
@@ -390,6 +395,8 @@ int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
     return sfail(s, LBX_E_ARG, "LBX_INIT_FIELDS needs rho and u");
   if (req->init_kind == LBX_INIT_PLANE_PULSE && (req->axis < 0 || req->axis > 2))
     return sfail(s, LBX_E_ARG, "pulse axis must be 0, 1 or 2");
+  if (req->init_kind == LBX_INIT_PLANE_PULSE && (req->plane < 0 || req->plane >= req->dims[req->axis]))
+    return sfail(s, LBX_E_ARG, "pulse plane %d outside the run", req->plane);
   Request q;
   q.id = s->next_id++;
   q.req = *req;

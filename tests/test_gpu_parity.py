@@ -248,7 +248,12 @@ def test_error_paths(ctx):
     assert rc == lbx.LBX_E_ARG  # box 3 does not exist
     assert b"box 3" in lib.lbx_last_error(ctx.h)
     assert lib.lbx_iterate(sim.h, -1) == lbx.LBX_E_ARG
+    assert lib.lbx_set_relax(sim.h, 0, float("nan"), 1.0) == lbx.LBX_E_ARG
     sim.close()
+    for dims in ((1 << 30, 1, 1), (1 << 20, 1 << 20, 1 << 20), (0, 4, 4)):  # no overflow, no allocation
+        with pytest.raises(lbx.LbxError) as e:
+            lbx.Sim(ctx, dims)
+        assert e.value.code == lbx.LBX_E_ARG
 
 
 def test_autotuned_shapes_do_not_change_results(ctx):

@@ -106,4 +106,11 @@ def test_bad_requests_are_refused(ctx):
     with pytest.raises(lbx.LbxError) as e:
         srv.submit((128, 128, 128), 5, 0.8)  # 252 MB > 16 MB arena
     assert e.value.code == lbx.LBX_E_NOMEM
+    for dims, steps, tau, plane in (((1 << 30, 1 << 30, 1 << 30), 5, 0.8, 0),  # cell count overflows int64
+                                    ((8, 8, 8), 5, -0.5, 4),                    # omega = 1/(tau + 1/2) infinite
+                                    ((8, 8, 8), 5, float("nan"), 4),
+                                    ((8, 8, 8), 5, 0.8, 8)):                    # pulse plane outside the run
+        with pytest.raises(lbx.LbxError) as e:
+            srv.submit(dims, steps, tau, plane=plane)
+        assert e.value.code == lbx.LBX_E_ARG, (dims, tau, plane)
     srv.close()
