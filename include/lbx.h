@@ -83,6 +83,14 @@ int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
  * Each fine box is stored with 2 ghost layers like the reference's FABs.
  * Defining level 0 again releases level 1. */
 
+/* Replace level 1's boxes, keeping its data (RemakeLevel,
+ * src/AmrSim.cpp:715-744, with DistFnFillPatch, :359-391): every valid cell
+ * of the new boxes takes the old fine populations where an old fine box
+ * covered it, else its coarse parent's (piecewise-constant interpolation).
+ * Box rules as for lbx_level_define; relaxation rates carry over; the fine
+ * mask is not changed. */
+int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
+
 /* Release a level's device storage (ClearLevel, src/AmrSim.cpp:746-751). */
 int lbx_level_clear(lbx_sim* sim, int level);
 

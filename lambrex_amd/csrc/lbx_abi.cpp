@@ -532,7 +532,9 @@ std::vector<unsigned char> coarse_flags(const Level& C, const std::vector<Box>& 
 }
 
 // Fine level: patches grown by 2; ghost-fill sources; average-down table.
-int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
+// Builds a fine level over `boxes` into F (a fresh Level); on failure F holds
+// nothing.
+int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
   lbx_ctx* ctx = sim->ctx;
   Level& C = sim->lev[0];
   if (!C.defined) return fail(ctx, LBX_E_STATE, "define level 0 before level 1");
@@ -545,8 +547,7 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
         return fail(ctx, LBX_E_ARG, "fine box %zu is not aligned to refinement ratio 2", b);
   }
   if (any_overlap(boxes)) return fail(ctx, LBX_E_ARG, "fine boxes overlap");
-  Level& F = sim->lev[1];
-  free_level(F);
+  F = Level{};
   F.boxes = boxes;
   F.patched = true;
   long long base = 0, fill_base = 0, avg_base = 0;
@@ -558,7 +559,10 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
       d.g[k] = b.n(k) + 4;
     }
     const long long cells = (long long)d.g[0] * d.g[1] * d.g[2];
-    if (cells >= (1ll << 31)) return fail(ctx, LBX_E_ARG, "fine box too large");
+    if (cells >= (1ll << 31) || fill_base + cells >= (1ll << 32)) {
+      F = Level{};
+      return fail(ctx, LBX_E_ARG, "fine box too large");
+    }
     d.cells = (int)cells;
     d.qs = (cells + 63) / 64 * 64;
     d.base = base;
@@ -572,7 +576,10 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
     F.max_blocks = std::max(F.max_blocks, blocks);
     F.desc.push_back(d);
   }
-  if (avg_base >= (1ll << 31)) return fail(ctx, LBX_E_ARG, "fine level too large");
+  if (avg_base >= (1ll << 31)) {
+    F = Level{};
+    return fail(ctx, LBX_E_ARG, "fine level too large");
+  }
   F.avg_stride = (avg_base + 63) / 64 * 64;
   const size_t bytes = (size_t)base * elem_size(sim->precision);
   {
@@ -688,6 +695,75 @@ int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
   return LBX_OK;
 }
 
+int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
+  Level nf;
+  if (int rc = build_level1(sim, boxes, nf)) return rc;
+  free_level(sim->lev[1]);
+  sim->lev[1] = nf;
+  return LBX_OK;
+}
+
+// RemakeLevel (src/AmrSim.cpp:715-744): a new fine BoxArray whose valid cells
+// take the old fine level's values where an old fine box covers them and the
+// coarse parent's otherwise (DistFnFillPatch: FillPatchTwoLevels with
+// piecewise-constant interpolation, src/AmrSim.cpp:359-391).  Relaxation
+// rates carry over.
+int remake_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
+  lbx_ctx* ctx = sim->ctx;
+  Level& C = sim->lev[0];
+  Level& F = sim->lev[1];
+  if (!F.defined) return fail(ctx, LBX_E_STATE, "level 1 is not defined");
+  Level nf;
+  if (int rc = build_level1(sim, boxes, nf)) return rc;
+  // Source of every new valid cell: (old patch << 32) | old grown index, or
+  // -1 - coarse element of population 0.
+  std::vector<long long> src(nf.desc.empty() ? 0 : (size_t)(nf.desc.back().fill_base + nf.desc.back().cells), 0);
+  for (size_t b = 0; b < boxes.size(); ++b) {
+    const PatchDesc& P = nf.desc[b];
+    for (int z = P.lo[2]; z <= P.hi[2]; ++z)
+      for (int y = P.lo[1]; y <= P.hi[1]; ++y)
+        for (int x = P.lo[0]; x <= P.hi[0]; ++x)
+          src[P.fill_base + ((long long)(z - P.lo[2] + 2) * P.g[1] + (y - P.lo[1] + 2)) * P.g[0] + (x - P.lo[0] + 2)] =
+              -1 - C.lat.at(0, (y / 2) * C.lat.nx + x / 2, z / 2);
+    for (size_t o = 0; o < F.boxes.size(); ++o) {
+      const PatchDesc& O = F.desc[o];
+      int lo[3], hi[3];
+      bool empty = false;
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::max(P.lo[d], O.lo[d]);
+        hi[d] = std::min(P.hi[d], O.hi[d]);
+        empty |= lo[d] > hi[d];
+      }
+      if (empty) continue;
+      for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+          for (int x = lo[0]; x <= hi[0]; ++x) {
+            const long long li = ((long long)(z - O.lo[2] + 2) * O.g[1] + (y - O.lo[1] + 2)) * O.g[0] + (x - O.lo[0] + 2);
+            src[P.fill_base + ((long long)(z - P.lo[2] + 2) * P.g[1] + (y - P.lo[1] + 2)) * P.g[0] +
+                (x - P.lo[0] + 2)] = ((long long)o << 32) | li;
+          }
+    }
+  }
+  DeviceTemp t;
+  hipError_t e = hipMalloc(&t.p, src.size() * sizeof(long long));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(t.p, src.data(), src.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = lbx::launch_fine_remake(sim->precision, nf.buf[0], F.buf[F.cur], C.buf[C.cur], C.lat, nf.d_desc,
+                                (int)nf.desc.size(), nf.max_cells, F.d_desc, (const long long*)t.p, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    free_level(nf);
+    return hip_fail(ctx, e, "level-1 remake");
+  }
+  nf.omega_s = F.omega_s;
+  nf.omega_b = F.omega_b;
+  nf.relax_set = F.relax_set;
+  free_level(F);
+  F = nf;
+  return LBX_OK;
+}
+
 }  // namespace
 
 int lbx::ctx_device(const lbx_ctx* ctx) { return ctx->device; }
@@ -793,6 +869,18 @@ int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) 
   const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
   sim->graphs.clear();
   return level == 0 ? define_level0(sim, boxes) : define_level1(sim, boxes);
+}
+
+int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) {
+  if (int rc = check_sim(sim)) return rc;
+  lbx_ctx* ctx = sim->ctx;
+  if (level != 1) return fail(ctx, LBX_E_UNSUPPORTED, "only level 1 is remade");
+  if (n_boxes < 1 || !lohi) return fail(ctx, LBX_E_ARG, "need at least one box");
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
+  sim->graphs.clear();
+  return remake_level1(sim, boxes);
 }
 
 int lbx_level_clear(lbx_sim* sim, int level) {

@@ -141,6 +141,12 @@ hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stri
 hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, const Lattice& clat,
                                    const PatchDesc* patches, int npatch, int max_cells, double rho0,
                                    hipStream_t st);
+// Valid cells of new patches from old patches or the coarse parent
+// (src per new grown cell: (old patch << 32) | old grown index, or -1 -
+// coarse element of population 0).
+hipError_t launch_fine_remake(int precision, void* fnew, const void* fold, const void* cnow, const Lattice& clat,
+                              const PatchDesc* patches, int npatch, int max_cells, const PatchDesc* old_patches,
+                              const long long* src, hipStream_t st);
 // op 0: pack valid cells of patch b into FP64 f a[15][n]; 1: unpack; 2: hydro
 // into a (rho), u.
 hipError_t launch_patch_io(int precision, int op, void* f, const PatchDesc* patches, int b, int valid_cells,

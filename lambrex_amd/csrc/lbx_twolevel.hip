@@ -324,6 +324,37 @@ __global__ __launch_bounds__(256) void k_fine_from_coarse(S* __restrict__ fnow, 
   }
 }
 
+// RemakeLevel's fill (src/AmrSim.cpp:715-744 via DistFnFillPatch): each
+// valid cell of the new patches copies its populations from an old fine
+// patch (src >= 0: (old patch << 32) | old grown index) or from its coarse
+// parent (src < 0: -1 - coarse element of population 0).
+template <typename S>
+__global__ __launch_bounds__(256) void k_fine_remake(S* __restrict__ fnew, const S* __restrict__ fold,
+                                                     const S* __restrict__ cnow, Lattice clat,
+                                                     const PatchDesc* __restrict__ patches,
+                                                     const PatchDesc* __restrict__ old_patches,
+                                                     const long long* __restrict__ src) {
+  const PatchDesc P = patches[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.cells) return;
+  int gx, gy, gz;
+  patch_coords(P, i, gx, gy, gz);
+  if (!patch_valid(P, gx, gy, gz)) return;
+  const long long d = src[P.fill_base + i];
+  const S* from;
+  long long stride;
+  if (d >= 0) {
+    const PatchDesc& O = old_patches[d >> 32];
+    from = fold + O.base + (d & 0xffffffffll);
+    stride = O.qs;
+  } else {
+    from = cnow + (-1 - d);
+    stride = clat.ps;
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) fnew[P.base + q * P.qs + i] = from[q * stride];
+}
+
 // Valid cells of one patch <-> packed FP64 f [15][nz][ny][nx]; hydro of one
 // patch.
 template <int OP, typename S>  // 0 pack, 1 unpack, 2 hydro
@@ -449,6 +480,19 @@ hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, 
     k_fine_from_coarse<double><<<grid, 256, 0, st>>>((double*)fnow, (const double*)cnow, clat, patches, rho0);
   else
     k_fine_from_coarse<float><<<grid, 256, 0, st>>>((float*)fnow, (const float*)cnow, clat, patches, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_remake(int precision, void* fnew, const void* fold, const void* cnow, const Lattice& clat,
+                              const PatchDesc* patches, int npatch, int max_cells, const PatchDesc* old_patches,
+                              const long long* src, hipStream_t st) {
+  const dim3 grid((max_cells + 255) / 256, npatch);
+  if (precision == 0)
+    k_fine_remake<double><<<grid, 256, 0, st>>>((double*)fnew, (const double*)fold, (const double*)cnow, clat,
+                                                patches, old_patches, src);
+  else
+    k_fine_remake<float><<<grid, 256, 0, st>>>((float*)fnew, (const float*)fold, (const float*)cnow, clat, patches,
+                                               old_patches, src);
   return hipGetLastError();
 }
 

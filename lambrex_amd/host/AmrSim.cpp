@@ -253,6 +253,27 @@ void AmrSim::MakeNewLevelFromCoarse(int level, double time, const std::vector<la
   MakeFineMask(level - 1);
 }
 
+// src/AmrSim.cpp:715-744: new boxes for the existing fine level; each new
+// valid cell keeps the old fine populations where an old fine box covered
+// it and takes its coarse parent's otherwise (DistFnFillPatch, :359-391).
+void AmrSim::RemakeLevel(int level, double time, const std::vector<lambrex::IBox>& ba) {
+  if (level != 1 || finest_level_ < 1) throw lambrex::Error("RemakeLevel: level 1 does not exist");
+  LevelHost& L = levels_.at(level);
+  const std::vector<int32_t> lohi = lambrex::to_lohi(ba);
+  Check(lbx_level_remake(sim_, level, (int)ba.size(), lohi.data()));
+  L.boxes = ba;
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (int s = 0; s < 2; ++s) {
+    L.rho[s].assign(ba.size(), {});
+    for (size_t b = 0; b < ba.size(); ++b) L.rho[s][b].assign(ba[b].numPts(), nan);
+  }
+  L.vel.assign(ba.size(), {});
+  for (size_t b = 0; b < ba.size(); ++b) L.vel[b].assign(3 * ba[b].numPts(), nan);
+  L.cur = 0;
+  L.time = time;
+  CalcHydroVars(level);
+}
+
 // src/AmrSim.cpp:419-428.  The reference builds the mask from the coarse
 // level's own BoxArray (its fba argument), i.e. the coarse cells covered by
 // coarsen(coarse boxes, 2); SetFineMaskFromFineGrids(true) uses the fine
@@ -285,11 +306,15 @@ void AmrSim::SetStaticRefinement(int level, const std::array<int, NDIMS>& lo_cor
   }
   if (!tag.ok()) throw lambrex::Error("SetStaticRefinement: empty refinement region");
   static_tag_ = tag;
-  if (finest_level_ >= 1) ClearLevel(1);
-  finest_level_ = 0;
   const std::vector<lambrex::IBox> fba =
       lambrex::max_size_chop(lambrex::refine(tag, kRefRatio), max_grid_size_);
-  MakeNewLevelFromCoarse(1, GetTime(0), fba);
+  // AmrCore::regrid: a new level is made from the coarse one, an existing
+  // level with a different BoxArray is remade (keeping its data), an
+  // unchanged one is left alone.
+  if (finest_level_ < 1)
+    MakeNewLevelFromCoarse(1, GetTime(0), fba);
+  else if (!(fba == levels_.at(1).boxes))
+    RemakeLevel(1, GetTime(0), fba);
   MakeFineMask(level);
 }
 

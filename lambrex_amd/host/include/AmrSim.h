@@ -121,6 +121,7 @@ class AmrSim {
   void ComputeDt(int level);
   void MakeNewLevelFromScratch(int level, double time, const std::vector<lambrex::IBox>& ba);
   void MakeNewLevelFromCoarse(int level, double time, const std::vector<lambrex::IBox>& ba);
+  void RemakeLevel(int level, double time, const std::vector<lambrex::IBox>& ba);
   void MakeFineMask(int coarse_level);
   void ClearLevel(int level);
   void InitDensity(int level);

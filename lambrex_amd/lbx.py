@@ -34,6 +34,7 @@ SIGNATURES = {
     "lbx_sim_create": (ctypes.c_int, [_vp, _i32p, _i32p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(_vp)]),
     "lbx_sim_destroy": (ctypes.c_int, [_vp]),
     "lbx_level_define": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
+    "lbx_level_remake": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
     "lbx_level_clear": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_set_relax": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "lbx_set_fine_mask": (ctypes.c_int, [_vp, ctypes.c_int, _i32p]),
@@ -203,6 +204,13 @@ class Sim:
     def define_level(self, level, boxes):
         lohi = _lohi(boxes)
         self.ctx.check(self._lib.lbx_level_define(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))
+        self.boxes[level] = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+
+    def remake_level(self, level, boxes):
+        """New boxes for level 1, keeping its data (old fine values where an
+        old box covered the cell, else the coarse parent's)."""
+        lohi = _lohi(boxes)
+        self.ctx.check(self._lib.lbx_level_remake(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))
         self.boxes[level] = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
 
     def set_relax(self, level, omega_s, omega_b):

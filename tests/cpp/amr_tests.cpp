@@ -2,6 +2,8 @@
 // (tests/catch2AMRTests.cpp:118-423) against this build's AmrSim facade.
 // Sections that inspect AMReX TagBoxArrays (ErrorEst) have no counterpart
 // here: tagging is reduced to the static refinement box.
+#include <vector>
+
 #include "check.h"
 #include "lambrex.h"
 
@@ -82,6 +84,41 @@ TEST_CASE("TwoLevel Iterate advances both levels") {
   sim->CalcHydroVars(1);
   REQUIRE(sim->OnProcessDensity(sim->GetDensity(16, 8, 4, 1)));
   REQUIRE(!sim->OnProcessDensity(sim->GetDensity(0, 0, 0, 1)));  // not refined there
+}
+
+// A second SetStaticRefinement with another region remakes the fine level
+// (AmrCore::regrid -> RemakeLevel, src/AmrSim.cpp:715-744): cells refined
+// before and after keep their fine data, newly refined cells take the
+// coarse parent's state.
+TEST_CASE("TwoLevel re-refinement keeps fine data") {
+  std::unique_ptr<AmrSim> sim(make());
+  sim->SetStaticRefinement(0, {8, 4, 2}, {23, 15, 9});
+  sim->Iterate(3);
+  sim->CalcHydroVars(0);
+  sim->CalcHydroVars(1);
+  std::vector<double> before;
+  for (int k = 8; k <= 19; ++k)
+    for (int j = 8; j <= 31; ++j)
+      for (int i = 24; i <= 47; ++i) before.push_back(sim->GetDensity(i, j, k, 1));
+  const auto fine_before = sim->boxArray(1);
+  sim->SetStaticRefinement(0, {12, 4, 4}, {31, 15, 9});
+  REQUIRE(sim->finestLevel() == 1);
+  REQUIRE(!(sim->boxArray(1) == fine_before));
+  REQUIRE(sim->GetTimeStep(1) == 6);
+  size_t n = 0;
+  bool all_same = true;
+  for (int k = 8; k <= 19; ++k)
+    for (int j = 8; j <= 31; ++j)
+      for (int i = 24; i <= 47; ++i) all_same &= sim->GetDensity(i, j, k, 1) == before[n++];
+  REQUIRE(all_same);  // refined before and after: the old fine values
+  bool parent = true;
+  for (int k = 8; k <= 19; ++k)
+    for (int j = 8; j <= 31; ++j)
+      for (int i = 48; i <= 63; ++i) parent &= sim->GetDensity(i, j, k, 1) == sim->GetDensity(i / 2, j / 2, k / 2, 0);
+  REQUIRE(parent);  // newly refined: the coarse parent's density
+  REQUIRE(!sim->OnProcessDensity(sim->GetDensity(16, 8, 4, 1)));  // no longer refined
+  sim->Iterate(1);
+  REQUIRE(sim->GetTimeStep(1) == 8);
 }
 
 TEST_CASE("TwoLevel UnsetStaticRefinement") {

@@ -253,3 +253,54 @@ def test_fp32_two_level_tracks_fp64(ctx):
     assert np.isfinite(s0).all() and np.isfinite(s1).all()
     assert e_rho <= 1e-6
     assert e_u <= 1e-3
+
+
+def test_remake_level_keeps_fine_data_then_matches_oracle(ctx):
+    """lbx_level_remake (RemakeLevel, src/AmrSim.cpp:715-744): new valid
+    cells covered by an old fine box keep its populations, the others take
+    their coarse parent's; stepping on from there equals the oracle started
+    from the same coarse and fine populations, bitwise."""
+    dims = (16, 8, 8)
+    rho, u = workloads.random_state(dims, seed=61, rho_sigma=1e-2, u_amp=0.02)
+    cboxes = workloads.chop(dims, 8)
+    old = fine_boxes_in((8, 0, 0), (15, 15, 15), 8)
+    new = fine_boxes_in((12, 0, 4), (23, 15, 11), 4)  # overlaps the old region and extends it
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, 0.8)
+        sim.init_from_fields(rho, u)
+        sim.define_level(1, old)
+        sim.set_tau(1, 1.1)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(old)
+        sim.iterate(3)
+        f_old, lo_old = sim.get_f_bbox(1)
+        fc = sim.get_f(0)
+        sim.remake_level(1, new)
+        sim.set_fine_mask(new)
+        f_new, lo_new = sim.get_f_bbox(1)
+        for (blo, bhi) in new:
+            for z in range(blo[2], bhi[2] + 1):
+                for y in range(blo[1], bhi[1] + 1):
+                    for x in range(blo[0], bhi[0] + 1):
+                        got = f_new[:, z - lo_new[2], y - lo_new[1], x - lo_new[0]]
+                        if any(all(a[d] <= p <= b[d] for d, p in enumerate((x, y, z))) for a, b in old):
+                            want = f_old[:, z - lo_old[2], y - lo_old[1], x - lo_old[0]]
+                        else:
+                            want = fc[:, z // 2, y // 2, x // 2]
+                        assert (got.view(np.uint64) == want.view(np.uint64)).all(), (x, y, z)
+        sim.iterate(2)
+        fg0 = sim.get_f(0)
+        fg1, _ = sim.get_f_bbox(1)
+    finally:
+        sim.close()
+    o = orc.Sim(*dims, 0.8, boxes=[(list(a), list(b)) for a, b in cboxes])
+    o.init(*c_order(rho, u))
+    o.set_f(fc, 0)
+    o.add_fine([(list(a), list(b)) for a, b in new], [(list(a), list(b)) for a, b in new])
+    o.set_f(f_new, 1)
+    o.iterate(2)
+    assert_bitwise(o.get_f(0), fg0, "coarse f after remake + 2 steps")
+    assert_bitwise(o.get_f(1), fg1, "fine f after remake + 2 steps")
+    o.close()
