@@ -91,6 +91,28 @@ int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
  * mask is not changed. */
 int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
 
+/* ErrorEst on device data.  The reference tags only a static box
+ * (TagCell, src/AmrSim.cpp:413-417, via ErrorEst :633-663); these tags come
+ * from the state.  tags[(z*ny + y)*nx + x] = 1 for the cells of level 0
+ * where criterion LBX_TAG_DENSITY_GRADIENT: |grad rho|^2 > threshold^2
+ * (central differences, periodic), or LBX_TAG_SPEED: |u|^2 > threshold^2,
+ * with rho and u exactly as lbx_hydro returns them.  Synchronous. */
+#define LBX_TAG_DENSITY_GRADIENT 0
+#define LBX_TAG_SPEED 1
+int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint8_t* tags);
+
+/* Cluster level-0 tags (as from lbx_tag_cells, domain dims) into fine boxes
+ * for lbx_level_define/lbx_level_remake: tags grown by `buffer` cells
+ * (periodic), tiles of `blocking` coarse cells refined when they hold a
+ * grown tag, tagged tiles merged into non-overlapping boxes, refined by 2
+ * and chopped at max_grid_size (even) with AMReX's BoxList::maxSize rule.
+ * Replaces AMReX's clustering in AmrCore::regrid with a simpler,
+ * deterministic one.  *n_boxes = number of boxes; with max_boxes = 0 only
+ * the count is returned; LBX_E_ARG if lohi (6 per box) is too small.
+ * Host only: needs no context or GPU. */
+int lbx_cluster_tags(const int32_t dims[3], const uint8_t* tags, int32_t blocking, int32_t buffer,
+                     int32_t max_grid_size, int32_t max_boxes, int32_t* lohi, int32_t* n_boxes);
+
 /* Release a level's device storage (ClearLevel, src/AmrSim.cpp:746-751). */
 int lbx_level_clear(lbx_sim* sim, int level);
 

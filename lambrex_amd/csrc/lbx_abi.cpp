@@ -883,6 +883,27 @@ int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) 
   return remake_level1(sim, boxes);
 }
 
+int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint8_t* tags) {
+  if (int rc = check_sim(sim)) return rc;
+  lbx_ctx* ctx = sim->ctx;
+  if (level != 0) return fail(ctx, LBX_E_UNSUPPORTED, "cells of level 0 are tagged");
+  if (!tags) return fail(ctx, LBX_E_ARG, "null tag buffer");
+  if (criterion != LBX_TAG_DENSITY_GRADIENT && criterion != LBX_TAG_SPEED)
+    return fail(ctx, LBX_E_ARG, "unknown tagging criterion %d", criterion);
+  if (!std::isfinite(threshold) || threshold < 0.0) return fail(ctx, LBX_E_ARG, "threshold must be finite, >= 0");
+  const Level& L = sim->lev[0];
+  if (!L.defined) return fail(ctx, LBX_E_STATE, "level 0 is not defined");
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t n = (size_t)L.lat.nx * L.lat.ny * L.lat.nz;
+  DeviceTemp t;
+  LBX_HIP(ctx, hipMalloc(&t.p, n));
+  LBX_HIP(ctx, lbx::launch_tag(sim->precision, criterion, L.buf[L.cur], L.lat, sim->rho0, threshold * threshold,
+                               (unsigned char*)t.p, ctx->stream));
+  LBX_HIP(ctx, hipMemcpyAsync(tags, t.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LBX_OK;
+}
+
 int lbx_level_clear(lbx_sim* sim, int level) {
   if (int rc = check_sim(sim)) return rc;
   if (level < 0 || level > 1) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);

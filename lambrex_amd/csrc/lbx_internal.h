@@ -99,6 +99,10 @@ hipError_t launch_unpack_f(int precision, void* f, const Lattice& L, const Regio
 hipError_t launch_hydro(int precision, const void* f, const Lattice& L, const Region& R,
                         double* rho, double* u, double rho0, hipStream_t st);
 // FP64 sums over the whole lattice: partials[nblocks*4] then out[4].
+// tag[(z*ny + y)*nx + x] = 1 where criterion 0 (|grad rho|^2) or 1 (|u|^2)
+// exceeds thr2.
+hipError_t launch_tag(int precision, int criterion, const void* f, const Lattice& L, double rho0, double thr2,
+                      unsigned char* tag, hipStream_t st);
 hipError_t launch_conservation(int precision, const void* f, const Lattice& L, double rho0,
                                double* partials, int nblocks, double* out, hipStream_t st);
 

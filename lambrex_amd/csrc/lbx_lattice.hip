@@ -222,6 +222,42 @@ __global__ __launch_bounds__(256) void k_hydro(const S* __restrict__ f, Lattice 
   u[2 * n + i] = u2;
 }
 
+// ErrorEst on device data: tag[c] = 1 (c = (z*ny + y)*nx + x) where
+// CRIT 0: |grad rho|^2 > thr2, central differences with periodic wrap;
+// CRIT 1: |u|^2 > thr2.  rho and u are exactly what k_hydro returns.
+template <typename S, int CRIT>
+__global__ __launch_bounds__(256) void k_tag(const S* __restrict__ f, Lattice L, double rho0, double thr2,
+                                             unsigned char* __restrict__ tag) {
+  const long long n = (long long)L.nx * L.ny * L.nz;
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int x = int(c % L.nx), y = int((c / L.nx) % L.ny), z = int(c / ((long long)L.nx * L.ny));
+  auto hydro_at = [&](int xx, int yy, int zz, double& r, double& u0, double& u1, double& u2) {
+    double g[NV];
+    cell_f64<S>(f, L.ps, L.at(0, yy * L.nx + xx, zz), rho0, g);
+    hydro_f64(g, r, u0, u1, u2);
+  };
+  double s;
+  if (CRIT == 0) {
+    double rp, rm, u0, u1, u2;
+    hydro_at(x == L.nx - 1 ? 0 : x + 1, y, z, rp, u0, u1, u2);
+    hydro_at(x == 0 ? L.nx - 1 : x - 1, y, z, rm, u0, u1, u2);
+    const double gx = (rp - rm) * 0.5;
+    hydro_at(x, y == L.ny - 1 ? 0 : y + 1, z, rp, u0, u1, u2);
+    hydro_at(x, y == 0 ? L.ny - 1 : y - 1, z, rm, u0, u1, u2);
+    const double gy = (rp - rm) * 0.5;
+    hydro_at(x, y, z == L.nz - 1 ? 0 : z + 1, rp, u0, u1, u2);
+    hydro_at(x, y, z == 0 ? L.nz - 1 : z - 1, rm, u0, u1, u2);
+    const double gz = (rp - rm) * 0.5;
+    s = gx * gx + gy * gy + gz * gz;
+  } else {
+    double r, u0, u1, u2;
+    hydro_at(x, y, z, r, u0, u1, u2);
+    s = u0 * u0 + u1 * u1 + u2 * u2;
+  }
+  tag[c] = s > thr2 ? 1 : 0;
+}
+
 // Block sum of 4 doubles in a fixed order: wavefront butterfly, then the
 // wavefront partials in index order.
 __device__ __forceinline__ void block_sum4(double v[4], double* out) {
@@ -374,6 +410,24 @@ hipError_t launch_hydro(int precision, const void* f, const Lattice& L, const Re
     k_hydro<double><<<blocks_for(n), 256, 0, st>>>((const double*)f, L, R, rho, u, rho0);
   else
     k_hydro<float><<<blocks_for(n), 256, 0, st>>>((const float*)f, L, R, rho, u, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_tag(int precision, int criterion, const void* f, const Lattice& L, double rho0, double thr2,
+                      unsigned char* tag, hipStream_t st) {
+  const long long n = (long long)L.nx * L.ny * L.nz;
+  const int blocks = int((n + 255) / 256);
+  if (precision == 0) {
+    if (criterion == 0)
+      k_tag<double, 0><<<blocks, 256, 0, st>>>((const double*)f, L, rho0, thr2, tag);
+    else
+      k_tag<double, 1><<<blocks, 256, 0, st>>>((const double*)f, L, rho0, thr2, tag);
+  } else {
+    if (criterion == 0)
+      k_tag<float, 0><<<blocks, 256, 0, st>>>((const float*)f, L, rho0, thr2, tag);
+    else
+      k_tag<float, 1><<<blocks, 256, 0, st>>>((const float*)f, L, rho0, thr2, tag);
+  }
   return hipGetLastError();
 }
 

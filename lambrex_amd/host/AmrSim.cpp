@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <limits>
 
 namespace {
@@ -251,6 +252,38 @@ void AmrSim::MakeNewLevelFromCoarse(int level, double time, const std::vector<la
   finest_level_ = level;
   CalcHydroVars(level);
   MakeFineMask(level - 1);
+}
+
+// ErrorEst from the state (lbx_tag_cells) + clustering (lbx_cluster_tags)
+// + the level update of AmrCore::regrid.  No reference counterpart: the
+// reference tags only its static box (TagCell, src/AmrSim.cpp:413-417).
+void AmrSim::Regrid(int criterion, double threshold, int blocking, int buffer) {
+  if (!sim_) throw lambrex::Error("Regrid before InitFromScratch");
+  if (max_level_ < 1) throw lambrex::Error("Regrid: max_level must be >= 1");
+  std::vector<uint8_t> tags((size_t)NUMEL);
+  Check(lbx_tag_cells(sim_, 0, criterion, threshold, tags.data()));
+  const int32_t dims[3] = {NX, NY, NZ};
+  int32_t n = 0;
+  Check(lbx_cluster_tags(dims, tags.data(), blocking, buffer, max_grid_size_, 0, nullptr, &n));
+  std::vector<int32_t> lohi(6 * (size_t)std::max(n, 1));
+  if (n > 0) Check(lbx_cluster_tags(dims, tags.data(), blocking, buffer, max_grid_size_, n, lohi.data(), &n));
+  std::vector<lambrex::IBox> fba((size_t)n);
+  for (int b = 0; b < n; ++b)
+    for (int d = 0; d < NDIMS; ++d) {
+      fba[b].lo[d] = lohi[6 * b + d];
+      fba[b].hi[d] = lohi[6 * b + 3 + d];
+    }
+  static_tag_ = lambrex::IBox{};
+  if (fba.empty()) {
+    if (finest_level_ >= 1) ClearLevel(1);
+    finest_level_ = 0;
+    return;
+  }
+  if (finest_level_ < 1)
+    MakeNewLevelFromCoarse(1, GetTime(0), fba);
+  else if (!(fba == levels_.at(1).boxes))
+    RemakeLevel(1, GetTime(0), fba);
+  MakeFineMask(0);
 }
 
 // src/AmrSim.cpp:715-744: new boxes for the existing fine level; each new

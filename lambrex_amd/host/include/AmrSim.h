@@ -90,6 +90,12 @@ class AmrSim {
   // Mask the coarse collide with the fine BoxArray instead of the coarse one
   // the reference passes to makeFineMask (src/AmrSim.cpp:419-428).
   void SetFineMaskFromFineGrids(bool on);
+  // Dynamic regrid on device data: tag the level-0 cells where `criterion`
+  // (LBX_TAG_DENSITY_GRADIENT or LBX_TAG_SPEED) exceeds `threshold`, grow
+  // the tags by `buffer` cells, refine the tiles of `blocking` coarse cells
+  // that hold one, and make, remake (keeping fine data) or clear level 1
+  // like AmrCore::regrid.  Replaces any static refinement box.
+  void Regrid(int criterion, double threshold, int blocking = 4, int buffer = 1);
   lbx_sim* Device() const { return sim_; }
   // Populations of the current state of one box, [15][nz][ny][nx] FP64.
   std::vector<double> DistFn(int level, int box) const;

@@ -36,6 +36,9 @@ SIGNATURES = {
     "lbx_level_define": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
     "lbx_level_remake": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
     "lbx_level_clear": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbx_tag_cells": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]),
+    "lbx_cluster_tags": (ctypes.c_int, [_i32p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_int32, _i32p, _i32p]),
     "lbx_set_relax": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "lbx_set_fine_mask": (ctypes.c_int, [_vp, ctypes.c_int, _i32p]),
     "lbx_fill_from_coarse": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -68,6 +71,8 @@ class Result(ctypes.Structure):
                 ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
 
 
+LBX_TAG_DENSITY_GRADIENT = 0
+LBX_TAG_SPEED = 1
 LBX_INIT_PLANE_PULSE = 0
 LBX_INIT_FIELDS = 1
 
@@ -180,6 +185,28 @@ class Context:
         self.close()
 
 
+def cluster_tags(dims, tags, blocking=4, buffer=1, max_grid_size=32):
+    """Fine boxes ((lo), (hi)) covering the tagged cells of a level-0 tag
+    array tags[nz][ny][nx] (lbx_cluster_tags; host only, no GPU)."""
+    lib = load()
+    d = np.array([int(x) for x in dims], dtype=np.int32)
+    t = np.ascontiguousarray(tags, dtype=np.uint8).reshape(-1)
+    if t.size != int(np.prod(d)):
+        raise LbxError(LBX_E_ARG, "tag array does not match dims")
+    n = ctypes.c_int32()
+    rc = lib.lbx_cluster_tags(_ptr(d, ctypes.c_int32), t.ctypes.data_as(ctypes.c_void_p), int(blocking),
+                              int(buffer), int(max_grid_size), 0, None, ctypes.byref(n))
+    if rc != LBX_OK:
+        raise LbxError(rc, "lbx_cluster_tags failed")
+    out = np.zeros(6 * max(n.value, 1), dtype=np.int32)
+    rc = lib.lbx_cluster_tags(_ptr(d, ctypes.c_int32), t.ctypes.data_as(ctypes.c_void_p), int(blocking),
+                              int(buffer), int(max_grid_size), n.value, _ptr(out, ctypes.c_int32), ctypes.byref(n))
+    if rc != LBX_OK:
+        raise LbxError(rc, "lbx_cluster_tags failed")
+    return [(tuple(int(v) for v in out[6 * b:6 * b + 3]), tuple(int(v) for v in out[6 * b + 3:6 * b + 6]))
+            for b in range(n.value)]
+
+
 class Sim:
     """A periodic D3Q15 simulation held on the GPU."""
 
@@ -212,6 +239,14 @@ class Sim:
         lohi = _lohi(boxes)
         self.ctx.check(self._lib.lbx_level_remake(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))
         self.boxes[level] = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+
+    def tag_cells(self, criterion, threshold, level=0):
+        """Refinement tags of level 0, [nz][ny][nx] uint8 (lbx_tag_cells)."""
+        nx, ny, nz = self.dims
+        tags = np.zeros((nz, ny, nx), dtype=np.uint8)
+        self.ctx.check(self._lib.lbx_tag_cells(self.h, level, int(criterion), float(threshold),
+                                               tags.ctypes.data_as(ctypes.c_void_p)))
+        return tags
 
     def set_relax(self, level, omega_s, omega_b):
         self.ctx.check(self._lib.lbx_set_relax(self.h, level, float(omega_s), float(omega_b)))

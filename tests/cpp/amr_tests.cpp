@@ -121,6 +121,40 @@ TEST_CASE("TwoLevel re-refinement keeps fine data") {
   REQUIRE(sim->GetTimeStep(1) == 8);
 }
 
+// Regrid from the state: a density pulse on one plane is refined, the fine
+// level starts from its coarse parents, a higher threshold remakes a
+// smaller level, an unreachable one removes it.
+TEST_CASE("TwoLevel Regrid from a density gradient") {
+  AmrSim sim(NX, NY, NZ, MAX_LEVEL, {1, 1, 1}, 0.8, 0.8);
+  std::vector<double> rho((size_t)NX * NY * NZ, 1.0);
+  for (int i = 0; i < NX; ++i)
+    for (int j = 0; j < NY; ++j)
+      for (int k = 0; k < NZ; ++k)
+        if (i == 20) rho[((size_t)i * NY + j) * NZ + k] = 1.01;  // C order, i slowest
+  sim.SetInitialDensity(rho);
+  sim.SetInitialVelocity(0.0);
+  sim.InitFromScratch(0.0);
+  sim.Regrid(LBX_TAG_DENSITY_GRADIENT, 1e-3);
+  REQUIRE(sim.finestLevel() == 1);
+  // the tagged cells are x = 19 and 21 (central differences); tiles of 4 with
+  // a buffer of 1 refine x in [16, 24) at every y and z
+  auto ext = sim.GetExtent(1);
+  REQUIRE(ext.first[0] == 32);
+  REQUIRE(ext.second[0] == 47);
+  REQUIRE(ext.first[1] == 0);
+  REQUIRE(ext.second[1] == 2 * NY - 1);
+  sim.CalcHydroVars(0);
+  bool parent = true;
+  for (int i = 32; i <= 47; ++i) parent &= sim.GetDensity(i, 5, 3, 1) == sim.GetDensity(i / 2, 2, 1, 0);
+  REQUIRE(parent);
+  sim.Iterate(4);
+  REQUIRE(sim.GetTimeStep(1) == 8);
+  sim.Regrid(LBX_TAG_DENSITY_GRADIENT, 1e9);
+  REQUIRE(sim.finestLevel() == 0);
+  REQUIRE(sim.boxArray(1).empty());
+  sim.Iterate(1);
+}
+
 TEST_CASE("TwoLevel UnsetStaticRefinement") {
   std::unique_ptr<AmrSim> sim(make());
   sim->SetStaticRefinement(0, {8, 4, 2}, {23, 15, 9});
