@@ -193,7 +193,7 @@ int lbx_bench_copy(lbx_ctx* ctx, size_t bytes, int reps, double* gbs);
  * staging buffer: at most 2,236,962 cells; 128^3 fits) with its own tau,
  * initial state and step count.  Requests are admitted in arrival order
  * while the server's device arena has room; all admitted runs advance
- * together, one persistent-kernel launch per step over the patch list of
+ * together, one patch-list kernel launch per step over the work items of
  * every active run.  Each run computes exactly what lbx_iterate computes for
  * it alone, bit for bit. */
 

@@ -1,11 +1,13 @@
 // lbx_serve.hip -- kernels of the batched serving path.
 //
 // Every admitted run is an independent periodic lattice (nx, ny, nz) in the
-// serving arena.  One launch advances every active run by one pass: a
-// persistent grid walks the patch list (the work items of all runs: one item
-// = 256 consecutive cells, x fastest, of one run, so every lane has a cell
-// whatever the run's shape) with a static stride, so every block ends after
-// a bounded number of items.  Each run's pass kind
+// serving arena.  One launch advances every active run by one pass over the
+// patch list (the work items of all runs: one item = 256 consecutive cells,
+// x fastest, of one run, so every lane has a cell whatever the run's shape).
+// The launch normally has one workgroup per item, with resident workgroups
+// per CU capped as for the stream-collide kernels; the loop below walks
+// further items with a static stride when the grid is smaller than the list,
+// so every workgroup ends after a bounded number of items.  Each run's pass kind
 // follows from the global step: pass p = step - start is collide-only for
 // p = 0, fused pull + collide for 0 < p < steps, stream-only for p = steps
 // (the same S (C S)^(n-1) C association as lbx_iterate), reading buffer
@@ -113,10 +115,12 @@ hipError_t launch_serve_step(int precision, const ServeBatch& b, long long step,
                              hipStream_t st) {
   if (b.total_items <= 0) return hipSuccess;
   const int blocks = b.total_items < grid ? b.total_items : grid;
+  // resident workgroups per CU capped like the stream-collide launches
+  const int lds = lds_bytes_for_cap(precision == 0 ? 3 : 4);
   if (precision == 0)
-    k_serve_step<double><<<blocks, 256, 0, st>>>(b, step);
+    k_serve_step<double><<<blocks, 256, lds > 0 ? lds : 0, st>>>(b, step);
   else
-    k_serve_step<float><<<blocks, 256, 0, st>>>(b, step);
+    k_serve_step<float><<<blocks, 256, lds > 0 ? lds : 0, st>>>(b, step);
   return hipGetLastError();
 }
 
