@@ -154,6 +154,12 @@ int lbx_iterate(lbx_sim* sim, int nsteps);
  * src/AmrSim.cpp:938-979).  Synchronous. */
 int lbx_hydro(lbx_sim* sim, int level, int box, double* rho, double* u);
 
+/* Density and velocity of every box of a level in one call (the bulk
+ * readback CalcHydroVars needs, src/AmrSim.cpp:938-979): boxes in
+ * definition order, each box's cells x fastest; rho holds sum(n_b) values,
+ * u holds, per box, its [3][n_b] block.  Synchronous. */
+int lbx_hydro_level(lbx_sim* sim, int level, double* rho, double* u);
+
 /* FP64 global sums over a level's valid cells: out[0] = sum rho,
  * out[1..3] = sum rho*u.  No reference counterpart (the per-level mass at
  * src/AmrSim.cpp:312,318 is never used); deterministic order. */

@@ -1069,6 +1069,37 @@ int lbx_hydro(lbx_sim* sim, int level, int box, double* rho, double* u) {
   return LBX_OK;
 }
 
+int lbx_hydro_level(lbx_sim* sim, int level, double* rho, double* u) {
+  if (int rc = check_sim(sim)) return rc;
+  lbx_ctx* ctx = sim->ctx;
+  if (level < 0 || level > 1) return fail(ctx, LBX_E_ARG, "level %d out of range", level);
+  Level& L = sim->lev[level];
+  if (!L.defined) return fail(ctx, LBX_E_STATE, "level %d is not defined", level);
+  if (!rho || !u) return fail(ctx, LBX_E_ARG, "null buffer");
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  size_t total = 0;
+  for (const Box& b : L.boxes) total += (size_t)b.cells();
+  DeviceTemp t;
+  LBX_HIP(ctx, hipMalloc(&t.p, total * 4 * sizeof(double)));
+  double* drho = (double*)t.p;
+  double* du = drho + total;
+  size_t off = 0;
+  for (size_t b = 0; b < L.boxes.size(); ++b) {
+    const size_t n = (size_t)L.boxes[b].cells();
+    if (L.patched)
+      LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 2, L.buf[L.cur], L.d_desc, (int)b, (int)n, drho + off,
+                                        du + 3 * off, sim->rho0, ctx->stream));
+    else
+      LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[b]), drho + off,
+                                     du + 3 * off, sim->rho0, ctx->stream));
+    off += n;
+  }
+  LBX_HIP(ctx, hipMemcpyAsync(rho, drho, total * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LBX_HIP(ctx, hipMemcpyAsync(u, du, 3 * total * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LBX_OK;
+}
+
 int lbx_conservation(lbx_sim* sim, int level, double out[4]) {
   if (int rc = check_sim(sim)) return rc;
   if (!out) return fail(sim->ctx, LBX_E_ARG, "null buffer");

@@ -2,10 +2,15 @@
 // the C ABI; all distribution-function work happens on the GPU.
 #include "AmrSim.h"
 
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <limits>
+#include <sstream>
 
 namespace {
 constexpr int kRefRatio = 2;  // ratio 2 on every level (src/AmrSim.cpp:766)
@@ -89,10 +94,19 @@ long long AmrSim::LocalIndex(const lambrex::IBox& b, int i, int j, int k) const 
   return ((long long)(k - b.lo[2]) * b.length(1) + (j - b.lo[1])) * b.length(0) + (i - b.lo[0]);
 }
 
+// The reference searches the FABs linearly for every point
+// (src/AmrSim.cpp:824-843); the box of the previous query is tried first, so
+// scans over neighbouring points cost O(1).
 int AmrSim::BoxOf(int level, int i, int j, int k) const {
   const LevelHost& L = levels_.at(level);
+  if (box_hint_.size() != levels_.size()) box_hint_.assign(levels_.size(), 0);
+  int& h = box_hint_[level];
+  if (h < (int)L.boxes.size() && L.boxes[h].contains(i, j, k)) return h;
   for (size_t b = 0; b < L.boxes.size(); ++b)
-    if (L.boxes[b].contains(i, j, k)) return (int)b;
+    if (L.boxes[b].contains(i, j, k)) {
+      h = (int)b;
+      return h;
+    }
   return -1;
 }
 
@@ -181,11 +195,20 @@ void AmrSim::CalcEquilibriumDist(int level) {
     Check(lbx_init_equilibrium(sim_, level, (int)b, L.rho[L.cur][b].data(), L.vel[b].data()));
 }
 
-// src/AmrSim.cpp:938-979
+// src/AmrSim.cpp:938-979, as one bulk readback of the level.
 void AmrSim::CalcHydroVars(int level) {
   LevelHost& L = levels_.at(level);
-  for (size_t b = 0; b < L.boxes.size(); ++b)
-    Check(lbx_hydro(sim_, level, (int)b, L.rho[L.cur][b].data(), L.vel[b].data()));
+  size_t total = 0;
+  for (const lambrex::IBox& b : L.boxes) total += (size_t)b.numPts();
+  std::vector<double> rho(total), u(3 * total);
+  Check(lbx_hydro_level(sim_, level, rho.data(), u.data()));
+  size_t off = 0;
+  for (size_t b = 0; b < L.boxes.size(); ++b) {
+    const size_t n = (size_t)L.boxes[b].numPts();
+    std::copy(rho.begin() + off, rho.begin() + off + n, L.rho[L.cur][b].begin());
+    std::copy(u.begin() + 3 * off, u.begin() + 3 * (off + n), L.vel[b].begin());
+    off += n;
+  }
 }
 
 // src/AmrSim.cpp:824-843
@@ -284,6 +307,92 @@ void AmrSim::Regrid(int criterion, double threshold, int blocking, int buffer) {
   else if (!(fba == levels_.at(1).boxes))
     RemakeLevel(1, GetTime(0), fba);
   MakeFineMask(0);
+}
+
+namespace {
+std::string box_text(const lambrex::IBox& b) {
+  std::ostringstream os;
+  os << "((" << b.lo[0] << ',' << b.lo[1] << ',' << b.lo[2] << ") (" << b.hi[0] << ',' << b.hi[1] << ','
+     << b.hi[2] << ") (0,0,0))";
+  return os.str();
+}
+
+void make_dir(const std::string& d) {
+  if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) throw lambrex::Error("cannot create directory " + d);
+}
+}  // namespace
+
+void AmrSim::WritePlotfile(const std::string& dir) {
+  if (!sim_) throw lambrex::Error("WritePlotfile before InitFromScratch");
+  const int nlev = finest_level_ + 1;
+  for (int l = 0; l < nlev; ++l) CalcHydroVars(l);
+  make_dir(dir);
+  const double time = GetTime(0);
+  std::ofstream h(dir + "/Header");
+  h.precision(17);
+  h << "HyperCLaw-V1.1\n4\ndensity\nx_velocity\ny_velocity\nz_velocity\n3\n" << time << '\n'
+    << finest_level_ << '\n';
+  h << "0 0 0 \n1 1 1 \n";
+  for (int l = 0; l < finest_level_; ++l) h << kRefRatio << ' ';
+  h << '\n';
+  for (int l = 0, r = 1; l < nlev; ++l, r *= kRefRatio) {
+    lambrex::IBox d;
+    d.lo = {0, 0, 0};
+    d.hi = {r * NX - 1, r * NY - 1, r * NZ - 1};
+    h << box_text(d) << ' ';
+  }
+  h << '\n';
+  for (int l = 0; l < nlev; ++l) h << levels_.at(l).step << ' ';
+  h << '\n';
+  for (int l = 0, r = 1; l < nlev; ++l, r *= kRefRatio)
+    h << 1.0 / (r * NX) << ' ' << 1.0 / (r * NY) << ' ' << 1.0 / (r * NZ) << " \n";
+  h << "0\n0\n";
+  for (int l = 0, r = 1; l < nlev; ++l, r *= kRefRatio) {
+    const LevelHost& L = levels_.at(l);
+    const int ext[3] = {r * NX, r * NY, r * NZ};
+    h << l << ' ' << L.boxes.size() << ' ' << time << '\n' << L.step << '\n';
+    for (const lambrex::IBox& b : L.boxes)
+      for (int d = 0; d < 3; ++d) h << double(b.lo[d]) / ext[d] << ' ' << double(b.hi[d] + 1) / ext[d] << '\n';
+    h << "Level_" << l << "/Cell\n";
+    // one FAB per box: "FAB <real descriptor><box> <ncomp>\n" then the data
+    const std::string ldir = dir + "/Level_" + std::to_string(l);
+    make_dir(ldir);
+    std::ofstream dfile(ldir + "/Cell_D_00000", std::ios::binary);
+    std::vector<long long> offsets;
+    std::vector<std::array<double, 4>> mins, maxs;
+    for (size_t b = 0; b < L.boxes.size(); ++b) {
+      offsets.push_back((long long)dfile.tellp());
+      dfile << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_text(L.boxes[b]) << " 4\n";
+      const size_t n = (size_t)L.boxes[b].numPts();
+      const double* comp[4] = {L.rho[L.cur][b].data(), L.vel[b].data(), L.vel[b].data() + n,
+                               L.vel[b].data() + 2 * n};
+      std::array<double, 4> mn, mx;
+      for (int c = 0; c < 4; ++c) {
+        dfile.write(reinterpret_cast<const char*>(comp[c]), (std::streamsize)(n * sizeof(double)));
+        // NaN entries (u where rho = 0) are skipped; all-NaN gives NaN
+        mn[c] = mx[c] = std::numeric_limits<double>::quiet_NaN();
+        for (size_t i = 0; i < n; ++i) {
+          mn[c] = std::fmin(mn[c], comp[c][i]);
+          mx[c] = std::fmax(mx[c], comp[c][i]);
+        }
+      }
+      mins.push_back(mn);
+      maxs.push_back(mx);
+    }
+    if (!dfile) throw lambrex::Error("cannot write " + ldir + "/Cell_D_00000");
+    std::ofstream ch(ldir + "/Cell_H");
+    ch.precision(17);
+    ch << "1\n1\n4\n0\n(" << L.boxes.size() << " 0\n";
+    for (const lambrex::IBox& b : L.boxes) ch << box_text(b) << '\n';
+    ch << ")\n" << L.boxes.size() << '\n';
+    for (long long o : offsets) ch << "FabOnDisk: Cell_D_00000 " << o << '\n';
+    ch << '\n' << L.boxes.size() << ",4\n";
+    for (const auto& m : mins) ch << m[0] << ',' << m[1] << ',' << m[2] << ',' << m[3] << ",\n";
+    ch << '\n' << L.boxes.size() << ",4\n";
+    for (const auto& m : maxs) ch << m[0] << ',' << m[1] << ',' << m[2] << ',' << m[3] << ",\n";
+    if (!ch) throw lambrex::Error("cannot write " + ldir + "/Cell_H");
+  }
+  if (!h) throw lambrex::Error("cannot write " + dir + "/Header");
 }
 
 // src/AmrSim.cpp:715-744: new boxes for the existing fine level; each new

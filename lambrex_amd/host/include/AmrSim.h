@@ -96,6 +96,13 @@ class AmrSim {
   // that hold one, and make, remake (keeping fine data) or clear level 1
   // like AmrCore::regrid.  Replaces any static refinement box.
   void Regrid(int criterion, double threshold, int blocking = 4, int buffer = 1);
+  // Bulk output: density and velocity of every level (CalcHydroVars first)
+  // written to `dir` in AMReX's plotfile layout: a HyperCLaw-V1.1 Header,
+  // Level_<l>/Cell_H (VisMF header) and one Level_<l>/Cell_D_00000 holding
+  // every box as a FAB of 4 components (density, x/y/z velocity),
+  // little-endian IEEE doubles, x fastest.  The problem domain is the
+  // reference's RealBox [0,1]^3.
+  void WritePlotfile(const std::string& dir);
   lbx_sim* Device() const { return sim_; }
   // Populations of the current state of one box, [15][nz][ny][nx] FP64.
   std::vector<double> DistFn(int level, int box) const;
@@ -134,6 +141,7 @@ class AmrSim {
   void InitVelocity(int level);
   void Check(int rc) const;
   int BoxOf(int level, int i, int j, int k) const;
+  mutable std::vector<int> box_hint_;  // last box BoxOf found, per level
   long long LocalIndex(const lambrex::IBox& b, int i, int j, int k) const;
 
   const int NX, NY, NZ, NUMEL;

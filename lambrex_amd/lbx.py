@@ -36,6 +36,7 @@ SIGNATURES = {
     "lbx_level_define": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
     "lbx_level_remake": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
     "lbx_level_clear": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbx_hydro_level": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "lbx_tag_cells": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]),
     "lbx_cluster_tags": (ctypes.c_int, [_i32p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_int32, _i32p, _i32p]),
@@ -297,6 +298,21 @@ class Sim:
         self.ctx.check(self._lib.lbx_hydro(self.h, level, box, _ptr(rho, ctypes.c_double),
                                            _ptr(u, ctypes.c_double)))
         return rho, u
+
+    def hydro_level(self, level=0):
+        """Per-box (rho, u) of every box of a level from one bulk call."""
+        bl = self.boxes[level]
+        sizes = [int(np.prod(self._shape(b))) for b in bl]
+        rho = np.empty(sum(sizes))
+        u = np.empty(3 * sum(sizes))
+        self.ctx.check(self._lib.lbx_hydro_level(self.h, level, _ptr(rho, ctypes.c_double),
+                                                 _ptr(u, ctypes.c_double)))
+        out, off = [], 0
+        for b, n in zip(bl, sizes):
+            shape = self._shape(b)
+            out.append((rho[off:off + n].reshape(shape), u[3 * off:3 * off + 3 * n].reshape((3,) + shape)))
+            off += n
+        return out
 
     def conservation(self, level=0):
         out = np.zeros(4)

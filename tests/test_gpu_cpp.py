@@ -5,6 +5,7 @@ import re
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -66,3 +67,95 @@ def test_amr_pulse_prints_the_reference_goldens():
                     assert abs(float(got) - float(exp)) <= eps * abs(float(exp)), (t, k, n, got, exp)
                     mismatched += got != exp
         assert mismatched == 0, f"t={t}: {mismatched} printed values differ from the golden strings"
+
+
+def read_plotfile(d):
+    """Header and FABs of an AMReX-layout plotfile written by
+    AmrSim::WritePlotfile: {"header": ..., "levels": [[(lo, hi, data[4][nz][ny][nx]), ...], ...]}."""
+    it = iter((d / "Header").read_text().split("\n"))
+    h = {"version": next(it)}
+    ncomp = int(next(it))
+    h["names"] = [next(it) for _ in range(ncomp)]
+    h["dim"] = int(next(it))
+    h["time"] = float(next(it))
+    h["finest"] = int(next(it))
+    h["prob_lo"] = next(it).split()
+    h["prob_hi"] = next(it).split()
+    h["ref"] = next(it).split()
+    h["domains"] = re.findall(r"\(\((\d+),(\d+),(\d+)\) \((\d+),(\d+),(\d+)\) \(0,0,0\)\)", next(it))
+    h["steps"] = [int(v) for v in next(it).split()]
+    h["dx"] = [[float(v) for v in next(it).split()] for _ in range(h["finest"] + 1)]
+    h["coord"] = int(next(it))
+    h["bwidth"] = int(next(it))
+    h["level_boxes"] = []
+    for lev in range(h["finest"] + 1):
+        l, nb, t = next(it).split()
+        assert int(l) == lev and float(t) == h["time"]
+        step = int(next(it))
+        phys = [[[float(v) for v in next(it).split()] for _ in range(3)] for _ in range(int(nb))]
+        path = next(it)
+        h["level_boxes"].append((step, phys, path))
+    levels = []
+    box_re = r"\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\) \(0,0,0\)\)"
+    for lev in range(h["finest"] + 1):
+        ch = (d / f"Level_{lev}" / "Cell_H").read_text()
+        lines = ch.split("\n")
+        assert lines[:4] == ["1", "1", "4", "0"]
+        boxes = [tuple(map(int, m)) for m in re.findall(box_re, ch)]
+        offsets = [int(v) for v in re.findall(r"FabOnDisk: Cell_D_00000 (\d+)", ch)]
+        stats = re.findall(r"^([-0-9.e+,]+),$", ch, re.M)
+        raw = (d / f"Level_{lev}" / "Cell_D_00000").read_bytes()
+        fabs = []
+        for box, off in zip(boxes, offsets):
+            end = raw.index(b"\n", off)
+            head = raw[off:end].decode()
+            assert head.startswith("FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))"), head
+            fb = tuple(map(int, re.search(box_re, head).groups()))
+            assert fb == box and head.endswith(" 4")
+            n = [fb[3 + a] - fb[a] + 1 for a in range(3)]
+            data = np.frombuffer(raw, dtype="<f8", count=4 * n[0] * n[1] * n[2], offset=end + 1)
+            fabs.append((fb[:3], fb[3:], data.reshape(4, n[2], n[1], n[0])))
+        levels.append({"fabs": fabs, "stats": stats, "nboxes": len(boxes)})
+    return h, levels
+
+
+def test_plotfile_round_trip(tmp_path):
+    """AmrSim::WritePlotfile (AMReX plotfile layout) holds the same density
+    and velocity the facade's GetDensity/GetVelocity return, with consistent
+    Header, Cell_H and FAB headers."""
+    exe = BIN / "plotfile_dump"
+    assert exe.exists()
+    out_dir = tmp_path / "plt00003"
+    r = subprocess.run([str(exe), str(out_dir)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    h, levels = read_plotfile(out_dir)
+    assert h["version"] == "HyperCLaw-V1.1" and h["dim"] == 3 and h["finest"] == 1
+    assert h["names"] == ["density", "x_velocity", "y_velocity", "z_velocity"]
+    assert h["ref"] == ["2"] and h["coord"] == 0 and h["steps"] == [3, 6]
+    assert [tuple(map(int, dm)) for dm in h["domains"]] == [(0, 0, 0, 23, 15, 11), (0, 0, 0, 47, 31, 23)]
+    assert h["dx"][0] == [1 / 24, 1 / 16, 1 / 12] and h["dx"][1] == [1 / 48, 1 / 32, 1 / 24]
+    for lev, (step, phys, path) in enumerate(h["level_boxes"]):
+        assert path == f"Level_{lev}/Cell" and len(phys) == levels[lev]["nboxes"]
+        ext = [24 * 2 ** lev, 16 * 2 ** lev, 12 * 2 ** lev]
+        for (lo, hi, _), p in zip(levels[lev]["fabs"], phys):
+            for a in range(3):
+                assert p[a] == [lo[a] / ext[a], (hi[a] + 1) / ext[a]]
+    for lev in range(2):
+        fabs = levels[lev]["fabs"]
+        stats = [[float(v) for v in s.split(",")] for s in levels[lev]["stats"]]
+        nb = len(fabs)
+        assert len(stats) == 2 * nb
+        for b, (_, _, data) in enumerate(fabs):
+            assert np.isfinite(data).all()
+            assert stats[b] == [float(data[c].min()) for c in range(4)]
+            assert stats[nb + b] == [float(data[c].max()) for c in range(4)]
+    samples = [line.split()[1:] for line in r.stdout.splitlines() if line.startswith("SAMPLE")]
+    assert len(samples) == 6
+    for s in samples:
+        lev, i, j, k = map(int, s[:4])
+        want = [float(v) for v in s[4:]]
+        hit = [data[:, k - lo[2], j - lo[1], i - lo[0]] for lo, hi, data in levels[lev]["fabs"]
+               if all(lo[a] <= p <= hi[a] for a, p in enumerate((i, j, k)))]
+        assert len(hit) == 1, (lev, i, j, k)
+        assert [float(v) for v in hit[0]] == want
