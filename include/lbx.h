@@ -63,11 +63,13 @@ int lbx_sync(lbx_ctx* ctx);
 /* Free and total device memory (bytes) as seen by the context's device. */
 int lbx_mem_info(lbx_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 
-/* A simulation on the periodic domain [0,dims).  Replaces the AmrSim
- * constructor's geometry and boundary set-up (src/AmrSim.cpp:753-802);
- * like the reference, any non-periodic direction is rejected
- * (LBX_E_UNSUPPORTED, src/AmrSim.cpp:794-796).  rho0 is the reference
- * density of LBX_FP32_SHIFTED storage (ignored for LBX_FP64). */
+/* A simulation on the domain [0,dims).  Replaces the AmrSim constructor's
+ * geometry and boundary set-up (src/AmrSim.cpp:753-802).  periodic[d] != 0
+ * wraps direction d; periodic[d] == 0 puts halfway bounce-back walls on
+ * both faces normal to d (the reference aborts instead, :794-796; walls are
+ * supported on single-level simulations, and level 1 is then refused with
+ * LBX_E_UNSUPPORTED).  rho0 is the reference density of LBX_FP32_SHIFTED
+ * storage (ignored for LBX_FP64). */
 int lbx_sim_create(lbx_ctx* ctx, const int32_t dims[3], const int32_t periodic[3],
                    int precision, double rho0, lbx_sim** out);
 int lbx_sim_destroy(lbx_sim* sim);

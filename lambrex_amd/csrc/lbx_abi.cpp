@@ -107,6 +107,7 @@ struct lbx_ctx {
 struct lbx_sim {
   lbx_ctx* ctx = nullptr;
   int dims[3] = {0, 0, 0};
+  unsigned walls = 0;  // bit d: bounce-back walls normal to axis d
   int precision = LBX_FP64;
   double rho0 = 1.0;
   Level lev[2];
@@ -478,6 +479,7 @@ int define_level0(lbx_sim* sim, const std::vector<Box>& boxes) {
   free_level(L);
   L.boxes = boxes;
   L.lat = lbx::make_lattice(sim->dims[0], sim->dims[1], sim->dims[2]);
+  L.lat.walls = sim->walls;
   const size_t bytes = (size_t)L.lat.elems() * elem_size(sim->precision);
   for (int i = 0; i < 2; ++i) {
     hipError_t e = hipMalloc(&L.buf[i], bytes);
@@ -538,6 +540,7 @@ int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
   lbx_ctx* ctx = sim->ctx;
   Level& C = sim->lev[0];
   if (!C.defined) return fail(ctx, LBX_E_STATE, "define level 0 before level 1");
+  if (sim->walls) return fail(ctx, LBX_E_UNSUPPORTED, "two-level runs need a fully periodic domain");
   const int fdom[3] = {2 * sim->dims[0], 2 * sim->dims[1], 2 * sim->dims[2]};
   for (size_t b = 0; b < boxes.size(); ++b) {
     if (!box_ok(boxes[b], fdom))
@@ -831,8 +834,6 @@ int lbx_sim_create(lbx_ctx* ctx, const int32_t dims[3], const int32_t periodic[3
   for (int d = 0; d < 3; ++d) {
     if (dims[d] < 1 || dims[d] > (1 << 28))
       return fail(ctx, LBX_E_ARG, "domain extent %d in direction %d", dims[d], d);
-    if (!periodic[d])
-      return fail(ctx, LBX_E_UNSUPPORTED, "Currently only periodic boundary conditions allowed.");
     cells *= dims[d];  // < 2^31 after every factor, so no overflow
     if (cells >= (1ll << 31)) return fail(ctx, LBX_E_ARG, "domain exceeds 2^31 cells");
   }
@@ -842,7 +843,10 @@ int lbx_sim_create(lbx_ctx* ctx, const int32_t dims[3], const int32_t periodic[3
   lbx_sim* sim = new (std::nothrow) lbx_sim;
   if (!sim) return fail(ctx, LBX_E_NOMEM, "out of host memory");
   sim->ctx = ctx;
-  for (int d = 0; d < 3; ++d) sim->dims[d] = dims[d];
+  for (int d = 0; d < 3; ++d) {
+    sim->dims[d] = dims[d];
+    if (!periodic[d]) sim->walls |= 1u << d;
+  }
   sim->precision = precision;
   sim->rho0 = rho0;
   *out = sim;

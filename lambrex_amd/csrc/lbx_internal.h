@@ -33,6 +33,9 @@ struct Lattice {
   int nx, ny, nz;
   long long ps;  // population-plane stride: nx*ny rounded up to 64 elements
   long long zs;  // z-plane stride: 15 * ps
+  // bit d set: halfway bounce-back walls on both faces normal to axis d
+  // instead of the periodic wrap (single-level lattices only)
+  unsigned walls;
   LBX_IHD long long at(int q, long long plane_cell, long long z) const { return z * zs + q * ps + plane_cell; }
   // linear cell index (z*ny + y)*nx + x
   LBX_IHD long long at_cell(int q, long long c) const {
@@ -49,6 +52,7 @@ inline Lattice make_lattice(int nx, int ny, int nz) {
   L.nz = nz;
   L.ps = ((long long)nx * ny + 63) / 64 * 64;
   L.zs = NV_TOTAL * L.ps;
+  L.walls = 0;
   return L;
 }
 

@@ -58,6 +58,35 @@ __device__ __forceinline__ void load_cell(const S* __restrict__ src, const Latti
   }
 }
 
+// Index of the population opposite to q (c_opp = -c_q).
+__device__ constexpr int opposite(int q) {
+  int o = 0;
+  for (int p = 0; p < NV; ++p)
+    if (kC[p][0] == -kC[q][0] && kC[p][1] == -kC[q][1] && kC[p][2] == -kC[q][2]) o = p;
+  return o;
+}
+
+// Pull with halfway bounce-back: population q of cell (x, y, z) comes from
+// x - c_q, unless that crosses a wall face (an axis with its bit in
+// L.walls), where the cell's own opposite population, which left towards
+// the wall, returns instead.  Periodic axes wrap as in load_cell.
+template <typename S>
+__device__ __forceinline__ void load_cell_walls(const S* __restrict__ src, const Lattice& L, int x, int y,
+                                                int z, long long selfz, int selfp, const Upstream& u, S* f) {
+  const bool wx = L.walls & 1u, wy = L.walls & 2u, wz = L.walls & 4u;
+  const bool lo_x = wx && x == 0, hi_x = wx && x == L.nx - 1;
+  const bool lo_y = wy && y == 0, hi_y = wy && y == L.ny - 1;
+  const bool lo_z = wz && z == 0, hi_z = wz && z == L.nz - 1;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const bool bounce = (kC[q][0] > 0 && lo_x) || (kC[q][0] < 0 && hi_x) || (kC[q][1] > 0 && lo_y) ||
+                        (kC[q][1] < 0 && hi_y) || (kC[q][2] > 0 && lo_z) || (kC[q][2] < 0 && hi_z);
+    const long long off = bounce ? selfz + opposite(q) * L.ps + selfp
+                                 : u.zo[1 + kC[q][2]] + q * L.ps + u.ys[1 + kC[q][1]] + u.xs[1 + kC[q][0]];
+    f[q] = __builtin_nontemporal_load(src + off);
+  }
+}
+
 template <typename S>
 __device__ __forceinline__ void store_cell(S* __restrict__ dst, const Lattice& L, long long selfz, int selfp,
                                            const S* f) {
@@ -72,7 +101,7 @@ __device__ __forceinline__ void store_cell(S* __restrict__ dst, const Lattice& L
 // per CU (FP32) run ~8% and ~7% faster than uncapped launches.  Every
 // element is loaded once per pass (the x+-1 neighbours of a wavefront are
 // in the lines its own row fetches), so loads are non-temporal too.
-template <int KIND>
+template <int KIND, bool WALLS>
 __global__ __launch_bounds__(512) void k_step_f64(const double* __restrict__ src,
                                                   double* __restrict__ dst, Lattice L, Relax r) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,6 +113,8 @@ __global__ __launch_bounds__(512) void k_step_f64(const double* __restrict__ src
   double f[NV];
   if (KIND == STEP_COLLIDE) {
     load_cell<double, false>(src, L, selfz, selfp, Upstream{}, f);
+  } else if (WALLS) {
+    load_cell_walls<double>(src, L, x, y, z, selfz, selfp, upstream(x, y, z, L), f);
   } else {
     const Upstream u = upstream(x, y, z, L);
     load_cell<double, true>(src, L, selfz, selfp, u, f);
@@ -92,7 +123,7 @@ __global__ __launch_bounds__(512) void k_step_f64(const double* __restrict__ src
   store_cell<double>(dst, L, selfz, selfp, f);
 }
 
-template <int KIND>
+template <int KIND, bool WALLS>
 __global__ __launch_bounds__(512) void k_step_f32s(const float* __restrict__ src,
                                                    float* __restrict__ dst, Lattice L, Relax r) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,6 +135,8 @@ __global__ __launch_bounds__(512) void k_step_f32s(const float* __restrict__ src
   float f[NV];
   if (KIND == STEP_COLLIDE) {
     load_cell<float, false>(src, L, selfz, selfp, Upstream{}, f);
+  } else if (WALLS) {
+    load_cell_walls<float>(src, L, x, y, z, selfz, selfp, upstream(x, y, z, L), f);
   } else {
     const Upstream u = upstream(x, y, z, L);
     load_cell<float, true>(src, L, selfz, selfp, u, f);
@@ -351,26 +384,34 @@ LaunchShape default_shape(const Lattice& L, int precision) {
   return s;
 }
 
+#define LBX_STEP_LAUNCH(KERNEL, SRC, DST)                                                          \
+  do {                                                                                          \
+    const dim3 grid = grid_of(L, s), block(s.bx, s.by, 1);                                      \
+    const bool w = L.walls != 0;                                                                \
+    switch (kind) {                                                                             \
+      case STEP_FUSED:                                                                          \
+        if (w) KERNEL<STEP_FUSED, true><<<grid, block, s.lds_bytes, st>>>(SRC, DST, L, r);      \
+        else KERNEL<STEP_FUSED, false><<<grid, block, s.lds_bytes, st>>>(SRC, DST, L, r);       \
+        break;                                                                                  \
+      case STEP_COLLIDE:                                                                        \
+        KERNEL<STEP_COLLIDE, false><<<grid, block, s.lds_bytes, st>>>(SRC, DST, L, r);          \
+        break;                                                                                  \
+      case STEP_STREAM:                                                                         \
+        if (w) KERNEL<STEP_STREAM, true><<<grid, block, s.lds_bytes, st>>>(SRC, DST, L, r);     \
+        else KERNEL<STEP_STREAM, false><<<grid, block, s.lds_bytes, st>>>(SRC, DST, L, r);      \
+        break;                                                                                  \
+    }                                                                                           \
+    return hipGetLastError();                                                                   \
+  } while (0)
+
 hipError_t launch_step_f64(StepKind kind, const double* src, double* dst, const Lattice& L,
                            const Relax& r, LaunchShape s, hipStream_t st) {
-  const dim3 grid = grid_of(L, s), block(s.bx, s.by, 1);
-  switch (kind) {
-    case STEP_FUSED: k_step_f64<STEP_FUSED><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
-    case STEP_COLLIDE: k_step_f64<STEP_COLLIDE><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
-    case STEP_STREAM: k_step_f64<STEP_STREAM><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
-  }
-  return hipGetLastError();
+  LBX_STEP_LAUNCH(k_step_f64, src, dst);
 }
 
 hipError_t launch_step_f32s(StepKind kind, const float* src, float* dst, const Lattice& L,
                             const Relax& r, LaunchShape s, hipStream_t st) {
-  const dim3 grid = grid_of(L, s), block(s.bx, s.by, 1);
-  switch (kind) {
-    case STEP_FUSED: k_step_f32s<STEP_FUSED><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
-    case STEP_COLLIDE: k_step_f32s<STEP_COLLIDE><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
-    case STEP_STREAM: k_step_f32s<STEP_STREAM><<<grid, block, s.lds_bytes, st>>>(src, dst, L, r); break;
-  }
-  return hipGetLastError();
+  LBX_STEP_LAUNCH(k_step_f32s, src, dst);
 }
 
 hipError_t launch_equilibrium(int precision, void* f, const Lattice& L, const Region& R,

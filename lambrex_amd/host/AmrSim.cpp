@@ -22,9 +22,8 @@ AmrSim::AmrSim(int nx, int ny, int nz, int max_ref_level,
       max_level_(max_ref_level) {
   if (nx < 1 || ny < 1 || nz < 1) throw lambrex::Error("domain extents must be positive");
   if (max_ref_level < 0) throw lambrex::Error("max_ref_level must be >= 0");
-  for (int d = 0; d < NDIMS; ++d)
-    if (!PERIODICITY[d])  // src/AmrSim.cpp:788-797
-      throw lambrex::Error("Currently only periodic boundary conditions allowed.");
+  // The reference aborts on a non-periodic direction (src/AmrSim.cpp:788-797);
+  // here such a direction gets halfway bounce-back walls (single level only).
   const int num_levels = max_level_ + 1;
   tau_s_.assign(num_levels, 0.0);
   tau_b_.assign(num_levels, 0.0);

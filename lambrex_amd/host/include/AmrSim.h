@@ -2,8 +2,10 @@
 // reference) over the MI355X C ABI (include/lbx.h).
 //
 // Same public names, argument meaning and state semantics as the reference:
-//   - domain [0,nx)x[0,ny)x[0,nz), fully periodic (anything else is
-//     rejected, src/AmrSim.cpp:788-797);
+//   - domain [0,nx)x[0,ny)x[0,nz), periodic in each direction whose
+//     periodicity flag is set; where the reference aborts on a non-periodic
+//     direction (src/AmrSim.cpp:788-797), this build puts halfway
+//     bounce-back walls on its faces (single-level simulations only);
 //   - initial density / velocity vectors in C order, i slowest, velocity
 //     component fastest (include/AmrSim.h:79-83);
 //   - omega = 1/(tau + 1/2) (src/AmrSim.cpp:125-126);

@@ -1,6 +1,8 @@
 // init_tests.cpp -- the reference's initialisation and single-level AMR
 // tests (tests/catch2InitTests.cpp:5-109, tests/catch2AMRTests.cpp:8-116)
 // against this build's AmrSim facade.
+#include <cmath>
+
 #include "check.h"
 #include "lambrex.h"
 
@@ -88,8 +90,20 @@ TEST_CASE("OneLevel") {
   REQUIRE(!sim.OnProcessVelocity(sim.GetVelocity(0, NY, 0, 0, LEVEL)));
 }
 
-TEST_CASE("Non-periodic domains are rejected") {
-  REQUIRE_THROWS(AmrSim(8, 8, 8, 0, {1, 0, 1}, 0.5, 0.5));
+// The reference aborts on a non-periodic direction (src/AmrSim.cpp:788-797);
+// here it gets bounce-back walls: a closed box keeps its mass, and a fine
+// level is refused.
+TEST_CASE("Non-periodic directions get bounce-back walls") {
+  AmrSim sim(8, 6, 4, 1, {1, 0, 1}, 0.7, 0.7);
+  sim.SetInitialDensity(1.0);
+  sim.SetInitialVelocity(0.02);
+  sim.InitFromScratch(0.0);
+  const double m0 = sim.Conservation(0)[0];
+  sim.Iterate(40);
+  REQUIRE(std::abs(sim.Conservation(0)[0] - m0) <= 1e-12 * m0);
+  sim.CalcHydroVars(0);
+  REQUIRE(std::abs(sim.GetVelocity(3, 0, 1, 1, 0)) < 0.02);  // the wall stops the normal flow
+  REQUIRE_THROWS(sim.SetStaticRefinement(0, {0, 0, 0}, {3, 3, 3}));
 }
 
 TEST_CASE("Conservation reductions") {

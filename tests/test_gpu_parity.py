@@ -228,9 +228,12 @@ def test_fp32_shifted_close_to_fp64(ctx):
 
 def test_error_paths(ctx):
     lib = ctx._lib
+    walled = lbx.Sim(ctx, (8, 8, 8), periodic=(1, 0, 1))  # bounce-back walls in y: single level only
+    walled.define_level(0, workloads.chop((8, 8, 8), 8))
     with pytest.raises(lbx.LbxError) as e:
-        lbx.Sim(ctx, (8, 8, 8), periodic=(1, 0, 1))
+        walled.define_level(1, [((0, 0, 0), (3, 3, 3))])
     assert e.value.code == lbx.LBX_E_UNSUPPORTED
+    walled.close()
     sim = lbx.Sim(ctx, (8, 8, 8))
     with pytest.raises(lbx.LbxError) as e:
         sim.define_level(0, [((0, 0, 0), (7, 7, 6))])  # does not tile
