@@ -677,10 +677,6 @@ int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
   start.push_back((int)list.size());
   F.sum_ncell = (int)cell.size();
   hipError_t e = upload(&F.d_desc, F.desc, ctx->stream);
-  // the scratch patches' outer ghost layer holds f = 0 for good
-  if (e == hipSuccess)
-    e = lbx::launch_patch_outer_zero(sim->precision, F.buf[2], F.d_desc, (int)F.desc.size(), F.max_cells,
-                                     sim->rho0, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_fill, fill, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_cell, cell, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_start, start, ctx->stream);

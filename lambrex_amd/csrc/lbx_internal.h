@@ -129,12 +129,8 @@ struct PatchDesc {
 hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
                                     const Lattice& clat, const PatchDesc* patches, int first, int npatch,
                                     int max_cells, const long long* fill, const Relax& r, hipStream_t st);
-// Leaves the outer ghost layer of `out` untouched: it must hold f = 0
-// (launch_patch_outer_zero, once per allocation).
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, const PatchDesc* patches,
                                      int first, int npatch, int max_cells, const Relax& r, hipStream_t st);
-hipError_t launch_patch_outer_zero(int precision, void* f, const PatchDesc* patches, int npatch, int max_cells,
-                                   double rho0, hipStream_t st);
 // avg holds 15 FP64 planes of avg_stride block averages; max_blocks = max (g/2)^3.
 hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
                                   const PatchDesc* patches, int first, int npatch, int max_blocks, double rho0,

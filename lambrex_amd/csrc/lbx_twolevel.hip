@@ -149,8 +149,7 @@ __device__ __forceinline__ bool fine_pull(const PatchDesc& P, const S* __restric
 }
 
 // First fine substep's stream + the second substep's collision: pull, then
-// collide valid cells.  The outer ghost layer of `out` is zero once and for
-// all (k_patch_outer_zero) and is not rewritten.
+// collide valid cells.
 template <typename S>
 __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict__ in, S* __restrict__ out,
                                                              const PatchDesc* __restrict__ patches, int first,
@@ -160,25 +159,10 @@ __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
-  if (patch_outer(P, gx, gy, gz)) return;
   S f[NV];
   if (fine_pull(P, in, i, gx, gy, gz, r.rho0, f)) collide_s(f, r);
 #pragma unroll
   for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
-}
-
-// Sets the outer ghost layer of every patch of f to f = 0.
-template <typename S>
-__global__ __launch_bounds__(256) void k_patch_outer_zero(S* __restrict__ f, const PatchDesc* __restrict__ patches,
-                                                          double rho0) {
-  const PatchDesc P = patches[blockIdx.y];
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= P.cells) return;
-  int gx, gy, gz;
-  patch_coords(P, i, gx, gy, gz);
-  if (!patch_outer(P, gx, gy, gz)) return;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) f[P.base + q * P.qs + i] = zero_pop<S>(q, rho0);
 }
 
 // Second fine substep's stream + ZeroInvalidComponents(1) (a ghost keeps
@@ -210,19 +194,15 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     const int gx = 2 * cx + di, gy = 2 * cy + dj, gz = 2 * cz + dk;
     const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
     S f[NV];
-    const bool valid = fine_pull(P, in, i, gx, gy, gz, rho0, f);
-    if (!valid && !patch_outer(P, gx, gy, gz)) {
+    if (!fine_pull(P, in, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz)) {
       const int x = P.lo[0] - 2 + gx, y = P.lo[1] - 2 + gy, z = P.lo[2] - 2 + gz;
 #pragma unroll
       for (int q = 0; q < NV; ++q)
         if (!in_valid(P, x - 2 * kC[q][0], y - 2 * kC[q][1], z - 2 * kC[q][2])) f[q] = zero_pop<S>(q, rho0);
     }
-    // Only valid cells are stored: the ghosts of the new state are never read
-    // again (the next fill rewrites every ghost from valid cells), so their
-    // final values only enter the block sums.
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      if (valid) out[P.base + q * P.qs + i] = f[q];
+      out[P.base + q * P.qs + i] = f[q];
       part[q][child][lb] = to_f64(f[q], q, rho0);
     }
   }
@@ -421,16 +401,6 @@ hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, 
     k_fine_stream_collide<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, r);
   else
     k_fine_stream_collide<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, r);
-  return hipGetLastError();
-}
-
-hipError_t launch_patch_outer_zero(int precision, void* f, const PatchDesc* patches, int npatch, int max_cells,
-                                   double rho0, hipStream_t st) {
-  const dim3 grid((max_cells + 255) / 256, npatch);
-  if (precision == 0)
-    k_patch_outer_zero<double><<<grid, 256, 0, st>>>((double*)f, patches, rho0);
-  else
-    k_patch_outer_zero<float><<<grid, 256, 0, st>>>((float*)f, patches, rho0);
   return hipGetLastError();
 }
 
