@@ -305,7 +305,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the domain would shard by boxes with one halo exchange per step; see DESIGN.md).",
+              "(the domain would shard by boxes with one halo exchange per step).",
               file=sys.stderr)
         sys.exit(2)
 

@@ -15,7 +15,11 @@
  *   - whole path: the reference's pulse goldens (tests/pulseRegression.h)
  *     under Catch2 Approx rules, and decomposition invariance (bitwise);
  *   - two-level cycle: parity unpinned (no reference test pins it), only
- *     invariants and the single-level limit are checked.
+ *     invariants and the single-level limit are checked;
+ *   - deeper hierarchies: the recursion is pinned by an exact reduction (a
+ *     level 1 covering the whole domain makes levels 1..n evolve bit for bit
+ *     like a hierarchy one level shallower whose level 0 is that level 1);
+ *     the level-0/level-1 coupling inside it is the two-level code above.
  */
 #include "lbx_oracle.h"
 
@@ -262,7 +266,7 @@ typedef struct {
 struct orc_sim {
   int n[3];
   int nlev;
-  orc_level lev[2];
+  orc_level lev[ORC_MAX_LEVELS];
 };
 
 static int bnx(const box3* b, int d) { return b->hi[d] - b->lo[d] + 1; }
@@ -567,48 +571,69 @@ static void iterate_single(orc_sim* s) {
   ++L->step;
 }
 
-/* RohdeCycle(0) with exactly one fine level (src/AmrSim.cpp:430-469). */
-static void iterate_two_level(orc_sim* s) {
-  orc_level* C = &s->lev[0];
-  orc_level* F = &s->lev[1];
-  /* InitPostCollision(0) and CoarseCollide(0) */
-  fill_single_level(C, C->next, C->now, ORC_HALO, 0);
-  collide_level(C, C->next, 1);
-  /* InitPostCollision(1): two-level fill, then comp 0 zeroed on the outer shell */
+/* The finest level's part of RohdeCycle (src/AmrSim.cpp:445-452):
+ * InitPostCollision (two-level fill, comp 0 zeroed on the outer shell), then
+ * FineCollide, Stream, FineCollide, Stream with no ghost refresh between,
+ * ZeroInvalidComponents, and UpdateDistribution's two steps and swap
+ * (src/AmrSim.cpp:619-631). */
+static void finest_pair(orc_level* C, orc_level* F) {
   fill_two_levels(C, F, F->next);
   zero_outer_comp0(F, F->next);
-  /* FineCollide, Stream, FineCollide, Stream (no ghost refresh between) */
   collide_level(F, F->next, 0);
   stream_level(F, &F->next);
   collide_level(F, F->next, 0);
   stream_level(F, &F->next);
   zero_invalid(F, F->next);
-  /* UpdateDistribution(1): two fine steps, swap (src/AmrSim.cpp:619-631) */
   F->time += 2 * F->dt;
   F->step += 2;
   swap_now_next(F);
-  /* Stream(0), SumFromFine(0), ZeroInvalidComponents(0), UpdateBoundaries(0) */
-  stream_level(C, &C->next);
-  sum_fine_to_coarse(F, C, C->next);
-  zero_invalid(C, C->next);
-  fill_single_level(C, C->next, C->next, ORC_HALO, 1);
-  C->time += C->dt;
-  C->step += 1;
-  swap_now_next(C);
+}
+
+/* RohdeCycle(l) (src/AmrSim.cpp:430-469): one step of level l, which has a
+ * finer level.  Level 0's InitPostCollision is a single-level fill; a finer
+ * level's is the two-level fill from its parent's `now` plus comp 0 zeroed on
+ * the outer ghost layer (:471-485).  The collide is masked by level l's fine
+ * mask (:487-580).  If level l+1 is the finest it takes its two substeps
+ * once, else RohdeCycle(l+1) runs refRatio = 2 times (:453-455).  Then
+ * Stream(l), SumFromFine(l) from level l+1's `now`, ZeroInvalidComponents(l),
+ * UpdateBoundaries on level 0 only, and UpdateDistribution(l): one step. */
+static void rohde_cycle(orc_sim* s, int l) {
+  orc_level* L = &s->lev[l];
+  orc_level* F = &s->lev[l + 1];
+  if (l == 0) {
+    fill_single_level(L, L->next, L->now, ORC_HALO, 0);
+  } else {
+    fill_two_levels(&s->lev[l - 1], L, L->next);
+    zero_outer_comp0(L, L->next);
+  }
+  collide_level(L, L->next, 1);
+  if (l + 1 == s->nlev - 1) {
+    finest_pair(L, F);
+  } else {
+    rohde_cycle(s, l + 1);
+    rohde_cycle(s, l + 1);
+  }
+  stream_level(L, &L->next);
+  sum_fine_to_coarse(F, L, L->next);
+  zero_invalid(L, L->next);
+  if (l == 0) fill_single_level(L, L->next, L->next, ORC_HALO, 1);
+  L->time += L->dt;
+  L->step += 1;
+  swap_now_next(L);
 }
 
 void orc_sim_iterate(orc_sim* s, int nsteps) {
   for (int t = 0; t < nsteps; ++t) {
     if (s->nlev == 1) iterate_single(s);
-    else iterate_two_level(s);
+    else rohde_cycle(s, 0);
   }
 }
 
 int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
                      const int* mask_lohi_fine) {
-  if (s->nlev != 1 || nboxes < 1) return -1;
-  orc_level* C = &s->lev[0];
-  orc_level* F = &s->lev[1];
+  if (s->nlev < 1 || s->nlev >= ORC_MAX_LEVELS || nboxes < 1) return -1;
+  orc_level* C = &s->lev[s->nlev - 1];
+  orc_level* F = &s->lev[s->nlev];
   memset(F, 0, sizeof(*F));
   F->nbox = nboxes;
   F->vb = (box3*)calloc(nboxes, sizeof(box3));
@@ -628,7 +653,24 @@ int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
   F->time = C->time;
   F->step = 0;
   level_alloc(F);
-  s->nlev = 2;
+  /* Proper nesting: every cell of each grown fine box that no fine box
+   * holds, and every valid fine cell, needs a parent in the coarse level's
+   * valid region (AMReX guarantees it when it builds the grids; a parent
+   * outside would leave FillPatchTwoLevels without data). */
+  for (int b = 0; b < F->nbox; ++b) {
+    const box3* B = &F->vb[b];
+    for (int i = B->lo[0] - ORC_HALO; i <= B->hi[0] + ORC_HALO; ++i)
+      for (int j = B->lo[1] - ORC_HALO; j <= B->hi[1] + ORC_HALO; ++j)
+        for (int k = B->lo[2] - ORC_HALO; k <= B->hi[2] + ORC_HALO; ++k) {
+          int si, sj, sk;
+          if (!in_box(B, i, j, k) && locate(F, i, j, k, &si, &sj, &sk) >= 0) continue;
+          if (locate(C, floor_div2(i), floor_div2(j), floor_div2(k), &si, &sj, &sk) < 0) {
+            level_free(F);
+            return -3;
+          }
+        }
+  }
+  s->nlev += 1;
   /* DistFnFillFromCoarse (src/AmrSim.cpp:393-411): PC interpolation of the
    * coarse `now` into fine valid cells.  Fine ghosts are not read before the
    * first two-level fill overwrites them. */
@@ -646,6 +688,8 @@ int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
   /* Fine mask on coarse valid cells: FINE_VAL where the coarse cell lies in
    * coarsen(mask box, 2) for some mask box (src/AmrSim.cpp:419-428). */
   size_t total = 0;
+  free(C->mask);
+  free(C->mask_off);
   C->mask_off = (int*)calloc(C->nbox, sizeof(int));
   for (int b = 0; b < C->nbox; ++b) {
     C->mask_off[b] = (int)total;

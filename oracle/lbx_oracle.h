@@ -19,7 +19,7 @@
  *   - equilibrium            src/AmrSim.cpp:845-936
  *   - hydro fields           src/AmrSim.cpp:938-979
  *   - C-order init layout    src/AmrSim.cpp:138-295, include/AmrSim.h:79-83
- *   - Rohde two-level cycle  src/AmrSim.cpp:419-631 (see orc_sim_* two-level)
+ *   - Rohde cycle            src/AmrSim.cpp:419-631, recursive over levels
  *   - loop drivers           include/amr_help.h:79-167
  *
  * Storage mirrors AMReX FABs: one array per box grown by 2 ghost layers,
@@ -35,6 +35,7 @@ extern "C" {
 
 #define ORC_NV 15
 #define ORC_HALO 2
+#define ORC_MAX_LEVELS 4
 
 /* Tables: velocity set (include/d3q15_bgk.h:14-22) and mode matrices. */
 void orc_tables(double M[15][15], double MI[15][15], int C[15][3]);
@@ -63,11 +64,14 @@ void orc_sim_destroy(orc_sim* s);
  * equilibrium distribution on level 0.  src/AmrSim.cpp:138-295, 845-936. */
 int orc_sim_init(orc_sim* s, const double* rho_c, const double* u_c);
 
-/* Add a fine level (refinement ratio 2) made of the given boxes in fine
- * index space, filled from the coarse level by piecewise-constant
- * interpolation (src/AmrSim.cpp:689-713).  mask_lohi lists the coarse boxes
- * whose coarsened(by 2) cover is masked in the coarse collide; the reference
- * passes the coarse BoxArray itself (src/AmrSim.cpp:419-428). */
+/* Add a new finest level (refinement ratio 2, at most ORC_MAX_LEVELS
+ * levels) made of the given boxes in its own index space, filled from the
+ * current finest level by piecewise-constant interpolation
+ * (src/AmrSim.cpp:689-713).  mask_lohi lists the boxes (new level's index
+ * space) whose coarsened(by 2) cover is masked in the parent's collide; the
+ * reference passes the parent's own BoxArray (src/AmrSim.cpp:419-428).
+ * Returns -3 if a grown new box needs a parent cell outside the parent
+ * level's valid region (not properly nested). */
 int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine,
                      int nmask, const int* mask_lohi_fine);
 

@@ -1,0 +1,207 @@
+"""The oracle's Rohde cycle over more than two levels (oracle/lbx_oracle.c,
+restating the recursion of src/AmrSim.cpp:430-469).
+
+No reference test runs more than two levels.  The recursion is pinned by:
+  * an exact reduction: when level 1 covers the whole domain, level 0 never
+    feeds it (every level-1 ghost has a level-1 periodic image), so levels
+    1..n must evolve bit for bit like the hierarchy one level shallower
+    whose level 0 is that level 1, stepped twice as often;
+  * a second restatement of the recursion written here with numpy arrays per
+    FAB (the two-level N-version of test_oracle_twolevel.py, made
+    recursive), on a hierarchy whose level 1 does not cover the domain;
+  * the time, step and relaxation bookkeeping of ComputeDt and
+    UpdateDistribution (src/AmrSim.cpp:297-322, 619-631);
+  * refusal of a level that is not properly nested in its parent.
+"""
+import numpy as np
+import pytest
+
+from lambrex_amd import workloads
+from oracle import oracle as orc
+from test_oracle_twolevel import G, Level, collide, fill_single, fill_two_levels, level_valid_array, stream, \
+    zero_invalid
+
+
+def c_order(rho, u):
+    return (np.ascontiguousarray(rho.transpose(2, 1, 0)).reshape(-1),
+            np.ascontiguousarray(u.transpose(3, 2, 1, 0)).reshape(-1))
+
+
+def as_lists(boxes):
+    return [(list(a), list(b)) for a, b in boxes]
+
+
+def sum_fine_to_coarse(F, Cl, cdst):
+    """As the two-level restatement, but a coarse position that no valid cell
+    of the (patched) coarse level holds is dropped, as ParallelAdd into valid
+    cells does."""
+    for b in range(len(F.boxes)):
+        lo, hi = F.boxes[b]
+        for ci in range(lo[0] // 2 - 1, hi[0] // 2 + 2):
+            for cj in range(lo[1] // 2 - 1, hi[1] // 2 + 2):
+                for ck in range(lo[2] // 2 - 1, hi[2] // 2 + 2):
+                    t, w = Cl.owner(np.array([ci, cj, ck]))
+                    if t is None:
+                        continue
+                    for q in range(15):
+                        s = 0.0
+                        for dk in range(2):
+                            for dj in range(2):
+                                for di in range(2):
+                                    s += F.now[b][(q,) + F.idx(b, (2 * ci + di, 2 * cj + dj, 2 * ck + dk))]
+                        cdst[t][(q,) + Cl.idx(t, w)] += s * 0.125
+
+
+def cycle(levels, masks, l):
+    L, F = levels[l], levels[l + 1]
+    if l == 0:
+        fill_single(L, L.next, L.now, False)
+    else:
+        fill_two_levels(levels[l - 1], L, L.next)  # with comp 0 zeroed on the outer layer
+    collide(L, L.next, masks[l])
+    if l + 1 == len(levels) - 1:
+        fill_two_levels(L, F, F.next)
+        collide(F, F.next)
+        F.next = stream(F, F.next)
+        collide(F, F.next)
+        F.next = stream(F, F.next)
+        zero_invalid(F, F.next)
+        F.now, F.next = F.next, F.now
+    else:
+        cycle(levels, masks, l + 1)
+        cycle(levels, masks, l + 1)
+    L.next = stream(L, L.next)
+    sum_fine_to_coarse(F, L, L.next)
+    zero_invalid(L, L.next)
+    if l == 0:
+        fill_single(L, L.next, L.next, True)
+    L.now, L.next = L.next, L.now
+
+
+def mask_fn(mboxes):
+    def mask(i, j, k):
+        return any(a[0] // 2 <= i <= b[0] // 2 and a[1] // 2 <= j <= b[1] // 2 and a[2] // 2 <= k <= b[2] // 2
+                   for a, b in mboxes)
+    return mask
+
+
+@pytest.mark.parametrize("mask_kind", ["reference", "fine"])
+def test_three_level_cycle_matches_independent_restatement(mask_kind):
+    dims = (4, 4, 4)
+    tau = 0.7
+    boxes = [[((0, 0, 0), (3, 3, 3))],
+             [((2, 2, 2), (3, 5, 5)), ((4, 2, 2), (5, 5, 5))],
+             [((6, 6, 6), (9, 9, 9))]]
+    if mask_kind == "reference":  # each parent's own BoxArray (src/AmrSim.cpp:419-428)
+        mboxes = [boxes[0], boxes[1]]
+    else:
+        mboxes = [boxes[1], boxes[2]]
+    rho, u = workloads.random_state(dims, seed=11, rho_sigma=1e-2, u_amp=0.02)
+    o = orc.Sim(*dims, tau, boxes=as_lists(boxes[0]))
+    o.init(*c_order(rho, u))
+    f0 = o.get_f(0)
+    o.add_fine(as_lists(boxes[1]), as_lists(mboxes[0]))
+    o.add_fine(as_lists(boxes[2]), as_lists(mboxes[1]))
+
+    levels = []
+    t = tau
+    for l, bl in enumerate(boxes):
+        levels.append(Level(bl, tuple((2 ** l) * d for d in dims), t))
+        t = 2 * (t - 0.5) + 0.5
+    levels[0].now[0][:, G:-G, G:-G, G:-G] = f0
+    for l in (1, 2):  # DistFnFillFromCoarse: piecewise-constant copy of the parent
+        F, Cl = levels[l], levels[l - 1]
+        for b in range(len(F.boxes)):
+            lo, hi = F.boxes[b]
+            for k in range(lo[2], hi[2] + 1):
+                for j in range(lo[1], hi[1] + 1):
+                    for i in range(lo[0], hi[0] + 1):
+                        cs, cw = Cl.owner(np.array([i // 2, j // 2, k // 2]))
+                        F.now[b][(slice(None),) + F.idx(b, (i, j, k))] = Cl.now[cs][(slice(None),) + Cl.idx(cs, cw)]
+    masks = [mask_fn(mboxes[0]), mask_fn(mboxes[1]), None]
+    for step in range(2):
+        cycle(levels, masks, 0)
+        o.iterate(1)
+        for l in range(3):
+            bb = o.bbox(l)
+            got = level_valid_array(levels[l], tuple(bb[:3]), tuple(bb[3:]))
+            assert (got.view(np.uint64) == o.get_f(l).view(np.uint64)).all(), f"level {l}, step {step + 1}"
+    o.close()
+
+
+@pytest.mark.parametrize("nlev", [3, 4])
+def test_full_level1_reduces_to_shallower_hierarchy(nlev):
+    dims = (8, 8, 8) if nlev == 3 else (4, 4, 4)
+    tau = 0.8
+    b0 = [(tuple(a), tuple(b)) for a, b in workloads.chop(dims, 4)]
+    b1 = [(tuple(a), tuple(b)) for a, b in workloads.chop(tuple(2 * d for d in dims), 8 if nlev == 3 else 4)]
+    n1 = 2 * dims[0]
+    # deeper levels inside level 1 (whole domain), properly nested
+    deeper = [[((n1 // 2, n1 // 2, n1 // 2), (3 * n1 // 2 - 1, n1 - 1, 3 * n1 // 2 - 1)),
+               ((n1 // 2, n1, n1 // 2), (3 * n1 // 2 - 1, 3 * n1 // 2 - 1, 3 * n1 // 2 - 1))]]
+    if nlev == 4:
+        # level-3 cells 3n1/2 .. 3n1/2+7: grown by 2, parents n1/2+... lie in
+        # level 2's valid region n1/2 .. 3n1/2-1
+        deeper.append([((3 * n1 // 2,) * 3, (3 * n1 // 2 + 7,) * 3)])
+    rho, u = workloads.random_state(dims, seed=3, rho_sigma=1e-2, u_amp=0.03)
+    deep = orc.Sim(*dims, tau, boxes=as_lists(b0))
+    deep.init(*c_order(rho, u))
+    deep.add_fine(as_lists(b1), as_lists(b0))  # the reference's mask: level 0's own boxes
+    # level 1 was filled by piecewise-constant copy; give it structure of its
+    # own so the comparison sees more than a smooth field
+    rng = np.random.default_rng(9)
+    f1 = deep.get_f(1)
+    deep.set_f(f1 * (1.0 + 1e-3 * rng.standard_normal(f1.shape)), 1)
+    parents = [b1] + deeper
+    for l, bl in enumerate(deeper):
+        deep.add_fine(as_lists(bl), as_lists(parents[l]))
+
+    tau1 = 2 * (tau - 0.5) + 0.5
+    shallow = orc.Sim(*(2 * d for d in dims), tau1, boxes=as_lists(b1))
+    shallow.init(np.ones(8 * dims[0] * dims[1] * dims[2]), np.zeros(24 * dims[0] * dims[1] * dims[2]))
+    shallow.set_f(deep.get_f(1), 0)
+    for l, bl in enumerate(deeper):
+        shallow.add_fine(as_lists(bl), as_lists(parents[l]))
+        # same piecewise-constant fill from the same parent data
+        assert (shallow.get_f(l + 1).view(np.uint64) == deep.get_f(l + 2).view(np.uint64)).all()
+
+    for step in range(2):
+        deep.iterate(1)
+        shallow.iterate(2)
+        for l in range(1, nlev):
+            assert (deep.get_f(l).view(np.uint64) == shallow.get_f(l - 1).view(np.uint64)).all(), \
+                f"level {l}, step {step + 1}"
+    deep.close()
+    shallow.close()
+
+
+def test_multilevel_bookkeeping():
+    dims = (4, 4, 4)
+    tau = 0.6
+    o = orc.Sim(*dims, tau, boxes=[([0, 0, 0], [3, 3, 3])])
+    rho, u = workloads.random_state(dims, seed=1, rho_sigma=1e-3, u_amp=0.01)
+    o.init(*c_order(rho, u))
+    o.add_fine([([2, 2, 2], [5, 5, 5])], [([0, 0, 0], [3, 3, 3])])
+    o.add_fine([([6, 6, 6], [9, 9, 9])], [([2, 2, 2], [5, 5, 5])])
+    o.add_fine([([14, 14, 14], [17, 17, 17])], [([6, 6, 6], [9, 9, 9])])
+    o.iterate(3)
+    t = tau
+    for l in range(4):
+        assert o.step(l) == 3 * 2 ** l
+        assert o.time(l) == 3.0  # dt_l = 2^-l: the sums are exact
+        assert o.tau(l, 0) == t and o.tau(l, 1) == t
+        t = 2 * (t - 0.5) + 0.5
+    o.close()
+
+
+def test_level_not_properly_nested_is_refused():
+    dims = (4, 4, 4)
+    o = orc.Sim(*dims, 0.7, boxes=[([0, 0, 0], [3, 3, 3])])
+    o.init(np.ones(64), np.zeros(192))
+    o.add_fine([([2, 2, 2], [5, 5, 5])], [])
+    # grown by 2 ghosts, level-2 cells 2..3 have parents at level-1 cell 1,
+    # outside level 1's valid box 2..5
+    with pytest.raises(ValueError):
+        o.add_fine([([4, 6, 6], [7, 9, 9])], [])
+    o.add_fine([([6, 6, 6], [9, 9, 9])], [])  # nested: accepted
+    o.close()
