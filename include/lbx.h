@@ -39,6 +39,9 @@ extern "C" {
 #define LBX_E_NOMEM (-4)    /* device or host allocation failed */
 #define LBX_E_UNSUPPORTED (-5)
 
+/* Levels 0 .. LBX_MAX_LEVELS-1, refinement ratio 2 between neighbours. */
+#define LBX_MAX_LEVELS 6
+
 #define LBX_FP64 0          /* populations stored and collided in FP64 */
 #define LBX_FP32_SHIFTED 1  /* stored as f_i - w_i*rho0 in FP32, collided in FP32; the
                                two-level average-down sums and all reductions in FP64 */
@@ -67,8 +70,8 @@ int lbx_mem_info(lbx_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
  * geometry and boundary set-up (src/AmrSim.cpp:753-802).  periodic[d] != 0
  * wraps direction d; periodic[d] == 0 puts halfway bounce-back walls on
  * both faces normal to d (the reference aborts instead, :794-796; walls are
- * supported on single-level simulations, and level 1 is then refused with
- * LBX_E_UNSUPPORTED).  rho0 is the reference density of LBX_FP32_SHIFTED
+ * supported on single-level simulations, and refined levels are then
+ * refused with LBX_E_UNSUPPORTED).  rho0 is the reference density of LBX_FP32_SHIFTED
  * storage (ignored for LBX_FP64). */
 int lbx_sim_create(lbx_ctx* ctx, const int32_t dims[3], const int32_t periodic[3],
                    int precision, double rho0, lbx_sim** out);
@@ -80,17 +83,22 @@ int lbx_sim_destroy(lbx_sim* sim);
  * (src/AmrSim.cpp:665-713, include/multilevel.h:62-67). */
 int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
 
-/* Level 1 (refinement ratio 2, fine index space): boxes must lie in
- * [0, 2*dims), be aligned to the ratio (lo even, hi odd) and not overlap.
- * Each fine box is stored with 2 ghost layers like the reference's FABs.
- * Defining level 0 again releases level 1. */
+/* Level l >= 1 (refinement ratio 2, index space [0, 2^l * dims)): level
+ * l-1 must be defined; boxes must lie in the level's domain, be aligned to
+ * the ratio (lo even, hi odd), not overlap, and be properly nested: the
+ * parent of every cell of each box grown by 2 ghost cells that no box of
+ * level l holds (periodic images included), and of every valid cell, must
+ * be a valid cell of level l-1 (LBX_E_ARG otherwise; AMReX builds its grids
+ * that way).  Each box is stored with 2 ghost layers like the reference's
+ * FABs.  Defining a level releases every finer level. */
 
-/* Replace level 1's boxes, keeping its data (RemakeLevel,
- * src/AmrSim.cpp:715-744, with DistFnFillPatch, :359-391): every valid cell
- * of the new boxes takes the old fine populations where an old fine box
- * covered it, else its coarse parent's (piecewise-constant interpolation).
- * Box rules as for lbx_level_define; relaxation rates carry over; the fine
- * mask is not changed. */
+/* Replace the boxes of the finest level l >= 1, keeping its data
+ * (RemakeLevel, src/AmrSim.cpp:715-744, with DistFnFillPatch, :359-391):
+ * every valid cell of the new boxes takes the old populations where an old
+ * box covered it, else its parent's (piecewise-constant interpolation).
+ * Box rules as for lbx_level_define; relaxation rates carry over; the
+ * parent's mask is not changed.  LBX_E_STATE if level l has finer levels
+ * (clear them first). */
 int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
 
 /* ErrorEst on device data.  The reference tags only a static box
@@ -115,21 +123,26 @@ int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint
 int lbx_cluster_tags(const int32_t dims[3], const uint8_t* tags, int32_t blocking, int32_t buffer,
                      int32_t max_grid_size, int32_t max_boxes, int32_t* lohi, int32_t* n_boxes);
 
-/* Release a level's device storage (ClearLevel, src/AmrSim.cpp:746-751). */
+/* Release a level's device storage and that of every finer level
+ * (ClearLevel, src/AmrSim.cpp:746-751). */
 int lbx_level_clear(lbx_sim* sim, int level);
 
 /* Relaxation rates of a level, omega = 1/(tau + 1/2) computed by the caller
  * exactly as src/AmrSim.cpp:125-126. */
 int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b);
 
-/* Mask of the coarse collide (MakeFineMask, src/AmrSim.cpp:419-428):
- * coarse cells inside coarsen(box, 2) of any of the given boxes (fine index
- * space) have their populations zeroed instead of collided in the
- * two-level cycle.  The reference passes the coarse BoxArray itself as
- * these boxes; passing the fine BoxArray masks the refined region. */
+/* Mask of level `level`'s collide when it has a finer level (MakeFineMask,
+ * src/AmrSim.cpp:419-428, used by CoarseCollide :487-500): its valid cells
+ * inside coarsen(box, 2) of any of the given boxes (index space of level
+ * level+1) have their populations zeroed instead of collided in the Rohde
+ * cycle.  The reference passes the level's own BoxArray as these boxes;
+ * passing the finer level's BoxArray masks the refined region.  A level
+ * starts unmasked. */
+int lbx_set_level_mask(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi_fine);
+/* lbx_set_level_mask of level 0. */
 int lbx_set_fine_mask(lbx_sim* sim, int n_boxes, const int32_t* lohi_fine);
 
-/* Fill level 1's valid cells from level 0 by piecewise-constant
+/* Fill level l's valid cells from level l-1 by piecewise-constant
  * interpolation (DistFnFillFromCoarse, src/AmrSim.cpp:393-411). */
 int lbx_fill_from_coarse(lbx_sim* sim, int level);
 
@@ -146,10 +159,12 @@ int lbx_download_f(lbx_sim* sim, int level, int box, double* f);
 
 /* Advance nsteps level-0 steps (Iterate, src/AmrSim.cpp:981-993): the
  * single-level collide-stream step (src/AmrSim.cpp:124-135, 109-122,
- * 324-333) on every box, or, when level 1 is defined, one Rohde cycle per
- * step (src/AmrSim.cpp:430-631: masked coarse collide and stream, two fine
- * substeps, average-down and add).  Stream-ordered: may return before the
- * GPU finishes; lbx_sync() waits. */
+ * 324-333) on every box, or, when finer levels are defined, one Rohde cycle
+ * per step (src/AmrSim.cpp:430-631: masked collide and stream of each
+ * level, two steps of each level per step of its parent, two substeps of
+ * the finest level per step of its parent, average-down and add).  A level
+ * l advances 2^l steps per call step.  Stream-ordered: may return before
+ * the GPU finishes; lbx_sync() waits. */
 int lbx_iterate(lbx_sim* sim, int nsteps);
 
 /* Density and velocity of one box's valid cells (CalcHydroVars,

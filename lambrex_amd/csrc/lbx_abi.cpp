@@ -59,9 +59,12 @@ struct Level {
   // coarse push (launch_coarse_push), and its launch shape
   unsigned char* d_mask = nullptr;
   LaunchShape push_shape{};
-  // level 1: ghost fill sources and the average-down contributions
+  // patched levels: ghost fill sources and the average-down contributions
+  // into the parent; d_mask is then one byte per grown cell (the parent's
+  // view of this level's own finer level)
   long long* d_fill = nullptr;
-  long long* d_sum_cell = nullptr;  // coarse element of population 0 per touched cell
+  long long grown_cells = 0;        // length of the fill table
+  long long* d_sum_cell = nullptr;  // parent cell reference per touched cell
   int* d_sum_start = nullptr;
   int* d_sum_contrib = nullptr;  // index into the block averages, per contribution
   int sum_ncell = 0;
@@ -110,7 +113,7 @@ struct lbx_sim {
   unsigned walls = 0;  // bit d: bounce-back walls normal to axis d
   int precision = LBX_FP64;
   double rho0 = 1.0;
-  Level lev[2];
+  Level lev[LBX_MAX_LEVELS];
   StepGraphs graphs;
   bool use_graphs = true;
 };
@@ -205,7 +208,7 @@ int check_sim(lbx_sim* sim) {
 }
 
 int check_box(lbx_sim* sim, int level, int box) {
-  if (level < 0 || level > 1) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
+  if (level < 0 || level >= LBX_MAX_LEVELS) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
   Level& L = sim->lev[level];
   if (!L.defined) return fail(sim->ctx, LBX_E_STATE, "level %d is not defined", level);
   if (box < 0 || box >= (int)L.boxes.size())
@@ -378,21 +381,35 @@ int launch_step(lbx_sim* sim, Level& L, lbx::StepKind kind) {
   return ps.close();
 }
 
-// The kernels of one coarse step of the two-level cycle
-// (src/AmrSim.cpp:430-469) for the current buffer parities.
+// The finest defined level (levels are defined contiguously from 0).
+int finest_level(const lbx_sim* sim) {
+  int f = 0;
+  while (f + 1 < LBX_MAX_LEVELS && sim->lev[f + 1].defined) ++f;
+  return f;
+}
+
+void free_levels_from(lbx_sim* sim, int level) {
+  for (int l = level; l < LBX_MAX_LEVELS; ++l) free_level(sim->lev[l]);
+}
+
+// The kernels of one level-0 step of the Rohde cycle (src/AmrSim.cpp:430-469)
+// for the current buffer parities.
 constexpr int kFinePatchGroup = 16;  // patches per fine launch group (0 = all)
 
-hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
-  Level& C = sim->lev[0];
-  Level& F = sim->lev[1];
+// The finest level's two substeps, from its parent's current state
+// (level 0's `now`, or a middle level's state, which stays put until its own
+// stream).  Its block averages land in F.d_avg; fpar is the buffer holding
+// its state and flips.
+hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
+  Level& F = sim->lev[f];
+  const Level& C = sim->lev[f - 1];
   const int prec = sim->precision;
-  const Relax rc{C.omega_s, C.omega_b, sim->rho0};
   const Relax rf{F.omega_s, F.omega_b, sim->rho0};
   const void* cnow = C.buf[C.cur];
-  void* cnext = C.buf[1 - C.cur];
-  void* fnow = F.buf[F.cur];
-  void* fnext = F.buf[1 - F.cur];
+  void* fnow = F.buf[fpar];
+  void* fnext = F.buf[1 - fpar];
   void* ftmp = F.buf[2];
+  const PatchDesc* cpatches = C.patched ? C.d_desc : nullptr;
   const int np = (int)F.boxes.size();
   // The fine patches' two substeps depend on no other patch's substeps (the
   // ghosts are filled once, from fnow and cnow), so they run in groups of
@@ -401,38 +418,94 @@ hipError_t enqueue_two_level_step(lbx_sim* sim, hipStream_t st) {
   hipError_t e = hipSuccess;
   for (int first = 0; first < np && e == hipSuccess; first += group) {
     const int n = std::min(group, np - first);
-    e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, F.d_desc, first, n, F.max_cells, F.d_fill,
-                                      rf, st);
+    e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, cpatches, F.d_desc, first, n, F.max_cells,
+                                      F.d_fill, nullptr, rf, st);
     if (e == hipSuccess)
       e = lbx::launch_fine_stream_collide(prec, fnext, ftmp, F.d_desc, first, n, F.max_cells, rf, st);
     if (e == hipSuccess)
       e = lbx::launch_fine_stream_avg(prec, ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n, F.max_blocks,
                                       sim->rho0, st);
   }
-  if (e == hipSuccess) e = lbx::launch_coarse_push(prec, cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
-  // the fine state after the step is fnext
-  if (e == hipSuccess)
-    e = lbx::launch_sum_apply(prec, F.d_avg, F.avg_stride, cnext, C.lat, F.d_sum_cell, F.d_sum_start,
-                              F.d_sum_contrib, F.sum_ncell, sim->rho0, st);
+  fpar = 1 - fpar;
   return e;
 }
 
-// One coarse step: replayed from a captured graph, or launched eagerly
+hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStream_t st);
+
+// One step of a middle level m (0 < m < finest), RohdeCycle(m): the
+// two-level fill from the parent with comp 0 zeroed on the outer layer and
+// the masked collide into the other buffer; the finer levels' steps, which
+// read m's state; then Stream + ZeroInvalidComponents back into the state
+// buffer and SumFromFine into its valid cells.  The state stays in buf[cur].
+hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipStream_t st) {
+  Level& L = sim->lev[m];
+  const Level& P = sim->lev[m - 1];
+  const Level& F = sim->lev[m + 1];
+  const int prec = sim->precision;
+  const Relax r{L.omega_s, L.omega_b, sim->rho0};
+  const int np = (int)L.boxes.size();
+  hipError_t e = lbx::launch_fine_fill_collide(prec, L.buf[L.cur], L.buf[1 - L.cur], P.buf[P.cur], P.lat,
+                                               P.patched ? P.d_desc : nullptr, L.d_desc, 0, np, L.max_cells,
+                                               L.d_fill, L.d_mask, r, st);
+  if (e == hipSuccess) e = enqueue_children(sim, m, finest, fpar, st);
+  if (e == hipSuccess)
+    e = lbx::launch_fine_stream_zero(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_desc, 0, np, L.max_cells, sim->rho0,
+                                     st);
+  if (e == hipSuccess)
+    e = lbx::launch_sum_apply(prec, F.d_avg, F.avg_stride, L.buf[L.cur], L.lat, L.d_desc, F.d_sum_cell,
+                              F.d_sum_start, F.d_sum_contrib, F.sum_ncell, sim->rho0, st);
+  return e;
+}
+
+// What level l's step needs of level l + 1: the finest level's two
+// substeps, or refRatio = 2 middle steps followed by the block averages of
+// the result (src/AmrSim.cpp:445-455).
+hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStream_t st) {
+  const int c = l + 1;
+  if (c == finest) return enqueue_finest_pair(sim, c, fpar, st);
+  Level& M = sim->lev[c];
+  hipError_t e = enqueue_middle_step(sim, c, finest, fpar, st);
+  if (e == hipSuccess) e = enqueue_middle_step(sim, c, finest, fpar, st);
+  if (e == hipSuccess)
+    e = lbx::launch_patch_avg(sim->precision, M.buf[M.cur], M.d_avg, M.avg_stride, M.d_desc, (int)M.boxes.size(),
+                              M.max_blocks, sim->rho0, st);
+  return e;
+}
+
+hipError_t enqueue_hierarchy_step(lbx_sim* sim, hipStream_t st) {
+  Level& C = sim->lev[0];
+  const Level& F = sim->lev[1];
+  const int finest = finest_level(sim);
+  const Relax rc{C.omega_s, C.omega_b, sim->rho0};
+  const void* cnow = C.buf[C.cur];
+  void* cnext = C.buf[1 - C.cur];
+  int fpar = sim->lev[finest].cur;
+  hipError_t e = enqueue_children(sim, 0, finest, fpar, st);
+  if (e == hipSuccess)
+    e = lbx::launch_coarse_push(sim->precision, cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
+  if (e == hipSuccess)
+    e = lbx::launch_sum_apply(sim->precision, F.d_avg, F.avg_stride, cnext, C.lat, nullptr, F.d_sum_cell,
+                              F.d_sum_start, F.d_sum_contrib, F.sum_ncell, sim->rho0, st);
+  return e;
+}
+
+// One level-0 step: replayed from a captured graph, or launched eagerly
 // while profiling (event records cannot sit inside the capture).
-int launch_two_level_step(lbx_sim* sim) {
+int launch_hierarchy_step(lbx_sim* sim) {
   lbx_ctx* ctx = sim->ctx;
   Level& C = sim->lev[0];
-  Level& F = sim->lev[1];
+  const int finest = finest_level(sim);
+  Level& F = sim->lev[finest];
   if (sim->use_graphs && !ctx->prof) {
     hipGraphExec_t& ex = sim->graphs.exec[C.cur][F.cur];
     if (!ex) {
       hipGraph_t g = nullptr;
       LBX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const hipError_t ek = enqueue_two_level_step(sim, ctx->stream);
+      const hipError_t ek = enqueue_hierarchy_step(sim, ctx->stream);
       const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
       if (ek != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
-        return hip_fail(ctx, ek, "two-level step capture");
+        return hip_fail(ctx, ek, "hierarchy step capture");
       }
       if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture");
       const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -446,10 +519,12 @@ int launch_two_level_step(lbx_sim* sim) {
   } else {
     ProfileScope ps(ctx, true);
     if (ps.rc) return ps.rc;
-    LBX_HIP(ctx, enqueue_two_level_step(sim, ctx->stream));
+    LBX_HIP(ctx, enqueue_hierarchy_step(sim, ctx->stream));
     if (int rc = ps.close()) return rc;
   }
-  F.cur = 1 - F.cur;
+  // the finest level takes 2^(finest-1) substep pairs per level-0 step; its
+  // buffer parity only changes when that count is odd
+  if (finest == 1) F.cur = 1 - F.cur;
   C.cur = 1 - C.cur;
   return LBX_OK;
 }
@@ -475,8 +550,7 @@ int define_level0(lbx_sim* sim, const std::vector<Box>& boxes) {
   if (total != (long long)sim->dims[0] * sim->dims[1] * sim->dims[2] || any_overlap(boxes))
     return fail(ctx, LBX_E_ARG, "level-0 boxes must tile the periodic domain exactly");
   Level& L = sim->lev[0];
-  free_level(sim->lev[1]);
-  free_level(L);
+  free_levels_from(sim, 0);
   L.boxes = boxes;
   L.lat = lbx::make_lattice(sim->dims[0], sim->dims[1], sim->dims[2]);
   L.lat.walls = sim->walls;
@@ -533,23 +607,97 @@ std::vector<unsigned char> coarse_flags(const Level& C, const std::vector<Box>& 
   return fl;
 }
 
-// Fine level: patches grown by 2; ghost-fill sources; average-down table.
-// Builds a fine level over `boxes` into F (a fresh Level); on failure F holds
-// nothing.
-int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
+// Grown index of absolute position (x, y, z) in patch P.
+long long grown_index(const PatchDesc& P, int x, int y, int z) {
+  return ((long long)(z - P.lo[2] + 2) * P.g[1] + (y - P.lo[1] + 2)) * P.g[0] + (x - P.lo[0] + 2);
+}
+
+// Where a level keeps the valid cell at (x, y, z) of its own index space
+// (already wrapped into the domain): the element of population 0 in a level
+// array, or (patch << 32) | grown index on a patched level.  False if no
+// valid cell of the level holds it.  `hint` is the box found last.
+bool cell_ref(const Level& C, int x, int y, int z, long long* ref, int& hint) {
+  if (!C.patched) {
+    *ref = C.lat.at(0, y * C.lat.nx + x, z);
+    return true;
+  }
+  const int nb = (int)C.boxes.size();
+  int src = -1;
+  if (hint < nb && C.boxes[hint].contains(x, y, z)) {
+    src = hint;
+  } else {
+    for (int s = 0; s < nb; ++s)
+      if (C.boxes[s].contains(x, y, z)) {
+        src = s;
+        break;
+      }
+  }
+  if (src < 0) return false;
+  hint = src;
+  *ref = ((long long)src << 32) | grown_index(C.desc[src], x, y, z);
+  return true;
+}
+
+// Mask bytes of a patched level, one per grown cell: 1 where a valid cell
+// lies in coarsen(mask box, 2) for some mask box (finer index space).
+std::vector<unsigned char> patch_mask(const Level& C, const std::vector<Box>& mask) {
+  std::vector<unsigned char> m((size_t)C.grown_cells, 0);
+  for (size_t b = 0; b < C.boxes.size(); ++b) {
+    const PatchDesc& P = C.desc[b];
+    for (const Box& mb : mask) {
+      int lo[3], hi[3];
+      bool empty = false;
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::max(P.lo[d], floor_div2(mb.lo[d]));
+        hi[d] = std::min(P.hi[d], floor_div2(mb.hi[d]));
+        empty |= lo[d] > hi[d];
+      }
+      if (empty) continue;
+      for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+          for (int x = lo[0]; x <= hi[0]; ++x) m[P.fill_base + grown_index(P, x, y, z)] = 1;
+    }
+  }
+  return m;
+}
+
+// Level l >= 1: patches grown by 2; ghost-fill sources; average-down table
+// into level l - 1.  Builds the level over `boxes` into F (a fresh Level); on
+// failure F holds nothing.
+int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F) {
   lbx_ctx* ctx = sim->ctx;
-  Level& C = sim->lev[0];
-  if (!C.defined) return fail(ctx, LBX_E_STATE, "define level 0 before level 1");
-  if (sim->walls) return fail(ctx, LBX_E_UNSUPPORTED, "two-level runs need a fully periodic domain");
-  const int fdom[3] = {2 * sim->dims[0], 2 * sim->dims[1], 2 * sim->dims[2]};
+  Level& C = sim->lev[level - 1];
+  if (!C.defined) return fail(ctx, LBX_E_STATE, "define level %d before level %d", level - 1, level);
+  if (sim->walls) return fail(ctx, LBX_E_UNSUPPORTED, "refined levels need a fully periodic domain");
+  int fdom[3];
+  for (int d = 0; d < 3; ++d) {
+    if (((long long)sim->dims[d] << level) > (1ll << 30))
+      return fail(ctx, LBX_E_ARG, "level %d's index space exceeds 2^30 cells per direction", level);
+    fdom[d] = sim->dims[d] << level;
+  }
   for (size_t b = 0; b < boxes.size(); ++b) {
     if (!box_ok(boxes[b], fdom))
-      return fail(ctx, LBX_E_ARG, "fine box %zu lies outside the refined domain or is empty", b);
+      return fail(ctx, LBX_E_ARG, "level-%d box %zu lies outside the refined domain or is empty", level, b);
     for (int d = 0; d < 3; ++d)
       if (boxes[b].lo[d] % 2 != 0 || boxes[b].hi[d] % 2 != 1)
-        return fail(ctx, LBX_E_ARG, "fine box %zu is not aligned to refinement ratio 2", b);
+        return fail(ctx, LBX_E_ARG, "level-%d box %zu is not aligned to refinement ratio 2", level, b);
   }
-  if (any_overlap(boxes)) return fail(ctx, LBX_E_ARG, "fine boxes overlap");
+  if (any_overlap(boxes)) return fail(ctx, LBX_E_ARG, "level-%d boxes overlap", level);
+  if (boxes.size() > 65535)  // one grid row of workgroups per patch
+    return fail(ctx, LBX_E_ARG, "level %d has more than 65535 boxes", level);
+  // Proper nesting: the parent of every valid cell lies in the parent
+  // level's valid region (the parent of a ghost is checked with the fill).
+  int chint = 0;
+  if (C.patched)
+    for (size_t b = 0; b < boxes.size(); ++b)
+      for (int z = boxes[b].lo[2] / 2; z <= boxes[b].hi[2] / 2; ++z)
+        for (int y = boxes[b].lo[1] / 2; y <= boxes[b].hi[1] / 2; ++y)
+          for (int x = boxes[b].lo[0] / 2; x <= boxes[b].hi[0] / 2; ++x) {
+            long long ref;
+            if (!cell_ref(C, x, y, z, &ref, chint))
+              return fail(ctx, LBX_E_ARG, "level-%d box %zu is not properly nested in level %d", level, b,
+                          level - 1);
+          }
   F = Level{};
   F.boxes = boxes;
   F.patched = true;
@@ -564,7 +712,7 @@ int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
     const long long cells = (long long)d.g[0] * d.g[1] * d.g[2];
     if (cells >= (1ll << 31) || fill_base + cells >= (1ll << 32)) {
       F = Level{};
-      return fail(ctx, LBX_E_ARG, "fine box too large");
+      return fail(ctx, LBX_E_ARG, "level-%d box too large", level);
     }
     d.cells = (int)cells;
     d.qs = (cells + 63) / 64 * 64;
@@ -581,31 +729,12 @@ int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
   }
   if (avg_base >= (1ll << 31)) {
     F = Level{};
-    return fail(ctx, LBX_E_ARG, "fine level too large");
+    return fail(ctx, LBX_E_ARG, "level %d too large", level);
   }
-  F.avg_stride = (avg_base + 63) / 64 * 64;
-  const size_t bytes = (size_t)base * elem_size(sim->precision);
-  {
-    const size_t abytes = (size_t)F.avg_stride * lbx::NV_TOTAL * sizeof(double);
-    hipError_t e = hipMalloc(&F.d_avg, abytes);
-    if (e != hipSuccess) {
-      free_level(F);
-      return fail(ctx, LBX_E_NOMEM, "allocation of %zu bytes for level 1 failed: %s", abytes,
-                  hipGetErrorString(e));
-    }
-  }
-  for (int i = 0; i < 3; ++i) {
-    hipError_t e = hipMalloc(&F.buf[i], bytes);
-    if (e == hipSuccess) e = hipMemsetAsync(F.buf[i], 0, bytes, ctx->stream);
-    if (e != hipSuccess) {
-      free_level(F);
-      return fail(ctx, LBX_E_NOMEM, "allocation of %zu bytes for level 1 failed: %s", bytes,
-                  hipGetErrorString(e));
-    }
-  }
+  F.grown_cells = fill_base;
   // Ghost fill sources (FillPatchTwoLevels with piecewise-constant
-  // interpolation, src/AmrSim.cpp:373-388): the fine valid cell holding the
-  // periodic image if a fine box has it, else the coarse parent.
+  // interpolation, src/AmrSim.cpp:373-388): the valid cell of this level
+  // holding the periodic image if a box has it, else the coarse parent.
   std::vector<long long> fill((size_t)fill_base, 0);
   int hint = 0;
   for (size_t b = 0; b < boxes.size(); ++b) {
@@ -629,26 +758,30 @@ int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
           long long d;
           if (src >= 0) {
             hint = src;
-            const PatchDesc& S = F.desc[src];
-            const long long li = ((long long)(wz - S.lo[2] + 2) * S.g[1] + (wy - S.lo[1] + 2)) * S.g[0] +
-                                 (wx - S.lo[0] + 2);
-            d = ((long long)src << 32) | li;
+            d = ((long long)src << 32) | grown_index(F.desc[src], wx, wy, wz);
           } else {
-            d = -1 - C.lat.at(0, (wy / 2) * C.lat.nx + (wx / 2), wz / 2);
+            long long ref;
+            if (!cell_ref(C, wx / 2, wy / 2, wz / 2, &ref, chint)) {
+              F = Level{};
+              return fail(ctx, LBX_E_ARG, "level-%d box %zu is not properly nested in level %d", level, b,
+                          level - 1);
+            }
+            d = -1 - ref;
           }
           fill[P.fill_base + ((long long)gz * P.g[1] + gy) * P.g[0] + gx] = d;
         }
   }
   // Average-down contributions (sum_fine_to_coarse, src/AmrSim.cpp:592-602):
-  // each fine box's coarsened box grown by one coarse cell, periodic image
-  // in the coarse level; per coarse cell, contributions in box order and,
-  // within a box, in (i, j, k) loop order.
-  // A contribution is the average of one 2x2x2 block of a grown patch,
-  // stored by k_fine_stream_avg at avg_base + block index.
+  // each box's coarsened box grown by one coarse cell, periodic image in the
+  // parent level, added only where a valid parent cell holds it; per parent
+  // cell, contributions in box order and, within a box, in (i, j, k) loop
+  // order.  A contribution is the average of one 2x2x2 block of a grown
+  // patch, stored at avg_base + block index.
   struct Contrib {
-    long long cell;  // coarse element of population 0
+    long long cell;  // parent cell reference
     int avg;         // block-average index
   };
+  const int cdom[3] = {sim->dims[0] << (level - 1), sim->dims[1] << (level - 1), sim->dims[2] << (level - 1)};
   std::vector<Contrib> list;
   for (size_t b = 0; b < boxes.size(); ++b) {
     const Box& B = boxes[b];
@@ -658,9 +791,10 @@ int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
     for (int ci = i0; ci <= floor_div2(B.hi[0]) + 1; ++ci)
       for (int cj = j0; cj <= floor_div2(B.hi[1]) + 1; ++cj)
         for (int ck = k0; ck <= floor_div2(B.hi[2]) + 1; ++ck) {
-          const long long c = C.lat.at(0, wrap(cj, C.lat.ny) * C.lat.nx + wrap(ci, C.lat.nx), wrap(ck, C.lat.nz));
+          long long ref;
+          if (!cell_ref(C, wrap(ci, cdom[0]), wrap(cj, cdom[1]), wrap(ck, cdom[2]), &ref, chint)) continue;
           const long long blk = ((long long)(ck - k0) * c1 + (cj - j0)) * c0 + (ci - i0);
-          list.push_back({c, (int)(P.avg_base + blk)});
+          list.push_back({ref, (int)(P.avg_base + blk)});
         }
   }
   std::stable_sort(list.begin(), list.end(), [](const Contrib& a, const Contrib& b) { return a.cell < b.cell; });
@@ -676,54 +810,82 @@ int build_level1(lbx_sim* sim, const std::vector<Box>& boxes, Level& F) {
   }
   start.push_back((int)list.size());
   F.sum_ncell = (int)cell.size();
+  F.avg_stride = (avg_base + 63) / 64 * 64;
+  const size_t bytes = (size_t)base * elem_size(sim->precision);
+  {
+    const size_t abytes = (size_t)F.avg_stride * lbx::NV_TOTAL * sizeof(double);
+    hipError_t e = hipMalloc(&F.d_avg, abytes);
+    if (e != hipSuccess) {
+      free_level(F);
+      return fail(ctx, LBX_E_NOMEM, "allocation of %zu bytes for level %d failed: %s", abytes, level,
+                  hipGetErrorString(e));
+    }
+  }
+  for (int i = 0; i < 3; ++i) {
+    hipError_t e = hipMalloc(&F.buf[i], bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(F.buf[i], 0, bytes, ctx->stream);
+    if (e != hipSuccess) {
+      free_level(F);
+      return fail(ctx, LBX_E_NOMEM, "allocation of %zu bytes for level %d failed: %s", bytes, level,
+                  hipGetErrorString(e));
+    }
+  }
   hipError_t e = upload(&F.d_desc, F.desc, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_fill, fill, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_cell, cell, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_start, start, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_contrib, contrib, ctx->stream);
   if (e == hipSuccess && !C.d_mask) {
-    const std::vector<unsigned char> flags = coarse_flags(C, {});
-    e = upload(&C.d_mask, flags, ctx->stream);
+    if (C.patched)
+      e = upload(&C.d_mask, patch_mask(C, {}), ctx->stream);
+    else
+      e = upload(&C.d_mask, coarse_flags(C, {}), ctx->stream);
   }
-  C.push_shape = lbx::default_shape(C.lat, sim->precision);
+  if (!C.patched) C.push_shape = lbx::default_shape(C.lat, sim->precision);
   if (e != hipSuccess) {
     free_level(F);
-    return hip_fail(ctx, e, "level-1 tables");
+    return hip_fail(ctx, e, "level tables");
   }
   F.defined = true;
   return LBX_OK;
 }
 
-int define_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
+// A level replaces any level at its index and discards the finer ones.
+int define_level(lbx_sim* sim, int level, const std::vector<Box>& boxes) {
   Level nf;
-  if (int rc = build_level1(sim, boxes, nf)) return rc;
-  free_level(sim->lev[1]);
-  sim->lev[1] = nf;
+  if (int rc = build_level(sim, level, boxes, nf)) return rc;
+  free_levels_from(sim, level);
+  sim->lev[level] = nf;
   return LBX_OK;
 }
 
-// RemakeLevel (src/AmrSim.cpp:715-744): a new fine BoxArray whose valid cells
-// take the old fine level's values where an old fine box covers them and the
-// coarse parent's otherwise (DistFnFillPatch: FillPatchTwoLevels with
+// RemakeLevel (src/AmrSim.cpp:715-744) of the finest level: a new BoxArray
+// whose valid cells take the old level's values where an old box covers
+// them and the parent's otherwise (DistFnFillPatch: FillPatchTwoLevels with
 // piecewise-constant interpolation, src/AmrSim.cpp:359-391).  Relaxation
 // rates carry over.
-int remake_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
+int remake_level(lbx_sim* sim, int level, const std::vector<Box>& boxes) {
   lbx_ctx* ctx = sim->ctx;
-  Level& C = sim->lev[0];
-  Level& F = sim->lev[1];
-  if (!F.defined) return fail(ctx, LBX_E_STATE, "level 1 is not defined");
+  Level& C = sim->lev[level - 1];
+  Level& F = sim->lev[level];
+  if (!F.defined) return fail(ctx, LBX_E_STATE, "level %d is not defined", level);
+  if (finest_level(sim) != level)
+    return fail(ctx, LBX_E_STATE, "level %d has finer levels; clear them before remaking it", level);
   Level nf;
-  if (int rc = build_level1(sim, boxes, nf)) return rc;
+  if (int rc = build_level(sim, level, boxes, nf)) return rc;
   // Source of every new valid cell: (old patch << 32) | old grown index, or
-  // -1 - coarse element of population 0.
-  std::vector<long long> src(nf.desc.empty() ? 0 : (size_t)(nf.desc.back().fill_base + nf.desc.back().cells), 0);
+  // -1 - parent reference.
+  std::vector<long long> src((size_t)nf.grown_cells, 0);
+  int chint = 0;
   for (size_t b = 0; b < boxes.size(); ++b) {
     const PatchDesc& P = nf.desc[b];
     for (int z = P.lo[2]; z <= P.hi[2]; ++z)
       for (int y = P.lo[1]; y <= P.hi[1]; ++y)
-        for (int x = P.lo[0]; x <= P.hi[0]; ++x)
-          src[P.fill_base + ((long long)(z - P.lo[2] + 2) * P.g[1] + (y - P.lo[1] + 2)) * P.g[0] + (x - P.lo[0] + 2)] =
-              -1 - C.lat.at(0, (y / 2) * C.lat.nx + x / 2, z / 2);
+        for (int x = P.lo[0]; x <= P.hi[0]; ++x) {
+          long long ref = 0;
+          (void)cell_ref(C, x / 2, y / 2, z / 2, &ref, chint);  // nesting checked by build_level
+          src[P.fill_base + grown_index(P, x, y, z)] = -1 - ref;
+        }
     for (size_t o = 0; o < F.boxes.size(); ++o) {
       const PatchDesc& O = F.desc[o];
       int lo[3], hi[3];
@@ -736,11 +898,8 @@ int remake_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
       if (empty) continue;
       for (int z = lo[2]; z <= hi[2]; ++z)
         for (int y = lo[1]; y <= hi[1]; ++y)
-          for (int x = lo[0]; x <= hi[0]; ++x) {
-            const long long li = ((long long)(z - O.lo[2] + 2) * O.g[1] + (y - O.lo[1] + 2)) * O.g[0] + (x - O.lo[0] + 2);
-            src[P.fill_base + ((long long)(z - P.lo[2] + 2) * P.g[1] + (y - P.lo[1] + 2)) * P.g[0] +
-                (x - P.lo[0] + 2)] = ((long long)o << 32) | li;
-          }
+          for (int x = lo[0]; x <= hi[0]; ++x)
+            src[P.fill_base + grown_index(P, x, y, z)] = ((long long)o << 32) | grown_index(O, x, y, z);
     }
   }
   DeviceTemp t;
@@ -748,18 +907,48 @@ int remake_level1(lbx_sim* sim, const std::vector<Box>& boxes) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(t.p, src.data(), src.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
-    e = lbx::launch_fine_remake(sim->precision, nf.buf[0], F.buf[F.cur], C.buf[C.cur], C.lat, nf.d_desc,
-                                (int)nf.desc.size(), nf.max_cells, F.d_desc, (const long long*)t.p, ctx->stream);
+    e = lbx::launch_fine_remake(sim->precision, nf.buf[0], F.buf[F.cur], C.buf[C.cur], C.lat,
+                                C.patched ? C.d_desc : nullptr, nf.d_desc, (int)nf.desc.size(), nf.max_cells,
+                                F.d_desc, (const long long*)t.p, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     free_level(nf);
-    return hip_fail(ctx, e, "level-1 remake");
+    return hip_fail(ctx, e, "level remake");
   }
   nf.omega_s = F.omega_s;
   nf.omega_b = F.omega_b;
   nf.relax_set = F.relax_set;
   free_level(F);
   F = nf;
+  return LBX_OK;
+}
+
+// DistFnFillFromCoarse (src/AmrSim.cpp:393-411) for a level whose parent is
+// patched: every valid cell copies its parent through the remake kernel.
+int fill_from_patched(lbx_sim* sim, int level) {
+  lbx_ctx* ctx = sim->ctx;
+  const Level& C = sim->lev[level - 1];
+  Level& F = sim->lev[level];
+  std::vector<long long> src((size_t)F.grown_cells, 0);
+  int chint = 0;
+  for (size_t b = 0; b < F.boxes.size(); ++b) {
+    const PatchDesc& P = F.desc[b];
+    for (int z = P.lo[2]; z <= P.hi[2]; ++z)
+      for (int y = P.lo[1]; y <= P.hi[1]; ++y)
+        for (int x = P.lo[0]; x <= P.hi[0]; ++x) {
+          long long ref;
+          if (!cell_ref(C, x / 2, y / 2, z / 2, &ref, chint))
+            return fail(ctx, LBX_E_STATE, "level %d is not nested in level %d", level, level - 1);
+          src[P.fill_base + grown_index(P, x, y, z)] = -1 - ref;
+        }
+  }
+  DeviceTemp t;
+  LBX_HIP(ctx, hipMalloc(&t.p, src.size() * sizeof(long long)));
+  LBX_HIP(ctx, hipMemcpyAsync(t.p, src.data(), src.size() * sizeof(long long), hipMemcpyHostToDevice, ctx->stream));
+  LBX_HIP(ctx, lbx::launch_fine_remake(sim->precision, F.buf[F.cur], nullptr, C.buf[C.cur], C.lat, C.d_desc,
+                                       F.d_desc, (int)F.boxes.size(), F.max_cells, F.d_desc, (const long long*)t.p,
+                                       ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }
 
@@ -862,25 +1051,26 @@ int lbx_sim_destroy(lbx_sim* sim) {
 int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) {
   if (int rc = check_sim(sim)) return rc;
   lbx_ctx* ctx = sim->ctx;
-  if (level < 0 || level > 1) return fail(ctx, LBX_E_UNSUPPORTED, "levels 0 and 1 are supported");
+  if (level < 0 || level >= LBX_MAX_LEVELS)
+    return fail(ctx, LBX_E_UNSUPPORTED, "levels 0 to %d are supported", LBX_MAX_LEVELS - 1);
   if (n_boxes < 1 || !lohi) return fail(ctx, LBX_E_ARG, "need at least one box");
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
   sim->graphs.clear();
-  return level == 0 ? define_level0(sim, boxes) : define_level1(sim, boxes);
+  return level == 0 ? define_level0(sim, boxes) : define_level(sim, level, boxes);
 }
 
 int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) {
   if (int rc = check_sim(sim)) return rc;
   lbx_ctx* ctx = sim->ctx;
-  if (level != 1) return fail(ctx, LBX_E_UNSUPPORTED, "only level 1 is remade");
+  if (level < 1 || level >= LBX_MAX_LEVELS) return fail(ctx, LBX_E_UNSUPPORTED, "level %d cannot be remade", level);
   if (n_boxes < 1 || !lohi) return fail(ctx, LBX_E_ARG, "need at least one box");
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
   sim->graphs.clear();
-  return remake_level1(sim, boxes);
+  return remake_level(sim, level, boxes);
 }
 
 int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint8_t* tags) {
@@ -906,17 +1096,16 @@ int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint
 
 int lbx_level_clear(lbx_sim* sim, int level) {
   if (int rc = check_sim(sim)) return rc;
-  if (level < 0 || level > 1) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
+  if (level < 0 || level >= LBX_MAX_LEVELS) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
   (void)hipStreamSynchronize(sim->ctx->stream);
   sim->graphs.clear();
-  if (level == 0) free_level(sim->lev[1]);
-  free_level(sim->lev[level]);
+  free_levels_from(sim, level);
   return LBX_OK;
 }
 
 int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b) {
   if (int rc = check_sim(sim)) return rc;
-  if (level < 0 || level > 1) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
+  if (level < 0 || level >= LBX_MAX_LEVELS) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
   if (!std::isfinite(omega_s) || !std::isfinite(omega_b))
     return fail(sim->ctx, LBX_E_ARG, "relaxation rates must be finite");
   sim->lev[level].omega_s = omega_s;
@@ -926,29 +1115,40 @@ int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b) {
   return LBX_OK;
 }
 
-int lbx_set_fine_mask(lbx_sim* sim, int n_boxes, const int32_t* lohi_fine) {
+int lbx_set_level_mask(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi_fine) {
   if (int rc = check_sim(sim)) return rc;
   lbx_ctx* ctx = sim->ctx;
-  Level& C = sim->lev[0];
-  if (!C.defined) return fail(ctx, LBX_E_STATE, "level 0 is not defined");
+  if (level < 0 || level >= LBX_MAX_LEVELS) return fail(ctx, LBX_E_ARG, "level %d out of range", level);
+  Level& C = sim->lev[level];
+  if (!C.defined) return fail(ctx, LBX_E_STATE, "level %d is not defined", level);
   if (n_boxes < 0 || (n_boxes > 0 && !lohi_fine)) return fail(ctx, LBX_E_ARG, "bad mask boxes");
-  const size_t ncell = (size_t)C.lat.nx * C.lat.ny * C.lat.nz;
-  const std::vector<unsigned char> flags = coarse_flags(C, read_boxes(n_boxes, lohi_fine));
+  const std::vector<Box> mask = read_boxes(n_boxes, lohi_fine);
+  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, mask) : coarse_flags(C, mask);
   LBX_HIP(ctx, hipSetDevice(ctx->device));
-  if (!C.d_mask) LBX_HIP(ctx, hipMalloc(&C.d_mask, ncell));
-  LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, flags.data(), ncell, hipMemcpyHostToDevice, ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (!C.d_mask) {
+    sim->graphs.clear();  // captured steps hold the mask's address
+    LBX_HIP(ctx, hipMalloc(&C.d_mask, bytes.size()));
+  }
+  LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, bytes.data(), bytes.size(), hipMemcpyHostToDevice, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
+}
+
+int lbx_set_fine_mask(lbx_sim* sim, int n_boxes, const int32_t* lohi_fine) {
+  return lbx_set_level_mask(sim, 0, n_boxes, lohi_fine);
 }
 
 int lbx_fill_from_coarse(lbx_sim* sim, int level) {
   if (int rc = check_sim(sim)) return rc;
   lbx_ctx* ctx = sim->ctx;
-  if (level != 1) return fail(ctx, LBX_E_ARG, "only level 1 is filled from a coarser level");
-  Level& C = sim->lev[0];
-  Level& F = sim->lev[1];
-  if (!C.defined || !F.defined) return fail(ctx, LBX_E_STATE, "levels 0 and 1 must be defined");
+  if (level < 1 || level >= LBX_MAX_LEVELS)
+    return fail(ctx, LBX_E_ARG, "level %d has no coarser level to fill from", level);
+  Level& C = sim->lev[level - 1];
+  Level& F = sim->lev[level];
+  if (!C.defined || !F.defined) return fail(ctx, LBX_E_STATE, "levels %d and %d must be defined", level - 1, level);
   LBX_HIP(ctx, hipSetDevice(ctx->device));
+  if (C.patched) return fill_from_patched(sim, level);
   LBX_HIP(ctx, lbx::launch_fine_from_coarse(sim->precision, F.buf[F.cur], C.buf[C.cur], C.lat, F.d_desc,
                                             (int)F.boxes.size(), F.max_cells, sim->rho0, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1028,10 +1228,11 @@ int lbx_iterate(lbx_sim* sim, int nsteps) {
   if (!L.relax_set) return fail(ctx, LBX_E_STATE, "relaxation rates of level 0 are not set");
   if (nsteps == 0) return LBX_OK;
   LBX_HIP(ctx, hipSetDevice(ctx->device));
-  if (sim->lev[1].defined) {
-    if (!sim->lev[1].relax_set) return fail(ctx, LBX_E_STATE, "relaxation rates of level 1 are not set");
+  if (const int finest = finest_level(sim)) {
+    for (int l = 1; l <= finest; ++l)
+      if (!sim->lev[l].relax_set) return fail(ctx, LBX_E_STATE, "relaxation rates of level %d are not set", l);
     for (int s = 0; s < nsteps; ++s)
-      if (int rc = launch_two_level_step(sim)) return rc;
+      if (int rc = launch_hierarchy_step(sim)) return rc;
     return LBX_OK;
   }
   // Reference order per step is f <- S(C(f)).  Over n steps:
@@ -1072,7 +1273,7 @@ int lbx_hydro(lbx_sim* sim, int level, int box, double* rho, double* u) {
 int lbx_hydro_level(lbx_sim* sim, int level, double* rho, double* u) {
   if (int rc = check_sim(sim)) return rc;
   lbx_ctx* ctx = sim->ctx;
-  if (level < 0 || level > 1) return fail(ctx, LBX_E_ARG, "level %d out of range", level);
+  if (level < 0 || level >= LBX_MAX_LEVELS) return fail(ctx, LBX_E_ARG, "level %d out of range", level);
   Level& L = sim->lev[level];
   if (!L.defined) return fail(ctx, LBX_E_STATE, "level %d is not defined", level);
   if (!rho || !u) return fail(ctx, LBX_E_ARG, "null buffer");
@@ -1103,7 +1304,7 @@ int lbx_hydro_level(lbx_sim* sim, int level, double* rho, double* u) {
 int lbx_conservation(lbx_sim* sim, int level, double out[4]) {
   if (int rc = check_sim(sim)) return rc;
   if (!out) return fail(sim->ctx, LBX_E_ARG, "null buffer");
-  if (level < 0 || level > 1 || !sim->lev[level].defined)
+  if (level < 0 || level >= LBX_MAX_LEVELS || !sim->lev[level].defined)
     return fail(sim->ctx, LBX_E_STATE, "level %d is not defined", level);
   lbx_ctx* ctx = sim->ctx;
   LBX_HIP(ctx, hipSetDevice(ctx->device));

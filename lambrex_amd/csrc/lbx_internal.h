@@ -123,34 +123,47 @@ struct PatchDesc {
   long long avg_base;   // offset of this patch's 2x2x2 block averages ((g/2)^3 per population)
 };
 
-// Two-level kernels (lbx_twolevel.hip); precision 0 = FP64 storage, else
-// FP32-shifted (rho0 in Relax or the argument).  The fine kernels run over
-// patches [first, first + npatch).
+// Level-hierarchy kernels (lbx_twolevel.hip); precision 0 = FP64 storage,
+// else FP32-shifted (rho0 in Relax or the argument).  The patch kernels run
+// over patches [first, first + npatch).  A coarse level is either a level
+// array (clat, cpatches / tpatches null) or patched (cpatches / tpatches its
+// descriptors); coarse references in the tables are then -1 - element or
+// -1 - ((patch << 32) | grown index), and sum targets element or
+// (patch << 32) | grown index.
+// mask (null = none): per grown cell of the level (fill_base + i), nonzero =
+// the valid cell lies under a finer level's mask and is zeroed, not collided.
 hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
-                                    const Lattice& clat, const PatchDesc* patches, int first, int npatch,
-                                    int max_cells, const long long* fill, const Relax& r, hipStream_t st);
+                                    const Lattice& clat, const PatchDesc* cpatches, const PatchDesc* patches,
+                                    int first, int npatch, int max_cells, const long long* fill,
+                                    const unsigned char* mask, const Relax& r, hipStream_t st);
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, const PatchDesc* patches,
                                      int first, int npatch, int max_cells, const Relax& r, hipStream_t st);
+// Stream + ZeroInvalidComponents of a middle level (no collide).
+hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
+                                   int npatch, int max_cells, double rho0, hipStream_t st);
 // avg holds 15 FP64 planes of avg_stride block averages; max_blocks = max (g/2)^3.
 hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
                                   const PatchDesc* patches, int first, int npatch, int max_blocks, double rho0,
                                   hipStream_t st);
+// The same block averages of a level's state as it stands (all patches).
+hipError_t launch_patch_avg(int precision, const void* in, double* avg, long long avg_stride,
+                            const PatchDesc* patches, int npatch, int max_blocks, double rho0, hipStream_t st);
 // flags: per coarse cell (x fastest), bits 0-5 = x+1, x-1, y+1, y-1, z+1,
 // z-1 leave the cell's box; bit 7 = masked by the fine level.
 hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, const Lattice& L,
                               const unsigned char* flags, const Relax& r, LaunchShape shape, hipStream_t st);
 hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stride, void* cnext, const Lattice& L,
-                            const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
-                            hipStream_t st);
+                            const PatchDesc* tpatches, const long long* cell, const int* start, const int* contrib,
+                            int ncell, double rho0, hipStream_t st);
 hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, const Lattice& clat,
                                    const PatchDesc* patches, int npatch, int max_cells, double rho0,
                                    hipStream_t st);
 // Valid cells of new patches from old patches or the coarse parent
-// (src per new grown cell: (old patch << 32) | old grown index, or -1 -
-// coarse element of population 0).
+// (src per new grown cell: (old patch << 32) | old grown index, or a coarse
+// reference as above).
 hipError_t launch_fine_remake(int precision, void* fnew, const void* fold, const void* cnow, const Lattice& clat,
-                              const PatchDesc* patches, int npatch, int max_cells, const PatchDesc* old_patches,
-                              const long long* src, hipStream_t st);
+                              const PatchDesc* cpatches, const PatchDesc* patches, int npatch, int max_cells,
+                              const PatchDesc* old_patches, const long long* src, hipStream_t st);
 // op 0: pack valid cells of patch b into FP64 f a[15][n]; 1: unpack; 2: hydro
 // into a (rho), u.
 hipError_t launch_patch_io(int precision, int op, void* f, const PatchDesc* patches, int b, int valid_cells,

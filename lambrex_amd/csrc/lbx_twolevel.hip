@@ -26,6 +26,14 @@
 //                         + ghosts) into coarse next, contributions to one
 //                         coarse cell in a fixed order
 //
+// Deeper hierarchies (RohdeCycle's recursion, src/AmrSim.cpp:453-455) reuse
+// these kernels on patched "middle" levels, which have both a parent and a
+// child: the fill-collide takes its coarse values from patches (PC) and
+// zeroes the cells under the child's mask (MASK); k_fine_stream_zero is the
+// middle level's Stream + ZeroInvalidComponents; k_fine_stream_avg without
+// its stream averages a middle level's final state for its parent; and
+// k_sum_apply adds into a patched level's valid cells (PT).
+//
 // Storage type S is double (FP64, bit-exact with the oracle) or float (the
 // shifted deviation d = f - w rho0 of the FP32 path).  Either way the block
 // sums and the coarse additions of SumFromFine are done in FP64 on
@@ -94,11 +102,15 @@ __device__ __forceinline__ bool in_valid(const PatchDesc& P, int x, int y, int z
 // Every lane first picks where its 15 populations come from (its own valid
 // cell, the fine valid cell holding its periodic image, or the coarse
 // parent) so the loads of valid and ghost lanes issue together; valid cells
-// are then collided.
-template <typename S>
+// are then collided.  Coarse sources are -1 - (element of population 0) in
+// a level array, or with PC -1 - ((coarse patch << 32) | grown index).  With
+// MASK, a valid cell whose mask byte is set is zeroed instead of collided
+// (CoarseCollide on a level that has a finer one, src/AmrSim.cpp:499-500).
+template <typename S, bool PC, bool MASK>
 __global__ __launch_bounds__(256) void k_fine_fill_collide(
     const S* __restrict__ fnow, S* __restrict__ fnext, const S* __restrict__ cnow, Lattice clat,
-    const PatchDesc* __restrict__ patches, int first, const long long* __restrict__ fill, Relax r) {
+    const PatchDesc* __restrict__ cpatches, const PatchDesc* __restrict__ patches, int first,
+    const long long* __restrict__ fill, const unsigned char* __restrict__ mask, Relax r) {
   const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
@@ -116,6 +128,11 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
       const int s = int(d >> 32);
       src = fnow + patches[s].base + (d & 0xffffffffll);
       stride = patches[s].qs;
+    } else if constexpr (PC) {
+      const long long e = -1 - d;
+      const PatchDesc& CP = cpatches[e >> 32];
+      src = cnow + CP.base + (e & 0xffffffffll);
+      stride = CP.qs;
     } else {
       src = cnow + (-1 - d);  // coarse element of population 0
       stride = clat.ps;
@@ -124,10 +141,16 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
   S f[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) f[q] = src[q * stride];
-  if (valid)
-    collide_s(f, r);
-  else if (patch_outer(P, gx, gy, gz))
+  if (valid) {
+    if (MASK && mask[P.fill_base + i]) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) f[q] = zero_pop<S>(q, r.rho0);
+    } else {
+      collide_s(f, r);
+    }
+  } else if (patch_outer(P, gx, gy, gz)) {
     f[0] = zero_pop<S>(0, r.rho0);
+  }
 #pragma unroll
   for (int q = 0; q < NV; ++q) fnext[P.base + q * P.qs + i] = f[q];
 }
@@ -165,17 +188,46 @@ __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict
   for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
 }
 
-// Second fine substep's stream + ZeroInvalidComponents(1) (a ghost keeps
-// population m only if pos - 2 c_m lies in the valid box), fused with the
+// ZeroInvalidComponents on a ghost cell at absolute (x, y, z): population m
+// survives only if pos - 2 c_m lies in the patch's valid box.
+template <typename S>
+__device__ __forceinline__ void zero_invalid(const PatchDesc& P, int gx, int gy, int gz, double rho0, S* f) {
+  const int x = P.lo[0] - 2 + gx, y = P.lo[1] - 2 + gy, z = P.lo[2] - 2 + gz;
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    if (!in_valid(P, x - 2 * kC[q][0], y - 2 * kC[q][1], z - 2 * kC[q][2])) f[q] = zero_pop<S>(q, rho0);
+}
+
+// A middle level's Stream + ZeroInvalidComponents (src/AmrSim.cpp:459-463):
+// pull over valid + 1 ghost layer, outer layer zero, then the ghost shell
+// keeps only populations that came from valid cells.
+template <typename S>
+__global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ in, S* __restrict__ out,
+                                                          const PatchDesc* __restrict__ patches, int first,
+                                                          double rho0) {
+  const PatchDesc P = patches[first + blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.cells) return;
+  int gx, gy, gz;
+  patch_coords(P, i, gx, gy, gz);
+  S f[NV];
+  if (!fine_pull(P, in, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz)) zero_invalid(P, gx, gy, gz, rho0, f);
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
+}
+
+// Second fine substep's stream + ZeroInvalidComponents(1), fused with the
 // first half of SumFromFine: the 1/8-weighted sum of each 2x2x2 block of the
 // grown patch (= one coarse cell of the coarsened box grown by 1), children
 // added in (k, j, i) order in FP64, to avg[q * avg_stride + avg_base + block].
 // A workgroup takes 32 consecutive blocks; thread t computes child t / 32 of
 // block t % 32, so each wavefront reads whole x-pairs of its blocks.  The
-// children meet in LDS for the ordered sums.
+// children meet in LDS for the ordered sums.  Without STREAM the block sums
+// are taken of `in` as it stands (a middle level's state after its steps)
+// and nothing is stored.
 constexpr int kAvgBlocks = 32;
 
-template <typename S>
+template <typename S, bool STREAM>
 __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ in, S* __restrict__ out,
                                                          double* __restrict__ avg, long long avg_stride,
                                                          const PatchDesc* __restrict__ patches, int first,
@@ -194,15 +246,16 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     const int gx = 2 * cx + di, gy = 2 * cy + dj, gz = 2 * cz + dk;
     const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
     S f[NV];
-    if (!fine_pull(P, in, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz)) {
-      const int x = P.lo[0] - 2 + gx, y = P.lo[1] - 2 + gy, z = P.lo[2] - 2 + gz;
+    if constexpr (STREAM) {
+      if (!fine_pull(P, in, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz))
+        zero_invalid(P, gx, gy, gz, rho0, f);
+    } else {
 #pragma unroll
-      for (int q = 0; q < NV; ++q)
-        if (!in_valid(P, x - 2 * kC[q][0], y - 2 * kC[q][1], z - 2 * kC[q][2])) f[q] = zero_pop<S>(q, rho0);
+      for (int q = 0; q < NV; ++q) f[q] = in[P.base + q * P.qs + i];
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      out[P.base + q * P.qs + i] = f[q];
+      if constexpr (STREAM) out[P.base + q * P.qs + i] = f[q];
       part[q][child][lb] = to_f64(f[q], q, rho0);
     }
   }
@@ -260,25 +313,36 @@ __global__ __launch_bounds__(512) void k_coarse_push(const S* __restrict__ cnow,
 // Second half of SumFromFine: thread per touched coarse cell; its
 // contributions [start[u], start[u+1]) (indices into avg, in the order the
 // reference's loops add them) are added in FP64 to the streamed coarse value.
-template <typename S>
+// A coarse cell is the element of population 0 in a level array, or with PT
+// (coarse patch << 32) | grown index of a valid cell of a patched level.
+template <typename S, bool PT>
 __global__ __launch_bounds__(256) void k_sum_apply(const double* __restrict__ avg, long long avg_stride,
                                                    S* __restrict__ cnext, Lattice L,
+                                                   const PatchDesc* __restrict__ tpatches,
                                                    const long long* __restrict__ cell,
                                                    const int* __restrict__ start,
                                                    const int* __restrict__ contrib, int ncell, double rho0) {
   const int u = blockIdx.x * 256 + threadIdx.x;
   if (u >= ncell) return;
-  const long long c = cell[u];  // coarse element of population 0
+  long long c, stride;
+  if constexpr (PT) {
+    const PatchDesc& T = tpatches[cell[u] >> 32];
+    c = T.base + (cell[u] & 0xffffffffll);
+    stride = T.qs;
+  } else {
+    c = cell[u];  // coarse element of population 0
+    stride = L.ps;
+  }
   double acc[NV];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = to_f64(cnext[c + q * L.ps], q, rho0);
+  for (int q = 0; q < NV; ++q) acc[q] = to_f64(cnext[c + q * stride], q, rho0);
   for (int j = start[u]; j < start[u + 1]; ++j) {
     const long long a = contrib[j];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] += avg[q * avg_stride + a];
   }
 #pragma unroll
-  for (int q = 0; q < NV; ++q) cnext[c + q * L.ps] = from_f64<S>(acc[q], q, rho0);
+  for (int q = 0; q < NV; ++q) cnext[c + q * stride] = from_f64<S>(acc[q], q, rho0);
 }
 
 // DistFnFillFromCoarse (src/AmrSim.cpp:393-411): valid fine cells take the
@@ -307,10 +371,13 @@ __global__ __launch_bounds__(256) void k_fine_from_coarse(S* __restrict__ fnow, 
 // RemakeLevel's fill (src/AmrSim.cpp:715-744 via DistFnFillPatch): each
 // valid cell of the new patches copies its populations from an old fine
 // patch (src >= 0: (old patch << 32) | old grown index) or from its coarse
-// parent (src < 0: -1 - coarse element of population 0).
-template <typename S>
+// parent (src < 0: -1 - coarse element of population 0, or with PC
+// -1 - ((coarse patch << 32) | grown index)).  With every source a parent it
+// is DistFnFillFromCoarse on a level whose parent is patched.
+template <typename S, bool PC>
 __global__ __launch_bounds__(256) void k_fine_remake(S* __restrict__ fnew, const S* __restrict__ fold,
                                                      const S* __restrict__ cnow, Lattice clat,
+                                                     const PatchDesc* __restrict__ cpatches,
                                                      const PatchDesc* __restrict__ patches,
                                                      const PatchDesc* __restrict__ old_patches,
                                                      const long long* __restrict__ src) {
@@ -327,6 +394,11 @@ __global__ __launch_bounds__(256) void k_fine_remake(S* __restrict__ fnew, const
     const PatchDesc& O = old_patches[d >> 32];
     from = fold + O.base + (d & 0xffffffffll);
     stride = O.qs;
+  } else if constexpr (PC) {
+    const long long e = -1 - d;
+    const PatchDesc& CP = cpatches[e >> 32];
+    from = cnow + CP.base + (e & 0xffffffffll);
+    stride = CP.qs;
   } else {
     from = cnow + (-1 - d);
     stride = clat.ps;
@@ -381,16 +453,56 @@ hipError_t patch_io(int op, S* f, const PatchDesc* patches, int b, int valid_cel
 
 }  // namespace
 
+template <typename S>
+hipError_t fill_collide(const S* fnow, S* fnext, const S* cnow, const Lattice& clat, const PatchDesc* cpatches,
+                        const PatchDesc* patches, int first, const long long* fill, const unsigned char* mask,
+                        const Relax& r, dim3 grid, hipStream_t st) {
+  if (cpatches && mask)
+    k_fine_fill_collide<S, true, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first, fill,
+                                                             mask, r);
+  else if (cpatches)
+    k_fine_fill_collide<S, true, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first,
+                                                              fill, mask, r);
+  else if (mask)
+    k_fine_fill_collide<S, false, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first,
+                                                              fill, mask, r);
+  else
+    k_fine_fill_collide<S, false, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first,
+                                                               fill, mask, r);
+  return hipGetLastError();
+}
+
 hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
-                                    const Lattice& clat, const PatchDesc* patches, int first, int npatch,
-                                    int max_cells, const long long* fill, const Relax& r, hipStream_t st) {
+                                    const Lattice& clat, const PatchDesc* cpatches, const PatchDesc* patches,
+                                    int first, int npatch, int max_cells, const long long* fill,
+                                    const unsigned char* mask, const Relax& r, hipStream_t st) {
   const dim3 grid((max_cells + 255) / 256, npatch);
   if (precision == 0)
-    k_fine_fill_collide<double><<<grid, 256, 0, st>>>((const double*)fnow, (double*)fnext, (const double*)cnow,
-                                                      clat, patches, first, fill, r);
+    return fill_collide((const double*)fnow, (double*)fnext, (const double*)cnow, clat, cpatches, patches, first,
+                        fill, mask, r, grid, st);
+  return fill_collide((const float*)fnow, (float*)fnext, (const float*)cnow, clat, cpatches, patches, first, fill,
+                      mask, r, grid, st);
+}
+
+hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
+                                   int npatch, int max_cells, double rho0, hipStream_t st) {
+  const dim3 grid((max_cells + 255) / 256, npatch);
+  if (precision == 0)
+    k_fine_stream_zero<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, rho0);
   else
-    k_fine_fill_collide<float><<<grid, 256, 0, st>>>((const float*)fnow, (float*)fnext, (const float*)cnow, clat,
-                                                     patches, first, fill, r);
+    k_fine_stream_zero<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_avg(int precision, const void* in, double* avg, long long avg_stride,
+                            const PatchDesc* patches, int npatch, int max_blocks, double rho0, hipStream_t st) {
+  const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
+  if (precision == 0)
+    k_fine_stream_avg<double, false><<<grid, 256, 0, st>>>((const double*)in, nullptr, avg, avg_stride, patches, 0,
+                                                           rho0);
+  else
+    k_fine_stream_avg<float, false><<<grid, 256, 0, st>>>((const float*)in, nullptr, avg, avg_stride, patches, 0,
+                                                          rho0);
   return hipGetLastError();
 }
 
@@ -409,11 +521,11 @@ hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, doub
                                   hipStream_t st) {
   const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
   if (precision == 0)
-    k_fine_stream_avg<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride, patches,
-                                                    first, rho0);
+    k_fine_stream_avg<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride,
+                                                          patches, first, rho0);
   else
-    k_fine_stream_avg<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride, patches,
-                                                   first, rho0);
+    k_fine_stream_avg<float, true><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride, patches,
+                                                         first, rho0);
   return hipGetLastError();
 }
 
@@ -429,16 +541,22 @@ hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, cons
 }
 
 hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stride, void* cnext, const Lattice& L,
-                            const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
-                            hipStream_t st) {
+                            const PatchDesc* tpatches, const long long* cell, const int* start, const int* contrib,
+                            int ncell, double rho0, hipStream_t st) {
   if (ncell <= 0) return hipSuccess;
   const int blocks = (ncell + 255) / 256;
-  if (precision == 0)
-    k_sum_apply<double><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, cell, start, contrib, ncell,
-                                                rho0);
+  if (precision == 0 && tpatches)
+    k_sum_apply<double, true><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, tpatches, cell, start,
+                                                      contrib, ncell, rho0);
+  else if (precision == 0)
+    k_sum_apply<double, false><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, tpatches, cell, start,
+                                                       contrib, ncell, rho0);
+  else if (tpatches)
+    k_sum_apply<float, true><<<blocks, 256, 0, st>>>(avg, avg_stride, (float*)cnext, L, tpatches, cell, start,
+                                                     contrib, ncell, rho0);
   else
-    k_sum_apply<float><<<blocks, 256, 0, st>>>(avg, avg_stride, (float*)cnext, L, cell, start, contrib, ncell,
-                                               rho0);
+    k_sum_apply<float, false><<<blocks, 256, 0, st>>>(avg, avg_stride, (float*)cnext, L, tpatches, cell, start,
+                                                      contrib, ncell, rho0);
   return hipGetLastError();
 }
 
@@ -454,15 +572,21 @@ hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, 
 }
 
 hipError_t launch_fine_remake(int precision, void* fnew, const void* fold, const void* cnow, const Lattice& clat,
-                              const PatchDesc* patches, int npatch, int max_cells, const PatchDesc* old_patches,
-                              const long long* src, hipStream_t st) {
+                              const PatchDesc* cpatches, const PatchDesc* patches, int npatch, int max_cells,
+                              const PatchDesc* old_patches, const long long* src, hipStream_t st) {
   const dim3 grid((max_cells + 255) / 256, npatch);
-  if (precision == 0)
-    k_fine_remake<double><<<grid, 256, 0, st>>>((double*)fnew, (const double*)fold, (const double*)cnow, clat,
-                                                patches, old_patches, src);
+  if (precision == 0 && cpatches)
+    k_fine_remake<double, true><<<grid, 256, 0, st>>>((double*)fnew, (const double*)fold, (const double*)cnow, clat,
+                                                      cpatches, patches, old_patches, src);
+  else if (precision == 0)
+    k_fine_remake<double, false><<<grid, 256, 0, st>>>((double*)fnew, (const double*)fold, (const double*)cnow, clat,
+                                                       cpatches, patches, old_patches, src);
+  else if (cpatches)
+    k_fine_remake<float, true><<<grid, 256, 0, st>>>((float*)fnew, (const float*)fold, (const float*)cnow, clat,
+                                                     cpatches, patches, old_patches, src);
   else
-    k_fine_remake<float><<<grid, 256, 0, st>>>((float*)fnew, (const float*)fold, (const float*)cnow, clat, patches,
-                                               old_patches, src);
+    k_fine_remake<float, false><<<grid, 256, 0, st>>>((float*)fnew, (const float*)fold, (const float*)cnow, clat,
+                                                      cpatches, patches, old_patches, src);
   return hipGetLastError();
 }
 

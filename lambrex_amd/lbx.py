@@ -20,6 +20,7 @@ LBX_E_NOMEM = -4
 LBX_E_UNSUPPORTED = -5
 LBX_FP64 = 0
 LBX_FP32_SHIFTED = 1
+LBX_MAX_LEVELS = 6
 
 # Every extern "C" entry point of include/lbx.h with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -42,6 +43,7 @@ SIGNATURES = {
                                         ctypes.c_int32, _i32p, _i32p]),
     "lbx_set_relax": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "lbx_set_fine_mask": (ctypes.c_int, [_vp, ctypes.c_int, _i32p]),
+    "lbx_set_level_mask": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
     "lbx_fill_from_coarse": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_init_equilibrium": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "lbx_upload_f": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
@@ -235,8 +237,8 @@ class Sim:
         self.boxes[level] = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
 
     def remake_level(self, level, boxes):
-        """New boxes for level 1, keeping its data (old fine values where an
-        old box covered the cell, else the coarse parent's)."""
+        """New boxes for the finest level, keeping its data (old values where
+        an old box covered the cell, else the parent's)."""
         lohi = _lohi(boxes)
         self.ctx.check(self._lib.lbx_level_remake(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))
         self.boxes[level] = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
@@ -257,6 +259,19 @@ class Sim:
         boxes are masked in the two-level coarse collide."""
         lohi = _lohi(boxes) if boxes else np.zeros(6, dtype=np.int32)
         self.ctx.check(self._lib.lbx_set_fine_mask(self.h, len(boxes), _ptr(lohi, ctypes.c_int32)))
+
+    def set_level_mask(self, level, boxes):
+        """Cells of `level` under coarsen(box, 2) of these boxes (index space
+        of level + 1) are masked in that level's collide."""
+        lohi = _lohi(boxes) if boxes else np.zeros(6, dtype=np.int32)
+        self.ctx.check(self._lib.lbx_set_level_mask(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))
+
+    def clear_level(self, level):
+        """Release `level` and every finer level."""
+        self.ctx.check(self._lib.lbx_level_clear(self.h, level))
+        for l in list(self.boxes):
+            if l >= level:
+                del self.boxes[l]
 
     def fill_from_coarse(self, level=1):
         self.ctx.check(self._lib.lbx_fill_from_coarse(self.h, level))

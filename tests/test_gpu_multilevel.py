@@ -1,0 +1,270 @@
+"""Hierarchies deeper than two levels on the GPU against the oracle, bitwise.
+
+RohdeCycle recurses over levels (src/AmrSim.cpp:430-469): a level between
+level 0 and the finest one is filled from its parent and its own other boxes
+with comp 0 zeroed on the outer ghost layer, collided under its child's mask,
+stepped twice per parent step, streamed, given its child's average-down and
+ZeroInvalidComponents.  The oracle's recursion is pinned in
+test_oracle_multilevel.py; these tests hold the device path to it.
+"""
+import numpy as np
+import pytest
+
+from lambrex_amd import lbx, workloads
+from oracle import oracle as orc
+from test_gpu_twolevel import assert_bitwise, c_order, fine_boxes_in
+
+pytestmark = pytest.mark.gpu
+
+
+def taus(tau, nlev):
+    out = [tau]
+    for _ in range(1, nlev):
+        out.append(2 * (out[-1] - 0.5) + 0.5)  # ComputeDt, src/AmrSim.cpp:313-314
+    return out
+
+
+def lists(boxes):
+    return [(list(a), list(b)) for a, b in boxes]
+
+
+def masks_for(kind, levels):
+    """Mask boxes of each parent level l (index space of level l+1): the
+    parent's own boxes, as MakeFineMask passes them (src/AmrSim.cpp:419-428),
+    or the child's boxes."""
+    if kind == "reference":
+        return [list(levels[l]) for l in range(len(levels) - 1)]
+    return [list(levels[l + 1]) for l in range(len(levels) - 1)]
+
+
+def make_gpu(ctx, dims, levels, masks, tau, rho, u, precision=lbx.LBX_FP64):
+    sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
+    t = taus(tau, len(levels))
+    sim.define_level(0, levels[0])
+    sim.set_tau(0, t[0])
+    sim.init_from_fields(rho, u)
+    for l in range(1, len(levels)):
+        sim.define_level(l, levels[l])
+        sim.set_tau(l, t[l])
+        sim.fill_from_coarse(l)
+        sim.set_level_mask(l - 1, masks[l - 1])
+    return sim
+
+
+def run_oracle(dims, levels, masks, tau, rho, u, steps):
+    o = orc.Sim(*dims, tau, boxes=lists(levels[0]))
+    o.init(*c_order(rho, u))
+    for l in range(1, len(levels)):
+        o.add_fine(lists(levels[l]), lists(masks[l - 1]))
+    o.iterate(steps)
+    out = [o.get_f(l) for l in range(len(levels))]
+    o.close()
+    return out
+
+
+def gpu_levels(sim, nlev):
+    return [sim.get_f(0)] + [sim.get_f_bbox(l)[0] for l in range(1, nlev)]
+
+
+def three_levels():
+    """Coarse 16x8x8 in boxes of 8^3; level 1 over coarse x in [4, 12), all y
+    and z (boxes of 8^3); level 2 over level-1 x in [10, 22), y in [2, 14),
+    all z: it wraps onto itself in z and its average-down wraps in level 1."""
+    dims = (16, 8, 8)
+    l0 = workloads.chop(dims, 8)
+    l1 = fine_boxes_in((8, 0, 0), (23, 15, 15), 8)
+    l2 = fine_boxes_in((20, 4, 0), (43, 27, 31), 8)
+    return dims, [l0, l1, l2]
+
+
+def four_levels():
+    """Coarse 8^3; each finer level a centred cube inside its parent, all cut
+    at 4 (8, 27 and 64 boxes)."""
+    dims = (8, 8, 8)
+    l0 = workloads.chop(dims, 4)
+    l1 = fine_boxes_in((4, 4, 4), (11, 11, 11), 4)
+    l2 = fine_boxes_in((10, 10, 10), (21, 21, 21), 4)
+    l3 = fine_boxes_in((24, 24, 24), (39, 39, 39), 4)
+    return dims, [l0, l1, l2, l3]
+
+
+@pytest.mark.parametrize("mask", ["reference", "fine"])
+@pytest.mark.parametrize("steps", [1, 3])
+def test_three_levels_bitwise(ctx, mask, steps):
+    dims, levels = three_levels()
+    masks = masks_for(mask, levels)
+    rho, u = workloads.random_state(dims, seed=71, rho_sigma=1e-2, u_amp=0.02)
+    want = run_oracle(dims, levels, masks, 0.8, rho, u, steps)
+    sim = make_gpu(ctx, dims, levels, masks, 0.8, rho, u)
+    try:
+        sim.iterate(steps)
+        got = gpu_levels(sim, 3)
+    finally:
+        sim.close()
+    for l in range(3):
+        assert_bitwise(want[l], got[l], f"level {l} f")
+
+
+@pytest.mark.parametrize("mask", ["reference", "fine"])
+def test_four_levels_bitwise(ctx, mask):
+    dims, levels = four_levels()
+    masks = masks_for(mask, levels)
+    rho, u = workloads.random_state(dims, seed=73, rho_sigma=1e-2, u_amp=0.02)
+    want = run_oracle(dims, levels, masks, 0.7, rho, u, 2)
+    sim = make_gpu(ctx, dims, levels, masks, 0.7, rho, u)
+    try:
+        sim.iterate(2)
+        got = gpu_levels(sim, 4)
+    finally:
+        sim.close()
+    for l in range(4):
+        assert_bitwise(want[l], got[l], f"level {l} f")
+
+
+def test_fill_from_patched_parent_is_piecewise_constant(ctx):
+    """DistFnFillFromCoarse of level 2 copies each cell's level-1 parent."""
+    dims, levels = three_levels()
+    rho, u = workloads.random_state(dims, seed=5, rho_sigma=1e-2, u_amp=0.02)
+    sim = make_gpu(ctx, dims, levels[:2], masks_for("fine", levels[:2]), 0.8, rho, u)
+    try:
+        sim.iterate(2)  # give level 1 structure of its own
+        sim.define_level(2, levels[2])
+        sim.fill_from_coarse(2)
+        f1, lo1 = sim.get_f_bbox(1)
+        f2, lo2 = sim.get_f_bbox(2)
+    finally:
+        sim.close()
+    n1 = 2 * np.array(dims)
+    for (blo, bhi) in levels[2]:
+        for z in range(blo[2], bhi[2] + 1):
+            for y in range(blo[1], bhi[1] + 1):
+                for x in range(blo[0], bhi[0] + 1):
+                    px, py, pz = (x // 2) % n1[0], (y // 2) % n1[1], (z // 2) % n1[2]
+                    want = f1[:, pz - lo1[2], py - lo1[1], px - lo1[0]]
+                    got = f2[:, z - lo2[2], y - lo2[1], x - lo2[0]]
+                    assert (got.view(np.uint64) == want.view(np.uint64)).all(), (x, y, z)
+
+
+def test_graph_replays_match_eager_steps_three_levels(ctx):
+    """The captured three-level step replayed with eager downloads between
+    replays equals the eagerly launched step (profiling forces the eager
+    path) at every step."""
+    dims, levels = three_levels()
+    masks = masks_for("fine", levels)
+    rho, u = workloads.random_state(dims, seed=43, rho_sigma=1e-2, u_amp=0.02)
+    eager = make_gpu(ctx, dims, levels, masks, 0.8, rho, u)
+    graph = make_gpu(ctx, dims, levels, masks, 0.8, rho, u)
+    try:
+        for step in range(4):
+            ctx.profile_enable(True)
+            eager.iterate(1)
+            ctx.profile_enable(False)
+            graph.iterate(1)
+            for l, (a, b) in enumerate(zip(gpu_levels(eager, 3), gpu_levels(graph, 3))):
+                assert_bitwise(a, b, f"level {l} after step {step + 1}")
+    finally:
+        eager.close()
+        graph.close()
+
+
+def test_fp32_three_levels_tracks_fp64(ctx):
+    """FP32-shifted storage on every level against the FP64 path: density
+    within 1e-6 relative on every level after 12 level-0 steps (48 steps of
+    level 2)."""
+    dims, levels = three_levels()
+    masks = masks_for("fine", levels)
+    rho, u = workloads.random_state(dims, seed=57, rho_sigma=1e-3, u_amp=0.01)
+    out = {}
+    for prec in (lbx.LBX_FP64, lbx.LBX_FP32_SHIFTED):
+        sim = make_gpu(ctx, dims, levels, masks, 0.8, rho, u, precision=prec)
+        try:
+            sim.iterate(12)
+            out[prec] = [sim.get_hydro(0)[0]] + [sim.get_hydro_bbox(l)[0] for l in (1, 2)]
+        finally:
+            sim.close()
+    errs = []
+    for a, b in zip(out[lbx.LBX_FP64], out[lbx.LBX_FP32_SHIFTED]):
+        valid = a != 0  # cells of the bounding box that no box holds are zero
+        assert np.isfinite(b[valid]).all()
+        errs.append(np.max(np.abs(b[valid] - a[valid]) / a[valid]))
+    print(f"[fp32 three levels] max rel rho error per level {errs}")
+    assert max(errs) <= 1e-6
+
+
+def test_remake_finest_of_three_then_matches_oracle(ctx):
+    """Remaking level 2 (whose parent is patched) keeps its data where the
+    old boxes covered it, takes level 1's otherwise, and steps on like the
+    oracle started from the same populations."""
+    dims, levels = three_levels()
+    masks = masks_for("fine", levels)
+    rho, u = workloads.random_state(dims, seed=67, rho_sigma=1e-2, u_amp=0.02)
+    new2 = fine_boxes_in((24, 8, 4), (39, 23, 19), 8)
+    sim = make_gpu(ctx, dims, levels, masks, 0.8, rho, u)
+    try:
+        sim.iterate(2)
+        f1, lo1 = sim.get_f_bbox(1)
+        f2old, lo2old = sim.get_f_bbox(2)
+        sim.remake_level(2, new2)
+        sim.set_level_mask(1, new2)
+        f2, lo2 = sim.get_f_bbox(2)
+        for (blo, bhi) in new2:
+            for z in range(blo[2], bhi[2] + 1):
+                for y in range(blo[1], bhi[1] + 1):
+                    for x in range(blo[0], bhi[0] + 1):
+                        got = f2[:, z - lo2[2], y - lo2[1], x - lo2[0]]
+                        if any(all(a[d] <= p <= b[d] for d, p in enumerate((x, y, z))) for a, b in levels[2]):
+                            want = f2old[:, z - lo2old[2], y - lo2old[1], x - lo2old[0]]
+                        else:
+                            want = f1[:, z // 2 - lo1[2], y // 2 - lo1[1], x // 2 - lo1[0]]
+                        assert (got.view(np.uint64) == want.view(np.uint64)).all(), (x, y, z)
+        f0 = sim.get_f(0)
+        sim.iterate(2)
+        got = gpu_levels(sim, 3)
+    finally:
+        sim.close()
+    o = orc.Sim(*dims, 0.8, boxes=lists(levels[0]))
+    o.init(*c_order(rho, u))
+    o.set_f(f0, 0)
+    o.add_fine(lists(levels[1]), lists(masks[0]))
+    o.set_f(f1, 1)
+    o.add_fine(lists(new2), lists(new2))
+    o.set_f(f2, 2)
+    o.iterate(2)
+    for l in range(3):
+        assert_bitwise(o.get_f(l), got[l], f"level {l} after remake + 2 steps")
+    o.close()
+
+
+def test_hierarchy_errors(ctx):
+    dims, levels = three_levels()
+    rho, u = workloads.random_state(dims, seed=1, rho_sigma=1e-3, u_amp=0.01)
+    sim = make_gpu(ctx, dims, levels[:1], [], 0.8, rho, u)
+    try:
+        with pytest.raises(lbx.LbxError) as e:
+            sim.define_level(2, levels[2])  # level 1 missing
+        assert e.value.code == lbx.LBX_E_STATE
+        sim.define_level(1, levels[1])
+        sim.set_tau(1, 1.1)
+        sim.fill_from_coarse(1)
+        with pytest.raises(lbx.LbxError) as e:
+            # level-2 x 12..19 needs level-1 parents from x 5 (ghosts), outside 8..23
+            sim.define_level(2, fine_boxes_in((12, 4, 0), (19, 11, 7), 8))
+        assert e.value.code == lbx.LBX_E_ARG
+        with pytest.raises(lbx.LbxError) as e:
+            sim.define_level(lbx.LBX_MAX_LEVELS, levels[1])
+        assert e.value.code == lbx.LBX_E_UNSUPPORTED
+        sim.define_level(2, levels[2])
+        sim.fill_from_coarse(2)
+        with pytest.raises(lbx.LbxError) as e:
+            sim.iterate(1)  # level 2's relaxation rates not set
+        assert e.value.code == lbx.LBX_E_STATE
+        with pytest.raises(lbx.LbxError) as e:
+            sim.remake_level(1, levels[1])  # level 1 has a finer level
+        assert e.value.code == lbx.LBX_E_STATE
+        sim.clear_level(1)  # releases level 2 too
+        with pytest.raises(lbx.LbxError) as e:
+            sim.conservation(2)
+        assert e.value.code == lbx.LBX_E_STATE
+        sim.iterate(1)  # single level again
+    finally:
+        sim.close()
