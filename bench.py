@@ -227,6 +227,77 @@ def two_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
             "graph": "one coarse step captured as a HIP graph"}, fields
 
 
+def three_level_geometry():
+    """Three-level configuration: the two-level one with a level 2 nested
+    inside level 1 as a slab too: level-1 x in [120, 136) and all y, z
+    (level 2 32x512x512 in boxes of 32^3, tau 1.7), each level's cells under
+    its child masked.  The interfaces are planes, as in the two-level case:
+    the reference's cycle adds the average of each fine box grown by one
+    coarse cell into unmasked coarse cells (SumFromFine, src/AmrSim.cpp:
+    592-602), which a level-2 box enclosed on all six sides turns into a
+    divergence within about 100 level-0 steps (in FP64 too).  A step is one
+    level-0 step: 128^3 + 2 x 64x256x256 + 4 x 32x512x512 lattice updates
+    (level l takes 2^l steps)."""
+    dims, cboxes, fboxes, mask, rho, u, _ = two_level_geometry()
+    ext = (32, 512, 512)
+    l2 = [((240 + a[0], a[1], a[2]), (240 + b[0], b[1], b[2])) for a, b in workloads.chop(ext, 32)]
+    levels = [cboxes, fboxes, l2]
+    masks = [fboxes, l2]
+    lu_per_step = 128 ** 3 + 2 * 64 * 256 * 256 + 4 * 32 * 512 * 512
+    return dims, levels, masks, rho, u, lu_per_step
+
+
+def three_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
+    """The three-level configuration on the GPU (one level-0 step = one
+    replay of the captured HIP graph)."""
+    dims, levels, masks, rho, u, lu_per_step = three_level_geometry()
+    sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
+    try:
+        sim.define_level(0, levels[0])
+        tau = 0.8
+        sim.set_tau(0, tau)
+        sim.init_from_fields(rho, u)
+        for l in (1, 2):
+            tau = 2 * (tau - 0.5) + 0.5
+            sim.define_level(l, levels[l])
+            sim.set_tau(l, tau)
+            sim.fill_from_coarse(l)
+            sim.set_level_mask(l - 1, masks[l - 1])
+        sim.iterate(warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        sim.iterate(steps)
+        ctx.sync()
+        t1 = time.perf_counter()
+        rhos = [sim.get_hydro(0)[0]] + [sim.get_hydro_bbox(l)[0] for l in (1, 2)]
+    finally:
+        sim.close()
+    lu = lu_per_step * steps
+    name = "FP64" if precision == lbx.LBX_FP64 else "FP32-shifted"
+    return {"workload": "coarse 128^3 (64 boxes of 32^3) + level 1 64x256x256 (128 boxes of 32^3) + level 2 "
+                        f"32x512x512 (256 boxes of 32^3), ratio 2, {name}",
+            "coarse_steps": steps, "warmup_steps": warmup, "mlups": lu / (t1 - t0) / 1e6,
+            "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
+            "graph": "one level-0 step captured as a HIP graph",
+            "final_rho_range_per_level": [[float(np.min(r)), float(np.max(r))] for r in rhos]}, rhos
+
+
+def cpu_three_level(steps):
+    """The oracle's recursive Rohde cycle on the three-level configuration."""
+    from oracle import oracle as orc
+    dims, levels, masks, rho, u, lu_per_step = three_level_geometry()
+    s = orc.Sim(*dims, 0.8, boxes=[(list(a), list(b)) for a, b in levels[0]])
+    s.init(np.ascontiguousarray(rho.transpose(2, 1, 0)).reshape(-1),
+           np.ascontiguousarray(u.transpose(3, 2, 1, 0)).reshape(-1))
+    for l in (1, 2):
+        s.add_fine([(list(a), list(b)) for a, b in levels[l]], [(list(a), list(b)) for a, b in masks[l - 1]])
+    t0 = time.perf_counter()
+    s.iterate(steps)
+    t1 = time.perf_counter()
+    s.close()
+    return lu_per_step * steps / (t1 - t0) / 1e6, t1 - t0
+
+
 def precision_errors(ref, test):
     """Max relative density error and max |du| / max |u|, over both levels."""
     r0, u0, r1, u1 = ref
@@ -285,6 +356,10 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--two-level-steps", type=int, default=500,
                     help="coarse steps of the two-level measurement (0 skips it)")
+    ap.add_argument("--three-level-steps", type=int, default=200,
+                    help="level-0 steps of the three-level measurement (0 skips it)")
+    ap.add_argument("--cpu-three-level-steps", type=int, default=1,
+                    help="level-0 steps of the three-level CPU baseline (0 skips it)")
     args = ap.parse_args()
 
     if args.copy_only:
@@ -343,6 +418,21 @@ def main():
                 log(f"[bench] two-level FP32-shifted: {tl32['mlups']:.0f} MLUPS, max rel rho err {e_rho:.2e}, "
                     f"max |du|/max|u| {e_u:.2e}")
             del f64_fields
+        threelev = None
+        if args.three_level_steps:
+            log(f"[bench] three-level: {args.three_level_steps} level-0 steps")
+            threelev, r64s = three_level(ctx, args.three_level_steps, 5)
+            log(f"[bench] three-level: {threelev['mlups']:.0f} MLUPS, "
+                f"{threelev['ms_per_coarse_step']:.3f} ms per level-0 step, "
+                f"rho ranges {threelev['final_rho_range_per_level']}")
+            if not args.no_fp32:
+                t32, r32s = three_level(ctx, args.three_level_steps, 5, lbx.LBX_FP32_SHIFTED)
+                e_rho = max(float(np.max(np.abs(b - a) / a)) for a, b in zip(r64s, r32s))
+                threelev["fp32_shifted"] = {"mlups": t32["mlups"], "ms_per_coarse_step": t32["ms_per_coarse_step"],
+                                            "speedup_vs_fp64": round(t32["mlups"] / threelev["mlups"], 3),
+                                            "max_rel_rho_err_vs_fp64": e_rho}
+                log(f"[bench] three-level FP32-shifted: {t32['mlups']:.0f} MLUPS, max rel rho err {e_rho:.2e}")
+            del r64s
         serving = None
         if args.serve_requests:
             trace = serving_trace(args.serve_requests, seed=7, rate_hz=200.0)
@@ -430,6 +520,8 @@ def main():
         }
     if twolev:
         out["two_level"] = twolev
+    if threelev:
+        out["three_level"] = threelev
     if serving:
         out["serving"] = serving
     if not args.no_cpu:
@@ -451,6 +543,14 @@ def main():
                 "value": round(mlups2, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
                 "sample": f"{args.cpu_two_level_steps} coarse step(s) of the same two-level workload "
                           f"({secs2:.1f} s), serial C restatement of the reference's Rohde cycle",
+            }
+        if threelev and args.cpu_three_level_steps:
+            log(f"[bench] CPU baseline, three-level: oracle, 1 thread, {args.cpu_three_level_steps} level-0 step(s)")
+            mlups3, secs3 = cpu_three_level(args.cpu_three_level_steps)
+            threelev["cpu_baseline"] = {
+                "value": round(mlups3, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+                "sample": f"{args.cpu_three_level_steps} level-0 step(s) of the same three-level workload "
+                          f"({secs3:.1f} s), serial C restatement of the reference's recursive Rohde cycle",
             }
     print(json.dumps(out), flush=True)
 

@@ -22,6 +22,8 @@ AmrSim::AmrSim(int nx, int ny, int nz, int max_ref_level,
       max_level_(max_ref_level) {
   if (nx < 1 || ny < 1 || nz < 1) throw lambrex::Error("domain extents must be positive");
   if (max_ref_level < 0) throw lambrex::Error("max_ref_level must be >= 0");
+  if (max_ref_level >= LBX_MAX_LEVELS)
+    throw lambrex::Error("max_ref_level must be below " + std::to_string(LBX_MAX_LEVELS));
   // The reference aborts on a non-periodic direction (src/AmrSim.cpp:788-797);
   // here such a direction gets halfway bounce-back walls (single level only).
   const int num_levels = max_level_ + 1;
@@ -29,6 +31,7 @@ AmrSim::AmrSim(int nx, int ny, int nz, int max_ref_level,
   tau_b_.assign(num_levels, 0.0);
   mass_.assign(num_levels, 0.0);
   levels_.resize(num_levels);
+  static_tags_.assign(num_levels, lambrex::IBox{});
   tau_s_.at(0) = tau_s;
   tau_b_.at(0) = tau_b;
 }
@@ -181,9 +184,8 @@ void AmrSim::InitFromScratch(double time) {
 
 // src/AmrSim.cpp:746-751
 void AmrSim::ClearLevel(int level) {
-  Check(lbx_level_clear(sim_, level));
-  LevelHost& L = levels_.at(level);
-  L = LevelHost{};
+  Check(lbx_level_clear(sim_, level));  // and every finer level
+  for (int l = level; l <= max_level_; ++l) levels_.at(l) = LevelHost{};
   if (level <= finest_level_) finest_level_ = level > 0 ? level - 1 : 0;
 }
 
@@ -239,21 +241,33 @@ void AmrSim::Iterate(int nsteps) {
     }
     if (nsteps % 2) C.cur = 1 - C.cur;  // UpdateNow swaps Density too
   } else {
-    // UpdateDistribution (:619-631): the finest level advances two steps.
-    LevelHost& F = levels_.at(1);
-    for (int t = 0; t < nsteps; ++t) {
-      F.time += 2 * F.dt;
-      F.step += 2;
-      C.time += C.dt;
-      ++C.step;
-    }
+    // UpdateDistribution (:619-631): a level between 0 and the finest takes
+    // one step per RohdeCycle, 2^l cycles per level-0 step; the finest level
+    // takes two steps per cycle of its parent.
+    for (int t = 0; t < nsteps; ++t)
+      for (int l = 0; l <= finest_level_; ++l) {
+        LevelHost& L = levels_.at(l);
+        if (l > 0 && l == finest_level_) {
+          for (int c = 0; c < (1 << (l - 1)); ++c) {
+            L.time += 2 * L.dt;
+            L.step += 2;
+          }
+        } else {
+          for (int c = 0; c < (1 << l); ++c) {
+            L.time += L.dt;
+            ++L.step;
+          }
+        }
+      }
   }
 }
 
 // src/AmrSim.cpp:689-713: the fine level from the coarse one.
 void AmrSim::MakeNewLevelFromCoarse(int level, double time, const std::vector<lambrex::IBox>& ba) {
   if (!level) throw lambrex::Error("Cannot construct level 0 from a coarser level.");
-  if (level != 1) throw lambrex::Error("only one refined level is supported");
+  if (level > max_level_ || level != finest_level_ + 1)
+    throw lambrex::Error("level " + std::to_string(level) + " cannot be made from level " +
+                         std::to_string(level - 1));
   LevelHost& L = levels_.at(level);
   L.boxes = ba;
   const double nan = std::numeric_limits<double>::quiet_NaN();
@@ -295,17 +309,19 @@ void AmrSim::Regrid(int criterion, double threshold, int blocking, int buffer) {
       fba[b].lo[d] = lohi[6 * b + d];
       fba[b].hi[d] = lohi[6 * b + 3 + d];
     }
-  static_tag_ = lambrex::IBox{};
+  static_tags_.at(0) = lambrex::IBox{};
   if (fba.empty()) {
     if (finest_level_ >= 1) ClearLevel(1);
-    finest_level_ = 0;
     return;
   }
-  if (finest_level_ < 1)
+  if (finest_level_ < 1) {
     MakeNewLevelFromCoarse(1, GetTime(0), fba);
-  else if (!(fba == levels_.at(1).boxes))
+  } else if (!(fba == levels_.at(1).boxes)) {
+    if (finest_level_ > 1) ClearLevel(2);  // level 1 is remade as the finest level
     RemakeLevel(1, GetTime(0), fba);
+  }
   MakeFineMask(0);
+  RegridFrom(1);
 }
 
 namespace {
@@ -398,7 +414,8 @@ void AmrSim::WritePlotfile(const std::string& dir) {
 // valid cell keeps the old fine populations where an old fine box covered
 // it and takes its coarse parent's otherwise (DistFnFillPatch, :359-391).
 void AmrSim::RemakeLevel(int level, double time, const std::vector<lambrex::IBox>& ba) {
-  if (level != 1 || finest_level_ < 1) throw lambrex::Error("RemakeLevel: level 1 does not exist");
+  if (level < 1 || level != finest_level_)
+    throw lambrex::Error("RemakeLevel: level " + std::to_string(level) + " is not the finest level");
   LevelHost& L = levels_.at(level);
   const std::vector<int32_t> lohi = lambrex::to_lohi(ba);
   Check(lbx_level_remake(sim_, level, (int)ba.size(), lohi.data()));
@@ -420,50 +437,73 @@ void AmrSim::RemakeLevel(int level, double time, const std::vector<lambrex::IBox
 // coarsen(coarse boxes, 2); SetFineMaskFromFineGrids(true) uses the fine
 // BoxArray instead.
 void AmrSim::MakeFineMask(int coarse_level) {
-  if (coarse_level != 0 || finest_level_ < 1) return;
+  if (coarse_level < 0 || coarse_level >= finest_level_) return;
   const std::vector<lambrex::IBox>& fba =
-      mask_from_fine_grids_ ? levels_.at(1).boxes : levels_.at(0).boxes;
+      mask_from_fine_grids_ ? levels_.at(coarse_level + 1).boxes : levels_.at(coarse_level).boxes;
   const std::vector<int32_t> lohi = lambrex::to_lohi(fba);
-  Check(lbx_set_fine_mask(sim_, (int)fba.size(), lohi.data()));
+  Check(lbx_set_level_mask(sim_, coarse_level, (int)fba.size(), lohi.data()));
 }
 
 void AmrSim::SetFineMaskFromFineGrids(bool on) {
   mask_from_fine_grids_ = on;
-  MakeFineMask(0);
+  for (int l = 0; l < finest_level_; ++l) MakeFineMask(l);
 }
 
 // src/AmrSim.cpp:995-1017.  AMReX's regrid (tagging, error buffer,
-// clustering) is not reproduced: the fine level is exactly the refined tag
-// box, clipped to the domain and cut at max_grid_size.
+// clustering) is not reproduced: level l+1 is exactly the refined tag box of
+// level l, clipped to level l's extent (for l >= 1 shrunk by one cell, so
+// the finer level's ghost cells keep parents in level l: proper nesting) and
+// cut at max_grid_size.
 void AmrSim::SetStaticRefinement(int level, const std::array<int, NDIMS>& lo_corner,
                                  const std::array<int, NDIMS>& hi_corner) {
   if (!sim_) throw lambrex::Error("SetStaticRefinement before InitFromScratch");
-  if (level != 0 || max_level_ < 1)
-    throw lambrex::Error("SetStaticRefinement: only level 0 can be refined (max_level >= 1)");
+  if (level < 0 || level >= max_level_ || level > finest_level_)
+    throw lambrex::Error("SetStaticRefinement: level " + std::to_string(level) +
+                         " cannot be refined (it must exist and be below max_level)");
   lambrex::IBox tag;
   for (int d = 0; d < NDIMS; ++d) {
-    tag.lo[d] = std::max(lo_corner[d], 0);
-    tag.hi[d] = std::min(hi_corner[d], GetDims()[d] - 1);
+    tag.lo[d] = lo_corner[d];
+    tag.hi[d] = hi_corner[d];
   }
   if (!tag.ok()) throw lambrex::Error("SetStaticRefinement: empty refinement region");
-  static_tag_ = tag;
-  const std::vector<lambrex::IBox> fba =
-      lambrex::max_size_chop(lambrex::refine(tag, kRefRatio), max_grid_size_);
-  // AmrCore::regrid: a new level is made from the coarse one, an existing
-  // level with a different BoxArray is remade (keeping its data), an
-  // unchanged one is left alone.
-  if (finest_level_ < 1)
-    MakeNewLevelFromCoarse(1, GetTime(0), fba);
-  else if (!(fba == levels_.at(1).boxes))
-    RemakeLevel(1, GetTime(0), fba);
-  MakeFineMask(level);
+  static_tags_.at(level) = tag;
+  RegridFrom(level);
+}
+
+void AmrSim::RegridFrom(int base) {
+  for (int l = base; l < max_level_ && l <= finest_level_; ++l) {
+    lambrex::IBox tag = static_tags_.at(l);
+    if (tag.ok()) {
+      const lambrex::IBox ext = lambrex::minimal_box(levels_.at(l).boxes);
+      const int shrink = l > 0 ? 1 : 0;
+      for (int d = 0; d < NDIMS; ++d) {
+        tag.lo[d] = std::max(tag.lo[d], ext.lo[d] + shrink);
+        tag.hi[d] = std::min(tag.hi[d], ext.hi[d] - shrink);
+      }
+    }
+    if (!tag.ok()) {  // nothing to refine: no finer levels
+      if (finest_level_ > l) ClearLevel(l + 1);
+      return;
+    }
+    const std::vector<lambrex::IBox> fba =
+        lambrex::max_size_chop(lambrex::refine(tag, kRefRatio), max_grid_size_);
+    // AmrCore::regrid: a new level is made from the coarse one, an existing
+    // level with a different BoxArray is remade (keeping its data; its finer
+    // levels are rebuilt above it), an unchanged one is left alone.
+    if (finest_level_ < l + 1) {
+      MakeNewLevelFromCoarse(l + 1, GetTime(l), fba);
+    } else if (!(fba == levels_.at(l + 1).boxes)) {
+      if (finest_level_ > l + 1) ClearLevel(l + 2);
+      RemakeLevel(l + 1, GetTime(l), fba);
+    }
+    MakeFineMask(l);
+  }
 }
 
 void AmrSim::UnsetStaticRefinement(int level) {
-  if (level != 0 || max_level_ < 1) throw lambrex::Error("UnsetStaticRefinement: level out of range");
-  static_tag_ = lambrex::IBox{};
-  if (finest_level_ >= 1) ClearLevel(1);
-  finest_level_ = 0;
+  if (level < 0 || level >= max_level_) throw lambrex::Error("UnsetStaticRefinement: level out of range");
+  static_tags_.at(level) = lambrex::IBox{};
+  if (finest_level_ > level) ClearLevel(level + 1);
 }
 
 // src/AmrSim.cpp:1019-1030

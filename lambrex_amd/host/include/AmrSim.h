@@ -6,6 +6,8 @@
 //     periodicity flag is set; where the reference aborts on a non-periodic
 //     direction (src/AmrSim.cpp:788-797), this build puts halfway
 //     bounce-back walls on its faces (single-level simulations only);
+//   - up to LBX_MAX_LEVELS levels (max_ref_level < LBX_MAX_LEVELS), ratio 2,
+//     advanced by the reference's recursive Rohde cycle;
 //   - initial density / velocity vectors in C order, i slowest, velocity
 //     component fastest (include/AmrSim.h:79-83);
 //   - omega = 1/(tau + 1/2) (src/AmrSim.cpp:125-126);
@@ -96,7 +98,8 @@ class AmrSim {
   // (LBX_TAG_DENSITY_GRADIENT or LBX_TAG_SPEED) exceeds `threshold`, grow
   // the tags by `buffer` cells, refine the tiles of `blocking` coarse cells
   // that hold one, and make, remake (keeping fine data) or clear level 1
-  // like AmrCore::regrid.  Replaces any static refinement box.
+  // like AmrCore::regrid.  Replaces level 0's static refinement box; finer
+  // levels are then rebuilt from their static boxes.
   void Regrid(int criterion, double threshold, int blocking = 4, int buffer = 1);
   // Bulk output: density and velocity of every level (CalcHydroVars first)
   // written to `dir` in AMReX's plotfile layout: a HyperCLaw-V1.1 Header,
@@ -139,6 +142,9 @@ class AmrSim {
   void RemakeLevel(int level, double time, const std::vector<lambrex::IBox>& ba);
   void MakeFineMask(int coarse_level);
   void ClearLevel(int level);
+  // AmrCore::regrid(base): levels base+1 .. max from the static boxes of
+  // levels base .. max-1.
+  void RegridFrom(int base);
   void InitDensity(int level);
   void InitVelocity(int level);
   void Check(int rc) const;
@@ -154,7 +160,7 @@ class AmrSim {
   int precision_ = LBX_FP64;
   double rho0_ = 1.0;
   bool mask_from_fine_grids_ = false;
-  lambrex::IBox static_tag_;
+  std::vector<lambrex::IBox> static_tags_;  // per level (TagCell, src/AmrSim.cpp:413-417)
   std::vector<double> initial_density;
   std::vector<double> initial_velocity;
   std::vector<double> tau_s_, tau_b_, mass_;

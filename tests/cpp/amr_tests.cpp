@@ -2,6 +2,7 @@
 // (tests/catch2AMRTests.cpp:118-423) against this build's AmrSim facade.
 // Sections that inspect AMReX TagBoxArrays (ErrorEst) have no counterpart
 // here: tagging is reduced to the static refinement box.
+#include <cmath>
 #include <vector>
 
 #include "check.h"
@@ -172,6 +173,59 @@ TEST_CASE("Refinement needs max_level >= 1") {
   sim.SetInitialVelocity(0.0);
   sim.InitFromScratch(0.0);
   REQUIRE_THROWS(sim.SetStaticRefinement(0, {0, 0, 0}, {3, 3, 3}));
+}
+
+// Three levels (RohdeCycle's recursion, src/AmrSim.cpp:453-455): static
+// boxes on levels 0 and 1, the dt / tau / step bookkeeping of ComputeDt and
+// UpdateDistribution, re-refinement of level 0 under an existing level 2
+// (level 1 is remade, level 2 rebuilt from its static box), and removal.
+TEST_CASE("ThreeLevel static refinement") {
+  AmrSim sim(32, 16, 16, 2, {1, 1, 1}, 0.8, 0.8);
+  sim.SetInitialDensity(1.0);
+  sim.SetInitialVelocity(0.01);
+  sim.InitFromScratch(0.0);
+  REQUIRE_THROWS(sim.SetStaticRefinement(1, {20, 10, 10}, {43, 21, 21}));  // level 1 does not exist yet
+  sim.SetStaticRefinement(0, {8, 4, 4}, {23, 11, 11});
+  sim.SetStaticRefinement(1, {20, 10, 10}, {43, 21, 21});
+  REQUIRE(sim.finestLevel() == 2);
+  auto e2 = sim.GetExtent(2);
+  REQUIRE(e2.first[0] == 40);
+  REQUIRE(e2.second[0] == 87);
+  REQUIRE(e2.first[1] == 20);
+  REQUIRE(e2.second[2] == 43);
+  REQUIRE(sim.GetDt(2) == 0.25);
+  REQUIRE(sim.GetTauS(2) == Approx(2 * (sim.GetTauS(1) - 0.5) + 0.5));
+  REQUIRE(sim.GetTauS(1) == Approx(2 * (sim.GetTauS(0) - 0.5) + 0.5));
+  sim.Iterate(3);
+  for (int l = 0; l <= 2; ++l) {
+    REQUIRE(sim.GetTimeStep(l) == 3 << l);
+    REQUIRE(sim.GetTime(l) == 3.0);
+  }
+  sim.CalcHydroVars(2);
+  REQUIRE(sim.OnProcessDensity(sim.GetDensity(60, 30, 30, 2)));
+  REQUIRE(std::isfinite(sim.GetDensity(60, 30, 30, 2)));
+  REQUIRE(!sim.OnProcessDensity(sim.GetDensity(10, 10, 10, 2)));
+  // a level-1 static box reaching past level 1 is clipped one cell inside it
+  sim.SetStaticRefinement(1, {0, 0, 0}, {63, 31, 31});
+  auto e2b = sim.GetExtent(2);
+  REQUIRE(e2b.first[0] == 2 * 17);
+  REQUIRE(e2b.second[0] == 2 * 46 + 1);
+  // moving level 0's box remakes level 1 and rebuilds level 2 inside it
+  sim.SetStaticRefinement(0, {6, 4, 4}, {21, 11, 11});
+  REQUIRE(sim.finestLevel() == 2);
+  auto e1 = sim.GetExtent(1);
+  REQUIRE(e1.first[0] == 12);
+  REQUIRE(sim.GetExtent(2).first[0] == 2 * 13);
+  sim.Iterate(1);
+  REQUIRE(sim.GetTimeStep(2) == 4);      // a level made from coarse restarts at step 0 (src/AmrSim.cpp:700)
+  REQUIRE(sim.GetTimeStep(1) == 6 + 2);  // a remade level keeps its count (:715-744)
+  sim.UnsetStaticRefinement(1);
+  REQUIRE(sim.finestLevel() == 1);
+  REQUIRE(sim.boxArray(2).empty());
+  sim.Iterate(1);
+  REQUIRE(sim.GetTimeStep(1) == 8 + 2);
+  REQUIRE_THROWS(sim.SetStaticRefinement(2, {0, 0, 0}, {3, 3, 3}));  // max_level is 2
+  REQUIRE_THROWS(AmrSim(8, 8, 8, LBX_MAX_LEVELS, {1, 1, 1}, 0.8, 0.8));
 }
 
 int main() {
