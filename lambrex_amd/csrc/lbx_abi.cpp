@@ -93,6 +93,10 @@ constexpr int kPartials = 1024;  // blocks of the conservation reduction
 struct lbx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) and the events that fork it from and
+  // join it back to `stream` for independent patch groups
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   double* scratch = nullptr;  // kPartials*4 partials + 4 results
   // profiling of stream-collide launches
@@ -394,7 +398,10 @@ void free_levels_from(lbx_sim* sim, int level) {
 
 // The kernels of one level-0 step of the Rohde cycle (src/AmrSim.cpp:430-469)
 // for the current buffer parities.
-constexpr int kFinePatchGroup = 16;  // patches per fine launch group (0 = all)
+// The finest level's patches run in groups of about kFineGroupBytes of state
+// (one buffer), alternating over two streams: two groups' intermediate
+// arrays (≈ 2 x 2 x 48 MB) then stay in the 256 MB MALL.
+constexpr long long kFineGroupBytes = 48ll << 20;
 
 // The finest level's two substeps, from its parent's current state
 // (level 0's `now`, or a middle level's state, which stays put until its own
@@ -414,17 +421,45 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
   // The fine patches' two substeps depend on no other patch's substeps (the
   // ghosts are filled once, from fnow and cnow), so they run in groups of
   // patches whose intermediate arrays stay in the MALL.
-  const int group = kFinePatchGroup > 0 ? kFinePatchGroup : np;
+  // Groups write disjoint patches (and block sums), so consecutive groups
+  // alternate between the context's stream and its side stream, forked from
+  // and joined back to `st` around the loop; one group's launch ramp then
+  // overlaps the other's tail.
+  lbx_ctx* ctx = sim->ctx;
+  std::vector<std::pair<int, int>> groups;  // (first patch, count)
+  for (int first = 0; first < np;) {
+    long long bytes = 0;
+    int n = 0;
+    while (first + n < np) {
+      const long long pb = F.desc[first + n].qs * lbx::NV_TOTAL * (long long)elem_size(prec);
+      if (n > 0 && bytes + pb > kFineGroupBytes) break;
+      bytes += pb;
+      ++n;
+    }
+    groups.push_back({first, n});
+    first += n;
+  }
+  const bool fork = groups.size() > 1;
   hipError_t e = hipSuccess;
-  for (int first = 0; first < np && e == hipSuccess; first += group) {
-    const int n = std::min(group, np - first);
+  if (fork) {
+    e = hipEventRecord(ctx->fork_ev, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0);
+  }
+  for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
+    const int first = groups[k].first, n = groups[k].second;
+    hipStream_t gs = (k & 1) ? ctx->side : st;
     e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, cpatches, F.d_desc, first, n, F.max_cells,
-                                      F.d_fill, nullptr, rf, st);
+                                      F.d_fill, nullptr, rf, gs);
     if (e == hipSuccess)
-      e = lbx::launch_fine_stream_collide(prec, fnext, ftmp, F.d_desc, first, n, F.max_cells, rf, st);
+      e = lbx::launch_fine_stream_collide(prec, fnext, ftmp, F.d_desc, first, n, F.max_cells, rf, gs);
     if (e == hipSuccess)
       e = lbx::launch_fine_stream_avg(prec, ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n, F.max_blocks,
-                                      sim->rho0, st);
+                                      sim->rho0, gs);
+  }
+  if (fork) {  // join even after a failed launch, so a capture can end
+    const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
+    const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(st, ctx->join_ev, 0) : ej;
+    if (e == hipSuccess) e = ew;
   }
   fpar = 1 - fpar;
   return e;
@@ -976,9 +1011,15 @@ int lbx_ctx_create(int device, lbx_ctx** out) {
   if (!ctx) return fail(nullptr, LBX_E_NOMEM, "out of host memory");
   ctx->device = device;
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&ctx->scratch, (kPartials * 4 + 4) * sizeof(double));
   if (e != hipSuccess) {
     hip_fail(nullptr, e, "context set-up");
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LBX_E_HIP;
@@ -992,7 +1033,11 @@ int lbx_ctx_destroy(lbx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  (void)hipStreamSynchronize(ctx->side);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  (void)hipEventDestroy(ctx->join_ev);
+  (void)hipEventDestroy(ctx->fork_ev);
+  (void)hipStreamDestroy(ctx->side);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LBX_OK;

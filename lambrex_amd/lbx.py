@@ -232,8 +232,11 @@ class Sim:
             self.h = None
 
     def define_level(self, level, boxes):
+        """Boxes of `level`; finer levels are released (lbx_level_define)."""
         lohi = _lohi(boxes)
         self.ctx.check(self._lib.lbx_level_define(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))
+        for l in [l for l in self.boxes if l > level]:
+            del self.boxes[l]
         self.boxes[level] = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
 
     def remake_level(self, level, boxes):
