@@ -304,3 +304,24 @@ def test_remake_level_keeps_fine_data_then_matches_oracle(ctx):
     assert_bitwise(o.get_f(0), fg0, "coarse f after remake + 2 steps")
     assert_bitwise(o.get_f(1), fg1, "fine f after remake + 2 steps")
     o.close()
+
+
+def test_patch_groups_on_two_streams_bitwise(ctx):
+    """12 fine boxes of 32^3 (5.6 MB of FP64 state each) form more than one
+    launch group, so the fine substeps run on both streams of the context,
+    in the captured step and eagerly (profiling on): both equal the oracle."""
+    dims = (64, 32, 32)
+    rho, u = workloads.random_state(dims, seed=31, rho_sigma=1e-2, u_amp=0.02)
+    (fo0, fo1, _), (fg0, fg1, _, _), fboxes = run_both(ctx, dims, 16, 0.8, rho, u, (16, 0, 0), (111, 63, 63), 32,
+                                                       "fine", 2)
+    assert len(fboxes) == 12
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
+    ctx.profile_enable(True)
+    try:
+        (_, _, _), (eg0, eg1, _, _), _ = run_both(ctx, dims, 16, 0.8, rho, u, (16, 0, 0), (111, 63, 63), 32,
+                                                  "fine", 2)
+    finally:
+        ctx.profile_enable(False)
+    assert_bitwise(fo0, eg0, "coarse f, eager")
+    assert_bitwise(fo1, eg1, "fine f, eager")
