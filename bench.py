@@ -142,10 +142,11 @@ def serving_precision_sample(ctx, trace, n):
     return e_rho, e_u
 
 
-def serve(ctx, trace, max_active):
+def serve(ctx, trace, max_active, policy=lbx.LBX_SERVE_FIFO):
     """Replay the trace in real time against one server; returns aggregate
     MLUPS over the makespan and completion latencies (finish - arrival)."""
-    srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=16 << 30, max_active=max_active)
+    srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=16 << 30, max_active=max_active,
+                     policy=policy)
     results = []
     submitted = 0
     arrival_of = {}
@@ -440,6 +441,10 @@ def main():
             batched = serve(ctx, trace, max_active=44)
             log(f"[bench] batched: {batched['mlups']:.0f} MLUPS, p50 {batched['p50_ms']:.0f} ms, "
                 f"p99 {batched['p99_ms']:.0f} ms, makespan {batched['makespan_s']:.1f} s")
+            log("[bench] serving: same trace, batched, shortest-first admission")
+            sjf = serve(ctx, trace, max_active=44, policy=lbx.LBX_SERVE_SHORTEST_FIRST)
+            log(f"[bench] shortest-first: {sjf['mlups']:.0f} MLUPS, p50 {sjf['p50_ms']:.0f} ms, "
+                f"p99 {sjf['p99_ms']:.0f} ms, makespan {sjf['makespan_s']:.1f} s")
             log("[bench] serving: same trace, one run at a time")
             seq = serve(ctx, trace, max_active=1)
             log(f"[bench] sequential: {seq['mlups']:.0f} MLUPS, p50 {seq['p50_ms']:.0f} ms, "
@@ -448,7 +453,7 @@ def main():
             log(f"[bench] serving FP32 vs FP64 on the first 16 requests: max rel rho err {e_rho:.2e}, "
                 f"max |du|/amplitude {e_u:.2e}")
             serving = {"requests": len(trace), "arrival": "Poisson 200/s (seed 7)", "precision": "FP32-shifted",
-                       "batched": batched, "sequential": seq,
+                       "batched": batched, "batched_shortest_first": sjf, "sequential": seq,
                        "speedup_aggregate_mlups": round(batched["mlups"] / seq["mlups"], 3),
                        "fp32_vs_fp64_first_16": {"max_rel_rho_err": e_rho, "max_du_over_amplitude": e_u}}
         r32 = None

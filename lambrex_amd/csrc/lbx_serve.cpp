@@ -3,7 +3,11 @@
 //
 // Requests queue on the host.  A request is admitted when the device arena
 // (one allocation made at server creation, carved first-fit) has room for its
-// two population buffers and fewer than kServeMaxRuns runs are active.  All
+// two population buffers and fewer than kServeMaxRuns runs are active.  The
+// default policy admits in arrival order and stops at the first request that
+// does not fit; LBX_SERVE_SHORTEST_FIRST admits the queued requests in order
+// of their work (cells x steps, ties by arrival), skipping any that do not
+// fit yet.  All
 // active runs advance together: one patch-list kernel launch per global step
 // walks the patch list of every active run.  At most kInFlight steps are
 // queued ahead of the GPU, so admission and completion are observed within a
@@ -116,6 +120,7 @@ struct lbx_server {
   int precision = LBX_FP32_SHIFTED;
   double rho0 = 1.0;
   int max_active = lbx::kServeMaxRuns;
+  int policy = LBX_SERVE_FIFO;
   int grid = 2048;
   Arena arena;
   void* arena_mem = nullptr;
@@ -158,19 +163,44 @@ size_t esize(int precision) { return precision == LBX_FP64 ? 8 : 4; }
 
 long long cells_of(const lbx_request& q) { return (long long)q.dims[0] * q.dims[1] * q.dims[2]; }
 
-// Admission: move queued requests into the active set while they fit.
+// Admission: move queued requests into the active set while they fit, in
+// arrival order (FIFO, stopping at the first that does not fit) or in order
+// of work (shortest first, skipping those that do not fit yet).
 int admit(lbx_server* s) {
-  while (!s->queue.empty() && (int)s->active.size() < s->max_active) {
-    Request& q = s->queue.front();
+  // admitted entries leave a tombstone (id 0) until the scan is over; clear
+  // any left by a scan that stopped on an error
+  auto sweep = [s]() {
+    s->queue.erase(std::remove_if(s->queue.begin(), s->queue.end(), [](const Request& q) { return q.id == 0; }),
+                   s->queue.end());
+  };
+  sweep();
+  std::vector<size_t> order;  // queue positions to try
+  if (s->policy == LBX_SERVE_SHORTEST_FIRST) {
+    order.resize(s->queue.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+      return cells_of(s->queue[a].req) * s->queue[a].req.steps < cells_of(s->queue[b].req) * s->queue[b].req.steps;
+    });
+  }
+  for (size_t k = 0; (int)s->active.size() < s->max_active && !s->free_slots.empty(); ++k) {
+    const bool fifo = s->policy != LBX_SERVE_SHORTEST_FIRST;
+    if (fifo ? s->queue.empty() : k >= order.size()) break;
+    Request& q = fifo ? s->queue.front() : s->queue[order[k]];
     const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2]);
     const size_t bytes = (size_t)2 * L.elems() * esize(s->precision);
     size_t off = 0;
-    if (s->free_slots.empty() || !s->arena.alloc(bytes, &off)) break;  // FIFO admission
+    if (!s->arena.alloc(bytes, &off)) {
+      if (fifo) break;  // head of line waits
+      continue;         // a smaller one later in the order may fit
+    }
     Run r;
     r.slot = s->free_slots.back();
     s->free_slots.pop_back();
     r.r = std::move(q);
-    s->queue.pop_front();
+    if (fifo)
+      s->queue.pop_front();
+    else
+      q.id = 0;
     r.off = off;
     r.bytes = bytes;
     r.lat = L;
@@ -197,6 +227,7 @@ int admit(lbx_server* s) {
     }
     s->active.push_back(std::move(r));
   }
+  sweep();
   return LBX_OK;
 }
 
@@ -354,6 +385,14 @@ int lbx_serve_create(lbx_ctx* ctx, int precision, double rho0, size_t arena_byte
   }
   s->arena.reset((char*)s->arena_mem, arena_bytes);
   *out = s;
+  return LBX_OK;
+}
+
+int lbx_serve_set_policy(lbx_server* s, int policy) {
+  if (!s) return LBX_E_ARG;
+  if (policy != LBX_SERVE_FIFO && policy != LBX_SERVE_SHORTEST_FIRST)
+    return sfail(s, LBX_E_ARG, "unknown admission policy %d", policy);
+  s->policy = policy;
   return LBX_OK;
 }
 

@@ -21,6 +21,8 @@ LBX_E_UNSUPPORTED = -5
 LBX_FP64 = 0
 LBX_FP32_SHIFTED = 1
 LBX_MAX_LEVELS = 6
+LBX_SERVE_FIFO = 0
+LBX_SERVE_SHORTEST_FIRST = 1
 
 # Every extern "C" entry point of include/lbx.h with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -83,6 +85,7 @@ SIGNATURES.update({
     "lbx_serve_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_int,
                                         ctypes.POINTER(_vp)]),
     "lbx_serve_destroy": (ctypes.c_int, [_vp]),
+    "lbx_serve_set_policy": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_serve_last_error": (ctypes.c_char_p, [_vp]),
     "lbx_serve_submit": (ctypes.c_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
     "lbx_serve_progress": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -395,7 +398,8 @@ class Sim:
 class Server:
     """Batched serving of independent periodic runs (lbx_serve_*)."""
 
-    def __init__(self, ctx, precision=LBX_FP32_SHIFTED, rho0=1.0, arena_bytes=0, max_active=0):
+    def __init__(self, ctx, precision=LBX_FP32_SHIFTED, rho0=1.0, arena_bytes=0, max_active=0,
+                 policy=None):
         self.ctx = ctx
         self._lib = ctx._lib
         h = ctypes.c_void_p()
@@ -405,6 +409,12 @@ class Server:
             raise LbxError(rc, "lbx_serve_create failed")
         self.h = h
         self._keep = {}  # request id -> buffers that must outlive the run
+        if policy is not None:
+            self.set_policy(policy)
+
+    def set_policy(self, policy):
+        """LBX_SERVE_FIFO or LBX_SERVE_SHORTEST_FIRST admission."""
+        self._check(self._lib.lbx_serve_set_policy(self.h, int(policy)))
 
     def _check(self, rc):
         if rc != LBX_OK:

@@ -114,3 +114,33 @@ def test_bad_requests_are_refused(ctx):
             srv.submit(dims, steps, tau, plane=plane)
         assert e.value.code == lbx.LBX_E_ARG, (dims, tau, plane)
     srv.close()
+
+
+def test_shortest_first_admission(ctx):
+    """LBX_SERVE_SHORTEST_FIRST: with one run active at a time, queued runs
+    are admitted in order of work (cells x steps, ties by arrival), and each
+    run's populations are the same bits as under FIFO admission."""
+    reqs = [((32, 32, 32), 40), ((16, 16, 16), 10), ((24, 24, 24), 20), ((16, 16, 16), 10), ((32, 32, 32), 5)]
+    out = {}
+    for policy in (lbx.LBX_SERVE_FIFO, lbx.LBX_SERVE_SHORTEST_FIRST):
+        srv = lbx.Server(ctx, precision=lbx.LBX_FP64, arena_bytes=1 << 28, max_active=1, policy=policy)
+        subs = [srv.submit(d, n, 0.8, amplitude=1e-3, want_f=True) for d, n in reqs]
+        srv.drain()
+        res = {r["id"]: r for r in srv.poll()}
+        srv.close()
+        assert len(res) == len(reqs)
+        admit_order = sorted(range(len(reqs)), key=lambda k: res[subs[k][0]]["t_admit"])
+        out[policy] = (admit_order, [o["f"] for _, o in subs])
+    assert out[lbx.LBX_SERVE_FIFO][0] == [0, 1, 2, 3, 4]
+    work = [np.prod(d) * n for d, n in reqs]
+    assert out[lbx.LBX_SERVE_SHORTEST_FIRST][0] == sorted(range(len(reqs)), key=lambda k: (work[k], k))
+    for a, b in zip(out[lbx.LBX_SERVE_FIFO][1], out[lbx.LBX_SERVE_SHORTEST_FIRST][1]):
+        assert bitwise_equal(a, b)
+
+
+def test_unknown_policy_is_refused(ctx):
+    srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=16 << 20)
+    with pytest.raises(lbx.LbxError) as e:
+        srv.set_policy(7)
+    assert e.value.code == lbx.LBX_E_ARG
+    srv.close()
