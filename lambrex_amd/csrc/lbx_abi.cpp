@@ -1371,7 +1371,7 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]) {
     LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
     LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 0, L.buf[L.cur], L.d_desc, (int)b, (int)n, (double*)t.p,
                                       nullptr, sim->rho0, ctx->stream));
-    const Lattice packed{(int)n, 1, 1, n, lbx::NV_TOTAL * n};  // population-major, stride n
+    const Lattice packed{(int)n, 1, 1, n, lbx::NV_TOTAL * n, 0u};  // population-major, stride n
     LBX_HIP(ctx, lbx::launch_conservation(LBX_FP64, t.p, packed, 0.0, partials, kPartials, res,
                                           ctx->stream));
     double part[4];
