@@ -159,3 +159,45 @@ def test_plotfile_round_trip(tmp_path):
                if all(lo[a] <= p <= hi[a] for a, p in enumerate((i, j, k)))]
         assert len(hit) == 1, (lev, i, j, k)
         assert [float(v) for v in hit[0]] == want
+
+
+def test_three_level_facade_matches_oracle(tmp_path):
+    """A three-level run driven only through the AmrSim facade
+    (SetStaticRefinement on levels 0 and 1, the reference's own fine masks,
+    tau_s != tau_b, two level-0 steps) equals the oracle run on the boxes the
+    facade made: density and velocity of every level, bitwise."""
+    from oracle import oracle as orc
+
+    exe = BIN / "three_level_dump"
+    assert exe.exists()
+    out_dir = tmp_path / "plt3"
+    r = subprocess.run([str(exe), str(out_dir)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert "FINEST 2" in r.stdout
+    h, levels = read_plotfile(out_dir)
+    assert h["finest"] == 2 and h["steps"] == [2, 4, 8]
+    NX, NY, NZ = 16, 12, 8
+    rho = np.empty((NX, NY, NZ))
+    u = np.empty((NX, NY, NZ, 3))
+    for i in range(NX):
+        for j in range(NY):
+            for k in range(NZ):
+                rho[i, j, k] = 1.0 + 0.01 * (i == 7) + 0.001 * ((j + 2 * k) % 5)
+                u[i, j, k] = (0.002 * ((i + k) % 3), -0.001 * (j % 2), 0.0005 * ((i * j) % 4))
+    boxes = [[(list(lo), list(hi)) for lo, hi, _ in lev["fabs"]] for lev in levels]
+    o = orc.Sim(NX, NY, NZ, 0.7, 0.9, boxes=boxes[0])
+    o.init(rho.reshape(-1), u.reshape(-1))
+    for lev in (1, 2):
+        o.add_fine(boxes[lev], boxes[lev - 1])  # MakeFineMask passes the parent's own BoxArray
+    o.iterate(2)
+    for lev in range(3):
+        orho, ou = o.hydro(lev)
+        bb = o.bbox(lev)
+        for lo, hi, data in levels[lev]["fabs"]:
+            sl = tuple(slice(lo[a] - bb[a], hi[a] - bb[a] + 1) for a in (2, 1, 0))
+            want = np.concatenate([orho[sl][None], ou[(slice(None),) + sl]])
+            assert (np.ascontiguousarray(want).view(np.uint64) == np.ascontiguousarray(data).view(np.uint64)).all(), \
+                (lev, lo, hi)
+    o.close()
+
