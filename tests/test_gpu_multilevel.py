@@ -261,6 +261,16 @@ def test_hierarchy_errors(ctx):
         with pytest.raises(lbx.LbxError) as e:
             sim.remake_level(1, levels[1])  # level 1 has a finer level
         assert e.value.code == lbx.LBX_E_STATE
+        with pytest.raises(lbx.LbxError) as e:
+            sim.set_level_mask(3, levels[2])  # level 3 not defined
+        assert e.value.code == lbx.LBX_E_STATE
+        with pytest.raises(lbx.LbxError) as e:
+            sim.set_level_mask(lbx.LBX_MAX_LEVELS, levels[2])
+        assert e.value.code == lbx.LBX_E_ARG
+        with pytest.raises(lbx.LbxError) as e:
+            sim.fill_from_coarse(0)  # level 0 has no coarser level
+        assert e.value.code == lbx.LBX_E_ARG
+        sim.iterate(0)  # no-op
         sim.clear_level(1)  # releases level 2 too
         with pytest.raises(lbx.LbxError) as e:
             sim.conservation(2)
@@ -268,3 +278,35 @@ def test_hierarchy_errors(ctx):
         sim.iterate(1)  # single level again
     finally:
         sim.close()
+
+
+def test_fp32_remake_from_patched_parent_copies_stored_values(ctx):
+    """FP32-shifted storage: remaking level 2 copies the stored values, so
+    the reconstructed populations equal the old level-2 ones where an old box
+    covered the cell and the level-1 parent's elsewhere, bitwise."""
+    dims, levels = three_levels()
+    masks = masks_for("fine", levels)
+    rho, u = workloads.random_state(dims, seed=19, rho_sigma=1e-3, u_amp=0.01)
+    new2 = fine_boxes_in((24, 8, 4), (39, 23, 19), 8)
+    sim = make_gpu(ctx, dims, levels, masks, 0.8, rho, u, precision=lbx.LBX_FP32_SHIFTED)
+    try:
+        sim.iterate(2)
+        f1, lo1 = sim.get_f_bbox(1)
+        f2old, lo2old = sim.get_f_bbox(2)
+        sim.remake_level(2, new2)
+        f2, lo2 = sim.get_f_bbox(2)
+        sim.set_level_mask(1, new2)
+        sim.iterate(1)
+        assert np.isfinite(sim.get_f_bbox(2)[0]).all()
+    finally:
+        sim.close()
+    for (blo, bhi) in new2:
+        for z in range(blo[2], bhi[2] + 1):
+            for y in range(blo[1], bhi[1] + 1):
+                for x in range(blo[0], bhi[0] + 1):
+                    got = f2[:, z - lo2[2], y - lo2[1], x - lo2[0]]
+                    if any(all(a[d] <= p <= b[d] for d, p in enumerate((x, y, z))) for a, b in levels[2]):
+                        want = f2old[:, z - lo2old[2], y - lo2old[1], x - lo2old[0]]
+                    else:
+                        want = f1[:, z // 2 - lo1[2], y // 2 - lo1[1], x // 2 - lo1[0]]
+                    assert (got.view(np.uint64) == want.view(np.uint64)).all(), (x, y, z)
