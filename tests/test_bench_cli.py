@@ -13,3 +13,32 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+def test_hierarchy_workloads_are_nested_and_counted():
+    """The two- and three-level bench workloads: lattice updates per level-0
+    step match the boxes (level l steps 2^l times), boxes do not overlap, and
+    every level-2 cell of each box grown by 2 has its parent in level 1
+    (periodic), as lbx_level_define requires."""
+    import numpy as np
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    def cells(boxes):
+        return sum(int(np.prod([hi[d] - lo[d] + 1 for d in range(3)])) for lo, hi in boxes)
+
+    dims, cboxes, fboxes, mask, rho, u, lu2 = bench.two_level_geometry()
+    assert lu2 == cells(cboxes) + 2 * cells(fboxes)
+    dims, levels, masks, rho, u, lu3 = bench.three_level_geometry()
+    assert lu3 == sum(2 ** l * cells(b) for l, b in enumerate(levels))
+    assert masks == [levels[1], levels[2]]
+    n1 = [2 * d for d in dims]
+    cover = np.zeros(n1[::-1], dtype=np.int8)
+    for lo, hi in levels[1]:
+        cover[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] += 1
+    assert cover.max() == 1 and cover.sum() == cells(levels[1])
+    for lo, hi in levels[2]:
+        zs = np.arange(lo[2] - 2, hi[2] + 3) // 2 % n1[2]
+        ys = np.arange(lo[1] - 2, hi[1] + 3) // 2 % n1[1]
+        xs = np.arange(lo[0] - 2, hi[0] + 3) // 2 % n1[0]
+        assert cover[np.ix_(zs, ys, xs)].all(), (lo, hi)
