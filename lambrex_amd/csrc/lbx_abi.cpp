@@ -4,9 +4,11 @@
 // failure becomes a negative status plus a message in lbx_last_error().
 //
 // Level 0 is a periodic level array (z-plane major; its boxes are index
-// ranges into it).  Level 1, when defined, is a set of patches, one per fine box,
-// each grown by 2 ghost layers; the tables that drive its ghost fill and its
-// average-down are built here on the host once per level definition.
+// ranges into it).  Every finer level is a set of patches, one per box, each
+// grown by 2 ghost layers; the tables that drive its ghost fill (from its own
+// patches or its parent) and its average-down into the parent are built here
+// on the host once per level definition.  One level-0 step of a hierarchy is
+// the recursive Rohde cycle, captured as one HIP graph per buffer parity.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,7 +46,7 @@ struct Box {
 
 struct Level {
   bool defined = false;
-  bool patched = false;  // per-box ghosted storage (fine level)
+  bool patched = false;  // per-box ghosted storage (levels >= 1)
   std::vector<Box> boxes;
   Lattice lat{};         // level-array geometry (level 0)
   void* buf[3] = {nullptr, nullptr, nullptr};  // state in buf[cur]; buf[2] scratch (patched)
@@ -55,7 +57,7 @@ struct Level {
   std::vector<PatchDesc> desc;
   PatchDesc* d_desc = nullptr;
   int max_cells = 0;
-  // level 0 with a fine level: per-cell box-face and fine-mask flags of the
+  // level 0 with a finer level: per-cell box-face and fine-mask flags of the
   // coarse push (launch_coarse_push), and its launch shape
   unsigned char* d_mask = nullptr;
   LaunchShape push_shape{};
@@ -73,7 +75,8 @@ struct Level {
   int max_blocks = 0;            // largest (g/2)^3 of a patch
 };
 
-// One captured two-level coarse step per (coarse parity, fine parity).
+// One captured level-0 step of a hierarchy per (level-0 parity, finest-level
+// parity).
 struct StepGraphs {
   hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   void clear() {
