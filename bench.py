@@ -142,11 +142,11 @@ def serving_precision_sample(ctx, trace, n):
     return e_rho, e_u
 
 
-def serve(ctx, trace, max_active, policy=lbx.LBX_SERVE_FIFO):
+def serve(ctx, trace, max_active, policy=lbx.LBX_SERVE_FIFO, max_wait_s=0.0):
     """Replay the trace in real time against one server; returns aggregate
     MLUPS over the makespan and completion latencies (finish - arrival)."""
     srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=16 << 30, max_active=max_active,
-                     policy=policy)
+                     policy=policy, max_wait_s=max_wait_s)
     results = []
     submitted = 0
     arrival_of = {}
@@ -441,8 +441,9 @@ def main():
             batched = serve(ctx, trace, max_active=44)
             log(f"[bench] batched: {batched['mlups']:.0f} MLUPS, p50 {batched['p50_ms']:.0f} ms, "
                 f"p99 {batched['p99_ms']:.0f} ms, makespan {batched['makespan_s']:.1f} s")
-            log("[bench] serving: same trace, batched, shortest-first admission")
-            sjf = serve(ctx, trace, max_active=44, policy=lbx.LBX_SERVE_SHORTEST_FIRST)
+            log("[bench] serving: same trace, batched, shortest-first admission (queued > 2 s first)")
+            sjf = serve(ctx, trace, max_active=44, policy=lbx.LBX_SERVE_SHORTEST_FIRST, max_wait_s=2.0)
+            sjf["max_wait_s"] = 2.0
             log(f"[bench] shortest-first: {sjf['mlups']:.0f} MLUPS, p50 {sjf['p50_ms']:.0f} ms, "
                 f"p99 {sjf['p99_ms']:.0f} ms, makespan {sjf['makespan_s']:.1f} s")
             log("[bench] serving: same trace, one run at a time")

@@ -257,11 +257,14 @@ const char* lbx_serve_last_error(const lbx_server* srv);
 /* Admission policy.  LBX_SERVE_FIFO (default): arrival order, a request that
  * does not fit the arena yet holds back those behind it.
  * LBX_SERVE_SHORTEST_FIRST: queued requests in order of their work
- * (cells x steps, ties by arrival), each admitted if it fits; under sustained
- * overload a large request can wait until the small ones are served. */
+ * (cells x steps, ties by arrival), each admitted if it fits; requests that
+ * have waited longer than max_wait_s seconds since submission come first,
+ * in arrival order (max_wait_s = 0: no limit, and under sustained overload a
+ * large request can wait until the small ones are served).  max_wait_s is
+ * ignored by LBX_SERVE_FIFO. */
 #define LBX_SERVE_FIFO 0
 #define LBX_SERVE_SHORTEST_FIRST 1
-int lbx_serve_set_policy(lbx_server* srv, int policy);
+int lbx_serve_set_policy(lbx_server* srv, int policy, double max_wait_s);
 int lbx_serve_submit(lbx_server* srv, const lbx_request* req, uint64_t* id);
 /* Retire finished runs, admit queued ones, launch up to max_steps batched
  * steps (at most 4 ahead of the GPU). */

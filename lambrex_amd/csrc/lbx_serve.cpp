@@ -121,6 +121,7 @@ struct lbx_server {
   double rho0 = 1.0;
   int max_active = lbx::kServeMaxRuns;
   int policy = LBX_SERVE_FIFO;
+  double max_wait_s = 0.0;  // shortest-first: runs queued longer go first (0 = no limit)
   int grid = 2048;
   Arena arena;
   void* arena_mem = nullptr;
@@ -176,9 +177,16 @@ int admit(lbx_server* s) {
   sweep();
   std::vector<size_t> order;  // queue positions to try
   if (s->policy == LBX_SERVE_SHORTEST_FIRST) {
+    // runs that have waited longer than max_wait_s first, in arrival order;
+    // then the rest by work (cells x steps), ties by arrival
+    const double now = now_s();
+    auto overdue = [&](size_t i) { return s->max_wait_s > 0.0 && now - s->queue[i].t_submit > s->max_wait_s; };
     order.resize(s->queue.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+      const bool oa = overdue(a), ob = overdue(b);
+      if (oa != ob) return oa;
+      if (oa) return false;  // both overdue: keep arrival order
       return cells_of(s->queue[a].req) * s->queue[a].req.steps < cells_of(s->queue[b].req) * s->queue[b].req.steps;
     });
   }
@@ -388,11 +396,14 @@ int lbx_serve_create(lbx_ctx* ctx, int precision, double rho0, size_t arena_byte
   return LBX_OK;
 }
 
-int lbx_serve_set_policy(lbx_server* s, int policy) {
+int lbx_serve_set_policy(lbx_server* s, int policy, double max_wait_s) {
   if (!s) return LBX_E_ARG;
   if (policy != LBX_SERVE_FIFO && policy != LBX_SERVE_SHORTEST_FIRST)
     return sfail(s, LBX_E_ARG, "unknown admission policy %d", policy);
+  if (!(max_wait_s >= 0.0) || !std::isfinite(max_wait_s))
+    return sfail(s, LBX_E_ARG, "max_wait_s must be finite and >= 0");
   s->policy = policy;
+  s->max_wait_s = max_wait_s;
   return LBX_OK;
 }
 

@@ -85,7 +85,7 @@ SIGNATURES.update({
     "lbx_serve_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_int,
                                         ctypes.POINTER(_vp)]),
     "lbx_serve_destroy": (ctypes.c_int, [_vp]),
-    "lbx_serve_set_policy": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lbx_serve_set_policy": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "lbx_serve_last_error": (ctypes.c_char_p, [_vp]),
     "lbx_serve_submit": (ctypes.c_int, [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
     "lbx_serve_progress": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -399,7 +399,7 @@ class Server:
     """Batched serving of independent periodic runs (lbx_serve_*)."""
 
     def __init__(self, ctx, precision=LBX_FP32_SHIFTED, rho0=1.0, arena_bytes=0, max_active=0,
-                 policy=None):
+                 policy=None, max_wait_s=0.0):
         self.ctx = ctx
         self._lib = ctx._lib
         h = ctypes.c_void_p()
@@ -410,11 +410,12 @@ class Server:
         self.h = h
         self._keep = {}  # request id -> buffers that must outlive the run
         if policy is not None:
-            self.set_policy(policy)
+            self.set_policy(policy, max_wait_s)
 
-    def set_policy(self, policy):
-        """LBX_SERVE_FIFO or LBX_SERVE_SHORTEST_FIRST admission."""
-        self._check(self._lib.lbx_serve_set_policy(self.h, int(policy)))
+    def set_policy(self, policy, max_wait_s=0.0):
+        """LBX_SERVE_FIFO or LBX_SERVE_SHORTEST_FIRST admission; with the
+        latter, requests queued longer than max_wait_s seconds go first."""
+        self._check(self._lib.lbx_serve_set_policy(self.h, int(policy), float(max_wait_s)))
 
     def _check(self, rc):
         if rc != LBX_OK:

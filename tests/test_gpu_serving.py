@@ -138,9 +138,27 @@ def test_shortest_first_admission(ctx):
         assert bitwise_equal(a, b)
 
 
+def test_shortest_first_with_wait_limit_serves_overdue_runs_first(ctx):
+    """With a wait limit every queued run has exceeded, shortest-first admits
+    in arrival order (the overdue rule), one run active at a time."""
+    reqs = [((32, 32, 32), 40), ((16, 16, 16), 10), ((24, 24, 24), 20), ((16, 16, 16), 5)]
+    srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=1 << 28, max_active=1,
+                     policy=lbx.LBX_SERVE_SHORTEST_FIRST, max_wait_s=1e-9)
+    ids = [srv.submit(d, n, 0.8, amplitude=1e-3)[0] for d, n in reqs]
+    srv.drain()
+    res = {r["id"]: r for r in srv.poll()}
+    srv.close()
+    admits = [res[i]["t_admit"] for i in ids]
+    assert admits == sorted(admits)
+
+
 def test_unknown_policy_is_refused(ctx):
     srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=16 << 20)
     with pytest.raises(lbx.LbxError) as e:
         srv.set_policy(7)
     assert e.value.code == lbx.LBX_E_ARG
+    for bad in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(lbx.LbxError) as e:
+            srv.set_policy(lbx.LBX_SERVE_SHORTEST_FIRST, bad)
+        assert e.value.code == lbx.LBX_E_ARG
     srv.close()
