@@ -122,6 +122,14 @@ int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint
  * Host only: needs no context or GPU. */
 int lbx_cluster_tags(const int32_t dims[3], const uint8_t* tags, int32_t blocking, int32_t buffer,
                      int32_t max_grid_size, int32_t max_boxes, int32_t* lohi, int32_t* n_boxes);
+/* The same over an index region [0, dims) of any level, with a mask: a tile
+ * that holds a cell with allowed[...] == 0 is not refined (allowed NULL:
+ * every cell allowed, as lbx_cluster_tags).  Callers pass the parent level's
+ * cells whose 3x3x3 neighbourhood lies in the parent, so the new level is
+ * properly nested.  Host only. */
+int lbx_cluster_tags_allowed(const int32_t dims[3], const uint8_t* tags, const uint8_t* allowed,
+                             int32_t blocking, int32_t buffer, int32_t max_grid_size, int32_t max_boxes,
+                             int32_t* lohi, int32_t* n_boxes);
 
 /* Release a level's device storage and that of every finer level
  * (ClearLevel, src/AmrSim.cpp:746-751). */

@@ -9,7 +9,9 @@
 // grown tag, and tagged tiles are merged into boxes -- runs along x, then
 // equal runs along y, then equal rectangles along z.  The boxes do not
 // overlap and cover every grown tag; they are refined by 2 and chopped at
-// max_grid_size with AMReX's BoxList::maxSize rule.
+// max_grid_size with AMReX's BoxList::maxSize rule.  With an `allowed` mask,
+// a tile holding any cell that is not allowed is not refined (proper nesting
+// of a level inside a patched parent).
 #include <algorithm>
 #include <cstdint>
 #include <vector>
@@ -54,8 +56,9 @@ void chop_dir(std::vector<B3>& boxes, int d, int mgs) {
 
 }  // namespace
 
-extern "C" int lbx_cluster_tags(const int32_t dims[3], const uint8_t* tags, int32_t blocking, int32_t buffer,
-                                int32_t max_grid_size, int32_t max_boxes, int32_t* lohi, int32_t* n_boxes) {
+extern "C" int lbx_cluster_tags_allowed(const int32_t dims[3], const uint8_t* tags, const uint8_t* allowed,
+                                        int32_t blocking, int32_t buffer, int32_t max_grid_size, int32_t max_boxes,
+                                        int32_t* lohi, int32_t* n_boxes) {
   if (!dims || !tags || !n_boxes || blocking < 1 || buffer < 0 || max_grid_size < 2 || max_grid_size % 2 ||
       max_boxes < 0 || (max_boxes > 0 && !lohi))
     return LBX_E_ARG;
@@ -78,6 +81,12 @@ extern "C" int lbx_cluster_tags(const int32_t dims[3], const uint8_t* tags, int3
               tile[((size_t)(gz / blocking) * ty + gy / blocking) * tx + gx / blocking] = 1;
             }
       }
+  // a tile holding a cell that is not allowed is never refined
+  if (allowed)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x)
+          if (!allowed[at(x, y, z)]) tile[((size_t)(z / blocking) * ty + y / blocking) * tx + x / blocking] = 0;
   // runs along x, merged along y, then along z
   struct Rect {
     int x0, x1, y0, y1;
@@ -161,4 +170,9 @@ extern "C" int lbx_cluster_tags(const int32_t dims[3], const uint8_t* tags, int3
       lohi[6 * b + 3 + d] = fine[b].hi[d];
     }
   return LBX_OK;
+}
+
+extern "C" int lbx_cluster_tags(const int32_t dims[3], const uint8_t* tags, int32_t blocking, int32_t buffer,
+                                int32_t max_grid_size, int32_t max_boxes, int32_t* lohi, int32_t* n_boxes) {
+  return lbx_cluster_tags_allowed(dims, tags, nullptr, blocking, buffer, max_grid_size, max_boxes, lohi, n_boxes);
 }

@@ -290,38 +290,145 @@ void AmrSim::MakeNewLevelFromCoarse(int level, double time, const std::vector<la
   MakeFineMask(level - 1);
 }
 
-// ErrorEst from the state (lbx_tag_cells) + clustering (lbx_cluster_tags)
-// + the level update of AmrCore::regrid.  No reference counterpart: the
-// reference tags only its static box (TagCell, src/AmrSim.cpp:413-417).
+// ErrorEst from the state + clustering + the level update of
+// AmrCore::regrid, for every level below max_level.  No reference
+// counterpart: the reference tags only its static boxes (TagCell,
+// src/AmrSim.cpp:413-417).
 void AmrSim::Regrid(int criterion, double threshold, int blocking, int buffer) {
   if (!sim_) throw lambrex::Error("Regrid before InitFromScratch");
   if (max_level_ < 1) throw lambrex::Error("Regrid: max_level must be >= 1");
-  std::vector<uint8_t> tags((size_t)NUMEL);
-  Check(lbx_tag_cells(sim_, 0, criterion, threshold, tags.data()));
-  const int32_t dims[3] = {NX, NY, NZ};
+  for (auto& t : static_tags_) t = lambrex::IBox{};
+  for (int l = 0; l < max_level_ && l <= finest_level_; ++l) {
+    const std::vector<lambrex::IBox> fba = RefineFromState(l, criterion, threshold, blocking, buffer);
+    if (fba.empty()) {
+      if (finest_level_ > l) ClearLevel(l + 1);
+      return;
+    }
+    if (finest_level_ < l + 1) {
+      MakeNewLevelFromCoarse(l + 1, GetTime(l), fba);
+    } else if (!(fba == levels_.at(l + 1).boxes)) {
+      if (finest_level_ > l + 1) ClearLevel(l + 2);  // level l+1 is remade as the finest level
+      RemakeLevel(l + 1, GetTime(l), fba);
+    }
+    MakeFineMask(l);
+  }
+}
+
+// Boxes of level `level`+1 from the tags of level `level`.  Level 0 is
+// tagged on the device (lbx_tag_cells) over the whole domain.  A finer level
+// is tagged here from its CalcHydroVars fields with the same arithmetic
+// (central differences, (rho+ - rho-) * 0.5 per axis, or |u|^2), a neighbour
+// outside the level standing in as the cell's own value; only cells whose
+// 3x3x3 neighbourhood lies in the level may be refined (proper nesting).  The
+// level's bounding box is padded by buffer + blocking cells in the directions
+// it does not span, so the clustering's periodic growth cannot wrap across it.
+std::vector<lambrex::IBox> AmrSim::RefineFromState(int level, int criterion, double threshold, int blocking,
+                                                   int buffer) {
+  std::vector<uint8_t> tags, allowed;
+  int32_t ext[3];
+  int org[3] = {0, 0, 0};
+  if (level == 0) {
+    ext[0] = NX;
+    ext[1] = NY;
+    ext[2] = NZ;
+    tags.resize((size_t)NUMEL);
+    Check(lbx_tag_cells(sim_, 0, criterion, threshold, tags.data()));
+  } else {
+    if (criterion != LBX_TAG_DENSITY_GRADIENT && criterion != LBX_TAG_SPEED)
+      throw lambrex::Error("Regrid: unknown tagging criterion");
+    if (!std::isfinite(threshold) || threshold < 0.0) throw lambrex::Error("Regrid: threshold must be finite, >= 0");
+    if (blocking < 1 || buffer < 0) throw lambrex::Error("Regrid: blocking must be >= 1 and buffer >= 0");
+    CalcHydroVars(level);
+    const LevelHost& L = levels_.at(level);
+    const lambrex::IBox B = lambrex::minimal_box(L.boxes);
+    const int dom[3] = {NX << level, NY << level, NZ << level};
+    bool wrap[3];
+    for (int d = 0; d < 3; ++d) {
+      wrap[d] = B.lo[d] == 0 && B.hi[d] == dom[d] - 1;
+      const int pad = wrap[d] ? 0 : buffer + blocking;
+      org[d] = B.lo[d] - pad;
+      ext[d] = B.length(d) + 2 * pad;
+    }
+    const size_t ncell = (size_t)ext[0] * ext[1] * ext[2];
+    std::vector<double> rho(ncell, 0.0), usq(ncell, 0.0);
+    std::vector<uint8_t> valid(ncell, 0);
+    auto idx = [&](int x, int y, int z) { return ((size_t)z * ext[1] + y) * ext[0] + x; };
+    for (size_t b = 0; b < L.boxes.size(); ++b) {
+      const lambrex::IBox& Bx = L.boxes[b];
+      const long long n = Bx.numPts();
+      for (int k = Bx.lo[2]; k <= Bx.hi[2]; ++k)
+        for (int j = Bx.lo[1]; j <= Bx.hi[1]; ++j)
+          for (int i = Bx.lo[0]; i <= Bx.hi[0]; ++i) {
+            const long long li = LocalIndex(Bx, i, j, k);
+            const size_t c = idx(i - org[0], j - org[1], k - org[2]);
+            rho[c] = L.rho[L.cur][b][li];
+            const double u0 = L.vel[b][li], u1 = L.vel[b][n + li], u2 = L.vel[b][2 * n + li];
+            usq[c] = u0 * u0 + u1 * u1 + u2 * u2;
+            valid[c] = 1;
+          }
+    }
+    // neighbour of (x, y, z) along axis d by s, or -1 outside the padded box
+    auto nb = [&](int x, int y, int z, int d, int s) -> long long {
+      int p[3] = {x, y, z};
+      p[d] += s;
+      if (wrap[d])
+        p[d] = (p[d] + ext[d]) % ext[d];
+      else if (p[d] < 0 || p[d] >= ext[d])
+        return -1;
+      return (long long)idx(p[0], p[1], p[2]);
+    };
+    const double thr2 = threshold * threshold;
+    tags.assign(ncell, 0);
+    allowed.assign(ncell, 0);
+    for (int z = 0; z < ext[2]; ++z)
+      for (int y = 0; y < ext[1]; ++y)
+        for (int x = 0; x < ext[0]; ++x) {
+          const size_t c = idx(x, y, z);
+          if (!valid[c]) continue;
+          double s = 0.0;
+          if (criterion == LBX_TAG_DENSITY_GRADIENT) {
+            for (int d = 0; d < 3; ++d) {
+              const long long ip = nb(x, y, z, d, 1), im = nb(x, y, z, d, -1);
+              const double rp = (ip >= 0 && valid[ip]) ? rho[ip] : rho[c];
+              const double rm = (im >= 0 && valid[im]) ? rho[im] : rho[c];
+              const double g = (rp - rm) * 0.5;
+              s = d == 0 ? g * g : s + g * g;
+            }
+          } else {
+            s = usq[c];
+          }
+          tags[c] = s > thr2 ? 1 : 0;
+          bool inner = true;
+          for (int dz = -1; dz <= 1 && inner; ++dz)
+            for (int dy = -1; dy <= 1 && inner; ++dy)
+              for (int dx = -1; dx <= 1 && inner; ++dx) {
+                long long q = (long long)c;
+                const int sh[3] = {dx, dy, dz};
+                int p[3] = {x, y, z};
+                for (int d = 0; d < 3 && q >= 0; ++d) {
+                  q = nb(p[0], p[1], p[2], d, sh[d]);
+                  if (q >= 0) {
+                    p[d] += sh[d];
+                    if (wrap[d]) p[d] = (p[d] + ext[d]) % ext[d];
+                  }
+                }
+                inner = q >= 0 && valid[q];
+              }
+          allowed[c] = inner ? 1 : 0;
+        }
+  }
   int32_t n = 0;
-  Check(lbx_cluster_tags(dims, tags.data(), blocking, buffer, max_grid_size_, 0, nullptr, &n));
+  const uint8_t* al = allowed.empty() ? nullptr : allowed.data();
+  Check(lbx_cluster_tags_allowed(ext, tags.data(), al, blocking, buffer, max_grid_size_, 0, nullptr, &n));
   std::vector<int32_t> lohi(6 * (size_t)std::max(n, 1));
-  if (n > 0) Check(lbx_cluster_tags(dims, tags.data(), blocking, buffer, max_grid_size_, n, lohi.data(), &n));
+  if (n > 0) Check(lbx_cluster_tags_allowed(ext, tags.data(), al, blocking, buffer, max_grid_size_, n, lohi.data(), &n));
   std::vector<lambrex::IBox> fba((size_t)n);
   for (int b = 0; b < n; ++b)
     for (int d = 0; d < NDIMS; ++d) {
-      fba[b].lo[d] = lohi[6 * b + d];
-      fba[b].hi[d] = lohi[6 * b + 3 + d];
+      fba[b].lo[d] = lohi[6 * b + d] + 2 * org[d];
+      fba[b].hi[d] = lohi[6 * b + 3 + d] + 2 * org[d];
     }
-  static_tags_.at(0) = lambrex::IBox{};
-  if (fba.empty()) {
-    if (finest_level_ >= 1) ClearLevel(1);
-    return;
-  }
-  if (finest_level_ < 1) {
-    MakeNewLevelFromCoarse(1, GetTime(0), fba);
-  } else if (!(fba == levels_.at(1).boxes)) {
-    if (finest_level_ > 1) ClearLevel(2);  // level 1 is remade as the finest level
-    RemakeLevel(1, GetTime(0), fba);
-  }
-  MakeFineMask(0);
-  RegridFrom(1);
+  return fba;
 }
 
 namespace {

@@ -94,12 +94,14 @@ class AmrSim {
   // Mask the coarse collide with the fine BoxArray instead of the coarse one
   // the reference passes to makeFineMask (src/AmrSim.cpp:419-428).
   void SetFineMaskFromFineGrids(bool on);
-  // Dynamic regrid on device data: tag the level-0 cells where `criterion`
-  // (LBX_TAG_DENSITY_GRADIENT or LBX_TAG_SPEED) exceeds `threshold`, grow
-  // the tags by `buffer` cells, refine the tiles of `blocking` coarse cells
-  // that hold one, and make, remake (keeping fine data) or clear level 1
-  // like AmrCore::regrid.  Replaces level 0's static refinement box; finer
-  // levels are then rebuilt from their static boxes.
+  // Dynamic regrid on device data, level by level from 0 to max_level-1:
+  // tag the cells where `criterion` (LBX_TAG_DENSITY_GRADIENT or
+  // LBX_TAG_SPEED) exceeds `threshold`, grow the tags by `buffer` cells,
+  // refine the tiles of `blocking` cells that hold one (on levels >= 1 only
+  // tiles whose cells have their whole 3x3x3 neighbourhood in the level, so
+  // the next level is properly nested), and make, remake (keeping fine data)
+  // or clear the next level like AmrCore::regrid.  Replaces every static
+  // refinement box.
   void Regrid(int criterion, double threshold, int blocking = 4, int buffer = 1);
   // Bulk output: density and velocity of every level (CalcHydroVars first)
   // written to `dir` in AMReX's plotfile layout: a HyperCLaw-V1.1 Header,
@@ -145,6 +147,7 @@ class AmrSim {
   // AmrCore::regrid(base): levels base+1 .. max from the static boxes of
   // levels base .. max-1.
   void RegridFrom(int base);
+  std::vector<lambrex::IBox> RefineFromState(int level, int criterion, double threshold, int blocking, int buffer);
   void InitDensity(int level);
   void InitVelocity(int level);
   void Check(int rc) const;

@@ -43,6 +43,8 @@ SIGNATURES = {
     "lbx_tag_cells": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p]),
     "lbx_cluster_tags": (ctypes.c_int, [_i32p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_int32, _i32p, _i32p]),
+    "lbx_cluster_tags_allowed": (ctypes.c_int, [_i32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i32p, _i32p]),
     "lbx_set_relax": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "lbx_set_fine_mask": (ctypes.c_int, [_vp, ctypes.c_int, _i32p]),
     "lbx_set_level_mask": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
@@ -191,22 +193,36 @@ class Context:
         self.close()
 
 
-def cluster_tags(dims, tags, blocking=4, buffer=1, max_grid_size=32):
-    """Fine boxes ((lo), (hi)) covering the tagged cells of a level-0 tag
-    array tags[nz][ny][nx] (lbx_cluster_tags; host only, no GPU)."""
+def cluster_tags(dims, tags, blocking=4, buffer=1, max_grid_size=32, allowed=None):
+    """Boxes ((lo), (hi)) of the next finer level covering the tagged cells of
+    a tag array tags[nz][ny][nx] over [0, dims) (lbx_cluster_tags; with an
+    `allowed` mask of the same shape, lbx_cluster_tags_allowed: tiles holding
+    a disallowed cell are not refined).  Host only, no GPU."""
     lib = load()
     d = np.array([int(x) for x in dims], dtype=np.int32)
     t = np.ascontiguousarray(tags, dtype=np.uint8).reshape(-1)
     if t.size != int(np.prod(d)):
         raise LbxError(LBX_E_ARG, "tag array does not match dims")
+    a = None
+    if allowed is not None:
+        a = np.ascontiguousarray(allowed, dtype=np.uint8).reshape(-1)
+        if a.size != t.size:
+            raise LbxError(LBX_E_ARG, "allowed mask does not match dims")
+
+    def call(max_boxes, out, n):
+        if a is None:
+            return lib.lbx_cluster_tags(_ptr(d, ctypes.c_int32), t.ctypes.data_as(ctypes.c_void_p), int(blocking),
+                                        int(buffer), int(max_grid_size), max_boxes, out, ctypes.byref(n))
+        return lib.lbx_cluster_tags_allowed(_ptr(d, ctypes.c_int32), t.ctypes.data_as(ctypes.c_void_p),
+                                            a.ctypes.data_as(ctypes.c_void_p), int(blocking), int(buffer),
+                                            int(max_grid_size), max_boxes, out, ctypes.byref(n))
+
     n = ctypes.c_int32()
-    rc = lib.lbx_cluster_tags(_ptr(d, ctypes.c_int32), t.ctypes.data_as(ctypes.c_void_p), int(blocking),
-                              int(buffer), int(max_grid_size), 0, None, ctypes.byref(n))
+    rc = call(0, None, n)
     if rc != LBX_OK:
         raise LbxError(rc, "lbx_cluster_tags failed")
     out = np.zeros(6 * max(n.value, 1), dtype=np.int32)
-    rc = lib.lbx_cluster_tags(_ptr(d, ctypes.c_int32), t.ctypes.data_as(ctypes.c_void_p), int(blocking),
-                              int(buffer), int(max_grid_size), n.value, _ptr(out, ctypes.c_int32), ctypes.byref(n))
+    rc = call(n.value, _ptr(out, ctypes.c_int32), n)
     if rc != LBX_OK:
         raise LbxError(rc, "lbx_cluster_tags failed")
     return [(tuple(int(v) for v in out[6 * b:6 * b + 3]), tuple(int(v) for v in out[6 * b + 3:6 * b + 6]))

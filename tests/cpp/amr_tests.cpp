@@ -228,6 +228,48 @@ TEST_CASE("ThreeLevel static refinement") {
   REQUIRE_THROWS(AmrSim(8, 8, 8, LBX_MAX_LEVELS, {1, 1, 1}, 0.8, 0.8));
 }
 
+// Regrid on three levels: level 0's pulse tags make level 1, level 1's own
+// gradient tags (from its fields) make level 2 inside it, properly nested;
+// an unreachable threshold removes both.
+TEST_CASE("ThreeLevel Regrid from a density gradient") {
+  const int nx = 32, ny = 16, nz = 16;
+  AmrSim sim(nx, ny, nz, 2, {1, 1, 1}, 0.8, 0.8);
+  std::vector<double> rho((size_t)nx * ny * nz, 1.0);
+  for (int j = 0; j < ny; ++j)
+    for (int k = 0; k < nz; ++k) rho[((size_t)16 * ny + j) * nz + k] = 1.01;  // plane x = 16, C order
+  sim.SetInitialDensity(rho);
+  sim.SetInitialVelocity(0.0);
+  sim.InitFromScratch(0.0);
+  sim.Regrid(LBX_TAG_DENSITY_GRADIENT, 1e-3);
+  REQUIRE(sim.finestLevel() == 2);
+  auto e1 = sim.GetExtent(1);
+  REQUIRE(e1.first[0] == 24);  // tags at coarse x 15, 17 grown by 1: tiles 12..19
+  REQUIRE(e1.second[0] == 39);
+  REQUIRE(e1.second[1] == 2 * ny - 1);
+  auto e2 = sim.GetExtent(2);
+  REQUIRE(e2.first[0] == 54);  // level-1 tags at x 31..34 grown by 1, tiles of 4 from x 19
+  REQUIRE(e2.second[0] == 77);
+  REQUIRE(e2.first[1] == 0);
+  REQUIRE(e2.second[2] == 4 * nz - 1);
+  sim.CalcHydroVars(1);
+  sim.CalcHydroVars(2);
+  bool parent = true;
+  for (int i = 54; i <= 77; ++i) parent &= sim.GetDensity(i, 9, 5, 2) == sim.GetDensity(i / 2, 4, 2, 1);
+  REQUIRE(parent);
+  sim.CalcHydroVars(0);
+  REQUIRE(sim.GetDensity(64, 0, 0, 2) == sim.GetDensity(16, 0, 0, 0));  // the pulse plane, copied down
+  REQUIRE(sim.GetDensity(64, 0, 0, 2) > sim.GetDensity(60, 0, 0, 2));
+  sim.Iterate(2);
+  REQUIRE(sim.GetTimeStep(2) == 8);
+  sim.CalcHydroVars(2);
+  REQUIRE(std::isfinite(sim.GetDensity(64, 0, 0, 2)));
+  sim.Regrid(LBX_TAG_DENSITY_GRADIENT, 1e9);
+  REQUIRE(sim.finestLevel() == 0);
+  REQUIRE(sim.boxArray(1).empty());
+  REQUIRE(sim.boxArray(2).empty());
+  sim.Iterate(1);
+}
+
 int main() {
   lambrexInit();
   const int rc = check::run_all();

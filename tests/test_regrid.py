@@ -89,3 +89,30 @@ def test_bad_arguments():
     op = out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     if n.value > 1:
         assert lib.lbx_cluster_tags(dp, tp, 4, 1, 32, 1, op, ctypes.byref(n)) == lbx.LBX_E_ARG
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_allowed_mask_keeps_tiles_with_a_disallowed_cell_unrefined(seed):
+    """lbx_cluster_tags_allowed: the refined tiles are exactly those holding a
+    grown tag and no disallowed cell."""
+    rng = np.random.default_rng(100 + seed)
+    dims = (24, 20, 12)
+    nx, ny, nz = dims
+    tags = (rng.random((nz, ny, nx)) < 0.02).astype(np.uint8)
+    allowed = np.ones((nz, ny, nx), np.uint8)
+    allowed[:, :, :5] = 0                                   # a slab
+    allowed[rng.random((nz, ny, nx)) < 0.003] = 0           # scattered cells
+    blocking, buffer = 4, 1
+    boxes = lbx.cluster_tags(dims, tags, blocking=blocking, buffer=buffer, max_grid_size=32, allowed=allowed)
+    want = tiles_expected(tags, blocking, buffer)
+    for k in range(0, nz, blocking):
+        for j in range(0, ny, blocking):
+            for i in range(0, nx, blocking):
+                if not allowed[k:k + blocking, j:j + blocking, i:i + blocking].all():
+                    want[k:k + blocking, j:j + blocking, i:i + blocking] = False
+    cov = coverage(boxes, (nz, ny, nx))
+    assert cov.max() <= 1
+    assert ((cov == 1) == want).all()
+    assert not cov[:, :, :5].any()
+    # with everything allowed it is lbx_cluster_tags
+    assert lbx.cluster_tags(dims, tags, allowed=np.ones_like(tags)) == lbx.cluster_tags(dims, tags)
