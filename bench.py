@@ -325,6 +325,18 @@ def cpu_two_level(steps):
     return lu_per_step * steps / (t1 - t0) / 1e6, t1 - t0
 
 
+def finite_or_none(x):
+    """NaN and infinities (e.g. a kernel average when no fused pass ran)
+    as JSON null, so the output line stays valid JSON."""
+    if isinstance(x, float):
+        return x if np.isfinite(x) else None
+    if isinstance(x, dict):
+        return {k: finite_or_none(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [finite_or_none(v) for v in x]
+    return x
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -558,7 +570,7 @@ def main():
                 "sample": f"{args.cpu_three_level_steps} level-0 step(s) of the same three-level workload "
                           f"({secs3:.1f} s), serial C restatement of the reference's recursive Rohde cycle",
             }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(finite_or_none(out), allow_nan=False), flush=True)
 
 
 if __name__ == "__main__":
