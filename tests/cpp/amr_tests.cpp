@@ -2,6 +2,7 @@
 // (tests/catch2AMRTests.cpp:118-423) against this build's AmrSim facade.
 // Sections that inspect AMReX TagBoxArrays (ErrorEst) have no counterpart
 // here: tagging is reduced to the static refinement box.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -268,6 +269,46 @@ TEST_CASE("ThreeLevel Regrid from a density gradient") {
   REQUIRE(sim.boxArray(1).empty());
   REQUIRE(sim.boxArray(2).empty());
   sim.Iterate(1);
+}
+
+// SetPrecision(LBX_FP32_SHIFTED, 1.0) through the facade: a two-level run
+// tracks the same run in FP64 to 1e-6 in density on both levels, and
+// SetPrecision after InitFromScratch is refused.
+TEST_CASE("FP32-shifted facade run tracks FP64") {
+  const int nx = 24, ny = 16, nz = 12;
+  std::vector<double> rho((size_t)nx * ny * nz), u(3 * (size_t)nx * ny * nz, 0.0);
+  for (size_t c = 0; c < rho.size(); ++c) {
+    rho[c] = 1.0 + 1e-3 * (double)((c * 2654435761u) % 1000) / 1000.0;
+    u[3 * c] = 1e-3 * (double)((c * 40503u) % 100) / 100.0;
+  }
+  double max_err = 0.0;
+  std::vector<double> dens[2][2];  // [precision][level]
+  for (int p = 0; p < 2; ++p) {
+    AmrSim sim(nx, ny, nz, 1, {1, 1, 1}, 0.8, 0.8);
+    if (p) sim.SetPrecision(LBX_FP32_SHIFTED, 1.0);
+    sim.SetInitialDensity(rho);
+    sim.SetInitialVelocity(u);
+    sim.SetMaxGridSize(8);
+    sim.InitFromScratch(0.0);
+    REQUIRE_THROWS(sim.SetPrecision(LBX_FP64, 1.0));
+    sim.SetFineMaskFromFineGrids(true);
+    sim.SetStaticRefinement(0, {6, 0, 0}, {17, ny - 1, nz - 1});
+    sim.Iterate(10);
+    for (int l = 0; l < 2; ++l) {
+      sim.CalcHydroVars(l);
+      auto e = sim.GetExtent(l);
+      for (int i = e.first[0]; i <= e.second[0]; ++i)
+        for (int j = e.first[1]; j <= e.second[1]; ++j)
+          for (int k = e.first[2]; k <= e.second[2]; ++k) dens[p][l].push_back(sim.GetDensity(i, j, k, l));
+    }
+  }
+  for (int l = 0; l < 2; ++l) {
+    REQUIRE(dens[0][l].size() == dens[1][l].size());
+    for (size_t c = 0; c < dens[0][l].size(); ++c)
+      max_err = std::max(max_err, std::fabs(dens[1][l][c] - dens[0][l][c]) / dens[0][l][c]);
+  }
+  REQUIRE(max_err > 0.0);  // the two storages really differ
+  REQUIRE(max_err <= 1e-6);
 }
 
 int main() {
