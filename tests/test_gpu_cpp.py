@@ -201,3 +201,39 @@ def test_three_level_facade_matches_oracle(tmp_path):
                 (lev, lo, hi)
     o.close()
 
+
+def test_ml_pulse_reference_case():
+    """The reference's "ml_pulse Regression" (tests/catch2RegressionTests.cpp:
+    95-196) through the facade.  Its t = 0 assertions pass: both levels,
+    level 1 sampled at 2 (i, j, k), equal the single-level goldens under
+    Catch2's Approx, since a new level is the piecewise-constant copy of its
+    parent.  Its t = 100 assertion on level 0 cannot pass and does not here:
+    the Rohde cycle is not the single-level step (and the reference's own
+    MakeFineMask zeroes the coarse low octant each step), so level 0 leaves
+    the goldens -- the reference test fails the same way, as the oracle,
+    which this build equals bitwise, shows."""
+    rc, out = run("ml_pulse_dump")
+    assert rc == 0, out[-2000:]
+    fx = json.loads(GOLDEN.read_text())
+    eps = 100 * 1.1920928955078125e-07
+    pts = [line.split()[1:] for line in out.splitlines() if line.startswith("P ")]
+    assert len(pts) == 3 * 10 * 10 * 50
+
+    def approx(got, exp):
+        return abs(got - exp) <= eps * abs(exp)
+
+    worst100 = 0.0
+    for t, level, k, r, u0, u1, u2 in pts:
+        t, level, k = int(t), int(level), int(k)
+        exp_r = float(fx["rho"][str(t)][k])
+        exp_u = [float(fx["u"][str(t)][n][k]) for n in range(3)]
+        got_u = [float(u0), float(u1), float(u2)]
+        if t == 0:
+            assert approx(float(r), exp_r), (level, k, r, exp_r)
+            for n in range(3):
+                assert approx(got_u[n], exp_u[n]), (level, k, n, got_u[n], exp_u[n])
+        else:
+            worst100 = max(worst100, abs(float(r) - exp_r) / abs(exp_r))
+    print(f"[ml_pulse] t=0 both levels within Approx; t=100 level 0 max rel rho deviation {worst100:.3e}")
+    assert worst100 > eps  # level 0 left the single-level goldens by t = 100
+
