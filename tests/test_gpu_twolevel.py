@@ -325,3 +325,38 @@ def test_patch_groups_on_two_streams_bitwise(ctx):
         ctx.profile_enable(False)
     assert_bitwise(fo0, eg0, "coarse f, eager")
     assert_bitwise(fo1, eg1, "fine f, eager")
+
+
+def test_fine_regions_meeting_across_the_periodic_x_face(ctx):
+    """Fine regions against both x faces of the domain (coarse x in [0, 4)
+    and [12, 16)): their ghosts come from each other across the periodic
+    face and their average-down wraps onto coarse cells on the far side."""
+    dims = (16, 8, 8)
+    rho, u = workloads.random_state(dims, seed=83, rho_sigma=1e-2, u_amp=0.02)
+    cboxes = workloads.chop(dims, 8)
+    fboxes = fine_boxes_in((0, 0, 0), (7, 15, 15), 8) + fine_boxes_in((24, 4, 2), (31, 11, 13), 4)
+    tau = 0.75
+    for mask in ("reference", "fine"):
+        mboxes = mask_boxes(mask, cboxes, fboxes)
+        o = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in cboxes])
+        o.init(*c_order(rho, u))
+        o.add_fine([(list(a), list(b)) for a, b in fboxes], [(list(a), list(b)) for a, b in mboxes])
+        o.iterate(3)
+        fo0, fo1 = o.get_f(0), o.get_f(1)
+        o.close()
+        sim = lbx.Sim(ctx, dims)
+        try:
+            sim.define_level(0, cboxes)
+            sim.set_tau(0, tau)
+            sim.init_from_fields(rho, u)
+            sim.define_level(1, fboxes)
+            sim.set_tau(1, 2 * (tau - 0.5) + 0.5)
+            sim.fill_from_coarse(1)
+            sim.set_fine_mask(mboxes)
+            sim.iterate(3)
+            fg0 = sim.get_f(0)
+            fg1, _ = sim.get_f_bbox(1)
+        finally:
+            sim.close()
+        assert_bitwise(fo0, fg0, f"coarse f ({mask} mask)")
+        assert_bitwise(fo1, fg1, f"fine f ({mask} mask)")
