@@ -1,0 +1,79 @@
+"""Bitwise parity at the benchmark's own full sizes, one step each: the
+256^3 single-level workload (64 boxes of 64^3) and the two-level workload
+(coarse 128^3 + fine 64x256x256), GPU against the oracle on the same seeded
+inputs.  The oracle takes a few seconds per step at these sizes on the GPU
+host; the smaller parity tests cover many steps."""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from lambrex_amd import lbx, workloads
+from oracle import oracle as orc
+
+ROOT = Path(__file__).resolve().parents[1]
+pytestmark = pytest.mark.gpu
+
+
+def c_order(rho_x, u_x):
+    return (np.ascontiguousarray(rho_x.transpose(2, 1, 0)).reshape(-1),
+            np.ascontiguousarray(u_x.transpose(3, 2, 1, 0)).reshape(-1))
+
+
+def lists(boxes):
+    return [(list(a), list(b)) for a, b in boxes]
+
+
+def same_bits(a, b):
+    return bool((np.ascontiguousarray(a).view(np.uint64) == np.ascontiguousarray(b).view(np.uint64)).all())
+
+
+def test_256_cubed_one_step_bitwise(ctx):
+    dims = (256, 256, 256)
+    boxes = workloads.chop(dims, 64)
+    rho, u = workloads.random_state(dims, seed=2024, rho_sigma=1e-3, u_amp=0.01)
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, boxes)
+        sim.set_tau(0, 0.6)
+        sim.init_from_fields(rho, u)
+        sim.iterate(1)
+        fg = sim.get_f(0)
+    finally:
+        sim.close()
+    o = orc.Sim(*dims, 0.6, boxes=lists(boxes))
+    o.init(*c_order(rho, u))
+    del rho, u
+    o.iterate(1)
+    fo = o.get_f(0)
+    o.close()
+    assert same_bits(fo, fg)
+
+
+def test_two_level_workload_one_coarse_step_bitwise(ctx):
+    sys.path.insert(0, str(ROOT))
+    import bench
+    dims, cboxes, fboxes, mask, rho, u, _ = bench.two_level_geometry()
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, 0.8)
+        sim.init_from_fields(rho, u)
+        sim.define_level(1, fboxes)
+        sim.set_tau(1, 2 * (0.8 - 0.5) + 0.5)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(mask)
+        sim.iterate(1)
+        fg0 = sim.get_f(0)
+        fg1, _ = sim.get_f_bbox(1)
+    finally:
+        sim.close()
+    o = orc.Sim(*dims, 0.8, boxes=lists(cboxes))
+    o.init(*c_order(rho, u))
+    o.add_fine(lists(fboxes), lists(mask))
+    o.iterate(1)
+    fo0, fo1 = o.get_f(0), o.get_f(1)
+    o.close()
+    assert same_bits(fo0, fg0)
+    assert same_bits(fo1, fg1)
