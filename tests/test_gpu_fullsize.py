@@ -1,7 +1,7 @@
 """Bitwise parity at the benchmark's own full sizes, one step each: the
-256^3 single-level workload (64 boxes of 64^3) and the two-level workload
-(coarse 128^3 + fine 64x256x256), GPU against the oracle on the same seeded
-inputs.  The oracle takes a few seconds per step at these sizes on the GPU
+256^3 single-level workload (64 boxes of 64^3), the two-level workload
+(coarse 128^3 + fine 64x256x256) and the three-level one (+ 32x512x512),
+GPU against the oracle on the same seeded inputs.  The oracle takes a few seconds per step at these sizes on the GPU
 host; the smaller parity tests cover many steps."""
 import sys
 from pathlib import Path
@@ -77,3 +77,35 @@ def test_two_level_workload_one_coarse_step_bitwise(ctx):
     o.close()
     assert same_bits(fo0, fg0)
     assert same_bits(fo1, fg1)
+
+
+def test_three_level_workload_one_step_bitwise(ctx):
+    sys.path.insert(0, str(ROOT))
+    import bench
+    dims, levels, masks, rho, u, _ = bench.three_level_geometry()
+    sim = lbx.Sim(ctx, dims)
+    tau = [0.8]
+    for _ in (1, 2):
+        tau.append(2 * (tau[-1] - 0.5) + 0.5)  # ComputeDt, src/AmrSim.cpp:313-314 (1.7000000000000002 on level 2)
+    try:
+        sim.define_level(0, levels[0])
+        sim.set_tau(0, tau[0])
+        sim.init_from_fields(rho, u)
+        for l in (1, 2):
+            sim.define_level(l, levels[l])
+            sim.set_tau(l, tau[l])
+            sim.fill_from_coarse(l)
+            sim.set_level_mask(l - 1, masks[l - 1])
+        sim.iterate(1)
+        fg = [sim.get_f(0)] + [sim.get_f_bbox(l)[0] for l in (1, 2)]
+    finally:
+        sim.close()
+    o = orc.Sim(*dims, tau[0], boxes=lists(levels[0]))
+    o.init(*c_order(rho, u))
+    for l in (1, 2):
+        o.add_fine(lists(levels[l]), lists(masks[l - 1]))
+        assert o.tau(l) == tau[l]
+    o.iterate(1)
+    for l in range(3):
+        assert same_bits(o.get_f(l), fg[l]), f"level {l}"
+    o.close()
