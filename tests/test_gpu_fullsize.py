@@ -109,3 +109,22 @@ def test_three_level_workload_one_step_bitwise(ctx):
     for l in range(3):
         assert same_bits(o.get_f(l), fg[l]), f"level {l}"
     o.close()
+
+
+def test_serving_largest_request_size_bitwise(ctx):
+    """A 128^3 FP64 serving run (the largest size of the serving trace, 8,192
+    work items per pass), batched beside two small runs, equals the oracle."""
+    dims, steps, tau = (128, 128, 128), 3, 0.65
+    rho, u = workloads.random_state(dims, seed=47, rho_sigma=1e-3, u_amp=0.01)
+    srv = lbx.Server(ctx, precision=lbx.LBX_FP64, arena_bytes=1 << 30)
+    _, o = srv.submit(dims, steps, tau, rho=rho, u=u, want_f=True)
+    srv.submit((32, 32, 32), 7, 0.9, amplitude=1e-3)
+    srv.submit((20, 12, 16), 2, 1.2, amplitude=1e-2)
+    srv.drain()
+    srv.poll()
+    srv.close()
+    s = orc.Sim(*dims, tau, boxes=lists(workloads.chop(dims, 64)))
+    s.init(*c_order(rho, u))
+    s.iterate(steps)
+    assert same_bits(o["f"], s.get_f())
+    s.close()
