@@ -66,10 +66,13 @@ struct Level {
   // view of this level's own finer level)
   long long* d_fill = nullptr;
   long long grown_cells = 0;        // length of the fill table
-  long long* d_sum_cell = nullptr;  // parent cell reference per touched cell
+  long long* d_sum_cell = nullptr;  // level-array parent: its element per touched cell
   int* d_sum_start = nullptr;
   int* d_sum_contrib = nullptr;  // index into the block averages, per contribution
   int sum_ncell = 0;
+  // patched parent: touched-cell index per grown cell of the parent (-1 if
+  // none), for the parent's fused stream + sum
+  int* d_sum_slot = nullptr;
   double* d_avg = nullptr;       // 15 planes of avg_stride 2x2x2 block averages
   long long avg_stride = 0;
   int max_blocks = 0;            // largest (g/2)^3 of a patch
@@ -205,6 +208,7 @@ void free_level(Level& L) {
   if (L.d_sum_cell) (void)hipFree(L.d_sum_cell);
   if (L.d_sum_start) (void)hipFree(L.d_sum_start);
   if (L.d_sum_contrib) (void)hipFree(L.d_sum_contrib);
+  if (L.d_sum_slot) (void)hipFree(L.d_sum_slot);
   if (L.d_avg) (void)hipFree(L.d_avg);
   L = Level{};
 }
@@ -474,7 +478,8 @@ hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStrea
 // two-level fill from the parent with comp 0 zeroed on the outer layer and
 // the masked collide into the other buffer; the finer levels' steps, which
 // read m's state; then Stream + ZeroInvalidComponents back into the state
-// buffer and SumFromFine into its valid cells.  The state stays in buf[cur].
+// buffer and SumFromFine into its valid cells (one fused kernel).  The state
+// stays in buf[cur].
 hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipStream_t st) {
   Level& L = sim->lev[m];
   const Level& P = sim->lev[m - 1];
@@ -488,10 +493,7 @@ hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipSt
   if (e == hipSuccess) e = enqueue_children(sim, m, finest, fpar, st);
   if (e == hipSuccess)
     e = lbx::launch_fine_stream_zero(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_desc, 0, np, L.max_cells, sim->rho0,
-                                     st);
-  if (e == hipSuccess)
-    e = lbx::launch_sum_apply(prec, F.d_avg, F.avg_stride, L.buf[L.cur], L.lat, L.d_desc, F.d_sum_cell,
-                              F.d_sum_start, F.d_sum_contrib, F.sum_ncell, sim->rho0, st);
+                                     F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start, F.d_sum_contrib, st);
   return e;
 }
 
@@ -522,7 +524,7 @@ hipError_t enqueue_hierarchy_step(lbx_sim* sim, hipStream_t st) {
   if (e == hipSuccess)
     e = lbx::launch_coarse_push(sim->precision, cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
   if (e == hipSuccess)
-    e = lbx::launch_sum_apply(sim->precision, F.d_avg, F.avg_stride, cnext, C.lat, nullptr, F.d_sum_cell,
+    e = lbx::launch_sum_apply(sim->precision, F.d_avg, F.avg_stride, cnext, C.lat, F.d_sum_cell,
                               F.d_sum_start, F.d_sum_contrib, F.sum_ncell, sim->rho0, st);
   return e;
 }
@@ -848,6 +850,12 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   }
   start.push_back((int)list.size());
   F.sum_ncell = (int)cell.size();
+  std::vector<int> slot;
+  if (C.patched) {
+    slot.assign((size_t)C.grown_cells, -1);
+    for (size_t u = 0; u < cell.size(); ++u)
+      slot[C.desc[cell[u] >> 32].fill_base + (cell[u] & 0xffffffffll)] = (int)u;
+  }
   F.avg_stride = (avg_base + 63) / 64 * 64;
   const size_t bytes = (size_t)base * elem_size(sim->precision);
   {
@@ -870,9 +878,10 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   }
   hipError_t e = upload(&F.d_desc, F.desc, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_fill, fill, ctx->stream);
-  if (e == hipSuccess) e = upload(&F.d_sum_cell, cell, ctx->stream);
+  if (e == hipSuccess && !C.patched) e = upload(&F.d_sum_cell, cell, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_start, start, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_contrib, contrib, ctx->stream);
+  if (e == hipSuccess && C.patched) e = upload(&F.d_sum_slot, slot, ctx->stream);
   if (e == hipSuccess && !C.d_mask) {
     if (C.patched)
       e = upload(&C.d_mask, patch_mask(C, {}), ctx->stream);

@@ -126,10 +126,9 @@ struct PatchDesc {
 // Level-hierarchy kernels (lbx_twolevel.hip); precision 0 = FP64 storage,
 // else FP32-shifted (rho0 in Relax or the argument).  The patch kernels run
 // over patches [first, first + npatch).  A coarse level is either a level
-// array (clat, cpatches / tpatches null) or patched (cpatches / tpatches its
-// descriptors); coarse references in the tables are then -1 - element or
-// -1 - ((patch << 32) | grown index), and sum targets element or
-// (patch << 32) | grown index.
+// array (clat, cpatches null) or patched (cpatches its descriptors); coarse
+// references in the tables are then -1 - element or
+// -1 - ((patch << 32) | grown index).
 // mask (null = none): per grown cell of the level (fill_base + i), nonzero =
 // the valid cell lies under a finer level's mask and is zeroed, not collided.
 hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
@@ -138,9 +137,13 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
                                     const unsigned char* mask, const Relax& r, hipStream_t st);
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, const PatchDesc* patches,
                                      int first, int npatch, int max_cells, const Relax& r, hipStream_t st);
-// Stream + ZeroInvalidComponents of a middle level (no collide).
+// Stream + ZeroInvalidComponents of a middle level (no collide), with the
+// child's block averages added into valid cells: slot[fill_base + i] is the
+// touched-cell index of grown cell i (-1 if none), start/contrib as in
+// launch_sum_apply.
 hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
-                                   int npatch, int max_cells, double rho0, hipStream_t st);
+                                   int npatch, int max_cells, double rho0, const int* slot, const double* avg,
+                                   long long avg_stride, const int* start, const int* contrib, hipStream_t st);
 // avg holds 15 FP64 planes of avg_stride block averages; max_blocks = max (g/2)^3.
 hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
                                   const PatchDesc* patches, int first, int npatch, int max_blocks, double rho0,
@@ -152,9 +155,12 @@ hipError_t launch_patch_avg(int precision, const void* in, double* avg, long lon
 // z-1 leave the cell's box; bit 7 = masked by the fine level.
 hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, const Lattice& L,
                               const unsigned char* flags, const Relax& r, LaunchShape shape, hipStream_t st);
+// Block averages into a level-array parent: cell[u] is the element of
+// population 0 of touched cell u, its contributions avg indices
+// contrib[start[u] .. start[u+1]).
 hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stride, void* cnext, const Lattice& L,
-                            const PatchDesc* tpatches, const long long* cell, const int* start, const int* contrib,
-                            int ncell, double rho0, hipStream_t st);
+                            const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
+                            hipStream_t st);
 hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, const Lattice& clat,
                                    const PatchDesc* patches, int npatch, int max_cells, double rho0,
                                    hipStream_t st);

@@ -30,9 +30,9 @@
 // these kernels on patched "middle" levels, which have both a parent and a
 // child: the fill-collide takes its coarse values from patches (PC) and
 // zeroes the cells under the child's mask (MASK); k_fine_stream_zero is the
-// middle level's Stream + ZeroInvalidComponents; k_fine_stream_avg without
-// its stream averages a middle level's final state for its parent; and
-// k_sum_apply adds into a patched level's valid cells (PT).
+// middle level's Stream + ZeroInvalidComponents with the child's block
+// averages added into its valid cells; and k_fine_stream_avg without its
+// stream averages a middle level's final state for its parent.
 //
 // Storage type S is double (FP64, bit-exact with the oracle) or float (the
 // shifted deviation d = f - w rho0 of the FP32 path).  Either way the block
@@ -200,18 +200,41 @@ __device__ __forceinline__ void zero_invalid(const PatchDesc& P, int gx, int gy,
 
 // A middle level's Stream + ZeroInvalidComponents (src/AmrSim.cpp:459-463):
 // pull over valid + 1 ghost layer, outer layer zero, then the ghost shell
-// keeps only populations that came from valid cells.
+// keeps only populations that came from valid cells.  Fused with the second
+// half of SumFromFine into this level (src/AmrSim.cpp:464-466, 592-602): a
+// valid cell whose slot (per grown cell, -1 if none) names a touched cell u
+// adds the child's block averages [start[u], start[u+1]) to its streamed
+// value in FP64, in the same order as k_sum_apply.
 template <typename S>
 __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ in, S* __restrict__ out,
                                                           const PatchDesc* __restrict__ patches, int first,
-                                                          double rho0) {
+                                                          double rho0, const int* __restrict__ slot,
+                                                          const double* __restrict__ avg, long long avg_stride,
+                                                          const int* __restrict__ start,
+                                                          const int* __restrict__ contrib) {
   const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
   S f[NV];
-  if (!fine_pull(P, in, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz)) zero_invalid(P, gx, gy, gz, rho0, f);
+  if (fine_pull(P, in, i, gx, gy, gz, rho0, f)) {
+    const int u = slot[P.fill_base + i];
+    if (u >= 0) {
+      double acc[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) acc[q] = to_f64(f[q], q, rho0);
+      for (int j = start[u]; j < start[u + 1]; ++j) {
+        const long long a = contrib[j];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] += avg[q * avg_stride + a];
+      }
+#pragma unroll
+      for (int q = 0; q < NV; ++q) f[q] = from_f64<S>(acc[q], q, rho0);
+    }
+  } else if (!patch_outer(P, gx, gy, gz)) {
+    zero_invalid(P, gx, gy, gz, rho0, f);
+  }
 #pragma unroll
   for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
 }
@@ -313,26 +336,16 @@ __global__ __launch_bounds__(512) void k_coarse_push(const S* __restrict__ cnow,
 // Second half of SumFromFine: thread per touched coarse cell; its
 // contributions [start[u], start[u+1]) (indices into avg, in the order the
 // reference's loops add them) are added in FP64 to the streamed coarse value.
-// A coarse cell is the element of population 0 in a level array, or with PT
-// (coarse patch << 32) | grown index of a valid cell of a patched level.
-template <typename S, bool PT>
+// A coarse cell is the element of population 0 in the level array.
+template <typename S>
 __global__ __launch_bounds__(256) void k_sum_apply(const double* __restrict__ avg, long long avg_stride,
                                                    S* __restrict__ cnext, Lattice L,
-                                                   const PatchDesc* __restrict__ tpatches,
                                                    const long long* __restrict__ cell,
                                                    const int* __restrict__ start,
                                                    const int* __restrict__ contrib, int ncell, double rho0) {
   const int u = blockIdx.x * 256 + threadIdx.x;
   if (u >= ncell) return;
-  long long c, stride;
-  if constexpr (PT) {
-    const PatchDesc& T = tpatches[cell[u] >> 32];
-    c = T.base + (cell[u] & 0xffffffffll);
-    stride = T.qs;
-  } else {
-    c = cell[u];  // coarse element of population 0
-    stride = L.ps;
-  }
+  const long long c = cell[u], stride = L.ps;
   double acc[NV];
 #pragma unroll
   for (int q = 0; q < NV; ++q) acc[q] = to_f64(cnext[c + q * stride], q, rho0);
@@ -485,12 +498,15 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
 }
 
 hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
-                                   int npatch, int max_cells, double rho0, hipStream_t st) {
+                                   int npatch, int max_cells, double rho0, const int* slot, const double* avg,
+                                   long long avg_stride, const int* start, const int* contrib, hipStream_t st) {
   const dim3 grid((max_cells + 255) / 256, npatch);
   if (precision == 0)
-    k_fine_stream_zero<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, rho0);
+    k_fine_stream_zero<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, rho0, slot,
+                                                     avg, avg_stride, start, contrib);
   else
-    k_fine_stream_zero<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, rho0);
+    k_fine_stream_zero<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, rho0, slot, avg,
+                                                    avg_stride, start, contrib);
   return hipGetLastError();
 }
 
@@ -541,22 +557,16 @@ hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, cons
 }
 
 hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stride, void* cnext, const Lattice& L,
-                            const PatchDesc* tpatches, const long long* cell, const int* start, const int* contrib,
-                            int ncell, double rho0, hipStream_t st) {
+                            const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
+                            hipStream_t st) {
   if (ncell <= 0) return hipSuccess;
   const int blocks = (ncell + 255) / 256;
-  if (precision == 0 && tpatches)
-    k_sum_apply<double, true><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, tpatches, cell, start,
-                                                      contrib, ncell, rho0);
-  else if (precision == 0)
-    k_sum_apply<double, false><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, tpatches, cell, start,
-                                                       contrib, ncell, rho0);
-  else if (tpatches)
-    k_sum_apply<float, true><<<blocks, 256, 0, st>>>(avg, avg_stride, (float*)cnext, L, tpatches, cell, start,
-                                                     contrib, ncell, rho0);
+  if (precision == 0)
+    k_sum_apply<double><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, cell, start, contrib, ncell,
+                                                rho0);
   else
-    k_sum_apply<float, false><<<blocks, 256, 0, st>>>(avg, avg_stride, (float*)cnext, L, tpatches, cell, start,
-                                                      contrib, ncell, rho0);
+    k_sum_apply<float><<<blocks, 256, 0, st>>>(avg, avg_stride, (float*)cnext, L, cell, start, contrib, ncell,
+                                               rho0);
   return hipGetLastError();
 }
 
