@@ -37,12 +37,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup, keep_fields=False):
+def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup, keep_fields=False, periodic=(1, 1, 1)):
     """Iterate(steps) after Iterate(warmup), timed; with keep_fields the final
-    density and velocity come back as result["fields"]."""
+    density and velocity come back as result["fields"].  A 0 in periodic puts
+    halfway bounce-back walls on both faces normal to that axis."""
     boxes = workloads.chop(dims, mgs)
     fields = None
-    sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
+    sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0, periodic=periodic)
     try:
         sim.define_level(0, boxes)
         sim.set_tau(0, tau)
@@ -373,6 +374,8 @@ def main():
                     help="level-0 steps of the three-level measurement (0 skips it)")
     ap.add_argument("--cpu-three-level-steps", type=int, default=1,
                     help="level-0 steps of the three-level CPU baseline (0 skips it)")
+    ap.add_argument("--walls-steps", type=int, default=200,
+                    help="steps of the FP64 measurement with bounce-back walls on both z faces (0 skips it)")
     args = ap.parse_args()
 
     if args.copy_only:
@@ -416,6 +419,22 @@ def main():
         log(f"[bench] FP64: {r64['mlups']:.0f} MLUPS, kernel {r64['kernel_avg_ms']:.4f} ms "
             f"({r64['kernel_gbs']:.0f} GB/s)")
         shape64 = ctx.tuned_shape(n, n, lbx.LBX_FP64)[0]
+        walls = None
+        if args.walls_steps:
+            log(f"[bench] FP64 {n}^3 with bounce-back walls on both z faces: {args.walls_steps} steps")
+            rw = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP64, args.walls_steps, 5,
+                         periodic=(1, 1, 0))
+            rw.pop("fields")
+            walls = {"workload": f"{n}^3, {rw['boxes']} boxes of {args.max_grid_size}^3, tau {tau}, halfway "
+                                 "bounce-back walls on both z faces (x, y periodic)",
+                     "periodic": [1, 1, 0], "steps": args.walls_steps, "mlups": rw["mlups"],
+                     "ms_per_step": rw["ms_per_step"], "kernel": "k_step_f64<STEP_FUSED, walls>",
+                     "kernel_avg_ms": rw["kernel_avg_ms"], "kernel_gbs": rw["kernel_gbs"],
+                     "frac_of_hbm_peak": rw["kernel_gbs"] / HBM_PEAK_GBS,
+                     "frac_of_copy_ceiling": rw["kernel_gbs"] / copy_gbs if copy_gbs else None,
+                     "mass_drift_rel": rw["mass_drift_rel"]}
+            log(f"[bench] walls: {rw['mlups']:.0f} MLUPS, kernel {rw['kernel_avg_ms']:.4f} ms "
+                f"({rw['kernel_gbs']:.0f} GB/s), mass drift {rw['mass_drift_rel']:.1e}")
         twolev = None
         if args.two_level_steps:
             log(f"[bench] two-level: {args.two_level_steps} coarse steps")
@@ -536,6 +555,8 @@ def main():
             "max_rel_rho_err_vs_fp64": r32["max_rel_rho_err_vs_fp64"],
             "max_du_over_max_u_vs_fp64": r32["max_du_over_max_u_vs_fp64"],
         }
+    if walls:
+        out["walls"] = walls
     if twolev:
         out["two_level"] = twolev
     if threelev:

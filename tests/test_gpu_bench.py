@@ -20,7 +20,8 @@ def _no_constants(name):
 def test_bench_prints_one_strict_json_line(steps):
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--n", "64", "--steps", str(steps), "--warmup", "0",
                         "--no-cpu", "--no-copy", "--serve-requests", "0", "--two-level-steps", "2",
-                        "--three-level-steps", "0"], capture_output=True, text=True, timeout=300, cwd=ROOT)
+                        "--three-level-steps", "0", "--walls-steps", "3"], capture_output=True, text=True,
+                       timeout=300, cwd=ROOT)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [line for line in r.stdout.splitlines() if line.strip()]
     assert len(lines) == 1
@@ -36,3 +37,6 @@ def test_bench_prints_one_strict_json_line(steps):
     else:
         assert rf["achieved"] > 0 and 0 < rf["frac"] < 1
     assert out["two_level"]["mlups"] > 0
+    w = out["walls"]
+    assert w["periodic"] == [1, 1, 0] and w["mlups"] > 0 and 0 < w["frac_of_hbm_peak"] < 1
+    assert w["mass_drift_rel"] < 1e-12

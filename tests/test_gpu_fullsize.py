@@ -128,3 +128,56 @@ def test_serving_largest_request_size_bitwise(ctx):
     s.iterate(steps)
     assert same_bits(o["f"], s.get_f())
     s.close()
+
+
+def walls_step(o, f, periodic):
+    """One collide + halfway bounce-back stream step of the whole lattice.
+    The oracle's periodic step gives g_q(x) = post_q(x - c_q) (wrapped), so
+    g already holds every population that does not cross a wall; one that
+    would enter x through a wall takes post_opp(q)(x) = g_opp(x - c_q)
+    (wrapped) instead -- index shifts only, so the result stays exact."""
+    _, _, C = orc.tables()
+    opp = [int(np.flatnonzero((C == -C[q]).all(axis=1))[0]) for q in range(15)]
+    o.set_f(f)
+    o.iterate(1)
+    g = o.get_f(0)
+    out = g.copy()
+    n = g.shape[1:][::-1]  # nx, ny, nz
+    for q in range(15):
+        cross = np.zeros(g.shape[1:], dtype=bool)
+        for d in range(3):  # axis d of the [z][y][x] array is 2 - d
+            if periodic[d] or C[q][d] == 0:
+                continue
+            face = [slice(None)] * 3
+            face[2 - d] = 0 if C[q][d] > 0 else n[d] - 1
+            cross[tuple(face)] = True
+        bounced = np.roll(g[opp[q]], shift=(int(C[q][2]), int(C[q][1]), int(C[q][0])), axis=(0, 1, 2))
+        out[q][cross] = bounced[cross]
+    return out
+
+
+@pytest.mark.parametrize("dims,mgs,periodic,steps", [((256, 256, 256), 64, (1, 1, 0), 2),
+                                                     ((96, 80, 72), 32, (0, 0, 0), 3)])
+def test_walls_bitwise(ctx, dims, mgs, periodic, steps):
+    """Bounce-back walls at the walls measurement's own size (256^3, walls on
+    both z faces) and a closed ragged box, through the fused kernel (Iterate
+    with more than one step), against the oracle's collision and the
+    bounce-back stream above, bitwise."""
+    boxes = workloads.chop(dims, mgs)
+    rho, u = workloads.random_state(dims, seed=71, rho_sigma=1e-3, u_amp=0.01)
+    sim = lbx.Sim(ctx, dims, periodic=periodic)
+    try:
+        sim.define_level(0, boxes)
+        sim.set_tau(0, 0.6)
+        sim.init_from_fields(rho, u)
+        f = sim.get_f(0)
+        sim.iterate(steps)
+        fg = sim.get_f(0)
+    finally:
+        sim.close()
+    del rho, u
+    o = orc.Sim(*dims, 0.6, boxes=lists(boxes))
+    for _ in range(steps):
+        f = walls_step(o, f, periodic)
+    o.close()
+    assert same_bits(f, fg)
