@@ -90,6 +90,18 @@ def traffic_for(kernel, workload):
         return None
 
 
+def step_traffic(key, workload):
+    """Measured bytes per level-0 step of a hierarchy (every kernel of the
+    step, FETCH_SIZE + WRITE_SIZE scaled) against its algorithmic 240 B per
+    lattice update, from the committed counter summary, or None."""
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())[key][workload]
+        return {"bytes_per_step": d["bytes_per_step"], "algorithmic_bytes_per_step": d["algorithmic_bytes"],
+                "over_algorithmic": d["traffic_over_algorithmic"], "source": d["source"]}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(dims, mgs, tau, rho, u, steps):
     """The CPU oracle (the reference's algorithm restated in C, serial, same
     FillPatch / collide / FillBoundary / stream structure) on the same
@@ -226,7 +238,9 @@ def two_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
                         f"ratio 2, 2 fine substeps per coarse step, {name}",
             "coarse_steps": steps, "warmup_steps": warmup, "mlups": lu / (t1 - t0) / 1e6,
             "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
-            "graph": "one coarse step captured as a HIP graph"}, fields
+            "graph": "one coarse step captured as a HIP graph",
+            "traffic_per_step": (step_traffic("two_level_step", "128^3 + 64x256x256")
+                                 if precision == lbx.LBX_FP64 else None)}, fields
 
 
 def three_level_geometry():
@@ -281,6 +295,8 @@ def three_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
             "coarse_steps": steps, "warmup_steps": warmup, "mlups": lu / (t1 - t0) / 1e6,
             "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
             "graph": "one level-0 step captured as a HIP graph",
+            "traffic_per_step": (step_traffic("three_level_step", "128^3 + 64x256x256 + 32x512x512")
+                                 if precision == lbx.LBX_FP64 else None),
             "final_rho_range_per_level": [[float(np.min(r)), float(np.max(r))] for r in rhos]}, rhos
 
 

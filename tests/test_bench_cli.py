@@ -42,3 +42,21 @@ def test_hierarchy_workloads_are_nested_and_counted():
         ys = np.arange(lo[1] - 2, hi[1] + 3) // 2 % n1[1]
         xs = np.arange(lo[0] - 2, hi[0] + 3) // 2 % n1[0]
         assert cover[np.ix_(zs, ys, xs)].all(), (lo, hi)
+
+
+def test_hierarchy_traffic_entries_match_the_workloads():
+    """The measured per-step traffic that bench.py reports beside the two- and
+    three-level lines is on file, and its algorithmic bytes are 240 B per
+    lattice update of the same workloads."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    lu2 = bench.two_level_geometry()[-1]
+    lu3 = bench.three_level_geometry()[-1]
+    for key, workload, lu in (("two_level_step", "128^3 + 64x256x256", lu2),
+                              ("three_level_step", "128^3 + 64x256x256 + 32x512x512", lu3)):
+        t = bench.step_traffic(key, workload)
+        assert t is not None, key
+        assert t["algorithmic_bytes_per_step"] == 240 * lu
+        assert 1.0 < t["over_algorithmic"] < 5.0
+        assert abs(t["bytes_per_step"] / t["algorithmic_bytes_per_step"] - t["over_algorithmic"]) < 1e-3
+        assert (ROOT / t["source"]).is_file()
