@@ -447,6 +447,19 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
     first += n;
   }
   const bool fork = groups.size() > 1;
+  // The middle array of a group (after the first substep's stream) is dead
+  // once the group ends, so each stream reuses one group-sized region of
+  // buf[2] for it: its lines are then rewritten while still in the MALL
+  // instead of being written back to HBM once per group.  Patches of a group
+  // are contiguous, group k's patch p lands at P.base - base(first) of its
+  // stream's region.  Without room for two regions, buf[2] is used in place.
+  long long span = 0;
+  for (const auto& g : groups) {
+    const PatchDesc& last = F.desc[g.first + g.second - 1];
+    span = std::max(span, last.base + last.qs * lbx::NV_TOTAL - F.desc[g.first].base);
+  }
+  const PatchDesc& tail = F.desc.back();
+  const bool regions = fork && 2 * span <= tail.base + tail.qs * lbx::NV_TOTAL;
   hipError_t e = hipSuccess;
   if (fork) {
     e = hipEventRecord(ctx->fork_ev, st);
@@ -455,13 +468,19 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
   for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
     const int first = groups[k].first, n = groups[k].second;
     hipStream_t gs = (k & 1) ? ctx->side : st;
+    void* mid = ftmp;
+    long long shift = 0;
+    if (regions) {
+      mid = static_cast<char*>(ftmp) + (k & 1) * span * elem_size(prec);
+      shift = F.desc[first].base;
+    }
     e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, cpatches, F.d_desc, first, n, F.max_cells,
                                       F.d_fill, nullptr, rf, gs);
     if (e == hipSuccess)
-      e = lbx::launch_fine_stream_collide(prec, fnext, ftmp, F.d_desc, first, n, F.max_cells, rf, gs);
+      e = lbx::launch_fine_stream_collide(prec, fnext, mid, shift, F.d_desc, first, n, F.max_cells, rf, gs);
     if (e == hipSuccess)
-      e = lbx::launch_fine_stream_avg(prec, ftmp, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n, F.max_blocks,
-                                      sim->rho0, gs);
+      e = lbx::launch_fine_stream_avg(prec, mid, shift, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n,
+                                      F.max_blocks, sim->rho0, gs);
   }
   if (fork) {  // join even after a failed launch, so a capture can end
     const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);

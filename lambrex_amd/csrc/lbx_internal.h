@@ -135,8 +135,11 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
                                     const Lattice& clat, const PatchDesc* cpatches, const PatchDesc* patches,
                                     int first, int npatch, int max_cells, const long long* fill,
                                     const unsigned char* mask, const Relax& r, hipStream_t st);
-hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, const PatchDesc* patches,
-                                     int first, int npatch, int max_cells, const Relax& r, hipStream_t st);
+// out holds each patch at element P.base - oshift (0: a level buffer; the
+// group's first base: a group-sized scratch array).
+hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, long long oshift,
+                                      const PatchDesc* patches, int first, int npatch, int max_cells, const Relax& r,
+                                      hipStream_t st);
 // Stream + ZeroInvalidComponents of a middle level (no collide), with the
 // child's block averages added into valid cells: slot[fill_base + i] is the
 // touched-cell index of grown cell i (-1 if none), start/contrib as in
@@ -144,10 +147,11 @@ hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, 
 hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
                                    int npatch, int max_cells, double rho0, const int* slot, const double* avg,
                                    long long avg_stride, const int* start, const int* contrib, hipStream_t st);
-// avg holds 15 FP64 planes of avg_stride block averages; max_blocks = max (g/2)^3.
-hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
-                                  const PatchDesc* patches, int first, int npatch, int max_blocks, double rho0,
-                                  hipStream_t st);
+// avg holds 15 FP64 planes of avg_stride block averages; max_blocks = max (g/2)^3;
+// in holds each patch at element P.base - ishift.
+hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
+                                  long long avg_stride, const PatchDesc* patches, int first, int npatch,
+                                  int max_blocks, double rho0, hipStream_t st);
 // The same block averages of a level's state as it stands (all patches).
 hipError_t launch_patch_avg(int precision, const void* in, double* avg, long long avg_stride,
                             const PatchDesc* patches, int npatch, int max_blocks, double rho0, hipStream_t st);

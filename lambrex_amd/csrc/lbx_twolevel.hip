@@ -155,11 +155,12 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
   for (int q = 0; q < NV; ++q) fnext[P.base + q * P.qs + i] = f[q];
 }
 
-// Pull of grown cell i over valid + 1 ghost layer; the outer ghost layer of
-// the fresh array is defined as zero.  Returns whether i is a valid cell.
+// Pull of grown cell i over valid + 1 ghost layer, from the patch stored at
+// element `base` of `in`; the outer ghost layer of the fresh array is
+// defined as zero.  Returns whether i is a valid cell.
 template <typename S>
-__device__ __forceinline__ bool fine_pull(const PatchDesc& P, const S* __restrict__ in, int i, int gx, int gy,
-                                          int gz, double rho0, S* f) {
+__device__ __forceinline__ bool fine_pull(const PatchDesc& P, const S* __restrict__ in, long long base, int i,
+                                          int gx, int gy, int gz, double rho0, S* f) {
   if (patch_outer(P, gx, gy, gz)) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) f[q] = zero_pop<S>(q, rho0);
@@ -167,25 +168,27 @@ __device__ __forceinline__ bool fine_pull(const PatchDesc& P, const S* __restric
   }
   const int sx = 1, sy = P.g[0], sz = P.g[0] * P.g[1];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) f[q] = in[P.base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
+  for (int q = 0; q < NV; ++q) f[q] = in[base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
   return patch_valid(P, gx, gy, gz);
 }
 
 // First fine substep's stream + the second substep's collision: pull, then
-// collide valid cells.
+// collide valid cells.  `out` holds the patch at element P.base - oshift (a
+// group-sized scratch array when oshift is the group's first base).
 template <typename S>
 __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict__ in, S* __restrict__ out,
                                                              const PatchDesc* __restrict__ patches, int first,
-                                                             Relax r) {
+                                                             Relax r, long long oshift) {
   const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
   S f[NV];
-  if (fine_pull(P, in, i, gx, gy, gz, r.rho0, f)) collide_s(f, r);
+  if (fine_pull(P, in, P.base, i, gx, gy, gz, r.rho0, f)) collide_s(f, r);
+  const long long ob = P.base - oshift;
 #pragma unroll
-  for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
+  for (int q = 0; q < NV; ++q) out[ob + q * P.qs + i] = f[q];
 }
 
 // ZeroInvalidComponents on a ghost cell at absolute (x, y, z): population m
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
   S f[NV];
-  if (fine_pull(P, in, i, gx, gy, gz, rho0, f)) {
+  if (fine_pull(P, in, P.base, i, gx, gy, gz, rho0, f)) {
     const int u = slot[P.fill_base + i];
     if (u >= 0) {
       double acc[NV];
@@ -254,7 +257,7 @@ template <typename S, bool STREAM>
 __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ in, S* __restrict__ out,
                                                          double* __restrict__ avg, long long avg_stride,
                                                          const PatchDesc* __restrict__ patches, int first,
-                                                         double rho0) {
+                                                         double rho0, long long ishift) {
   __shared__ double part[NV][8][kAvgBlocks];
   const PatchDesc P = patches[first + blockIdx.y];
   const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
@@ -270,7 +273,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
     S f[NV];
     if constexpr (STREAM) {
-      if (!fine_pull(P, in, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz))
+      if (!fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz))
         zero_invalid(P, gx, gy, gz, rho0, f);
     } else {
 #pragma unroll
@@ -515,33 +518,34 @@ hipError_t launch_patch_avg(int precision, const void* in, double* avg, long lon
   const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
   if (precision == 0)
     k_fine_stream_avg<double, false><<<grid, 256, 0, st>>>((const double*)in, nullptr, avg, avg_stride, patches, 0,
-                                                           rho0);
+                                                           rho0, 0);
   else
     k_fine_stream_avg<float, false><<<grid, 256, 0, st>>>((const float*)in, nullptr, avg, avg_stride, patches, 0,
-                                                          rho0);
+                                                          rho0, 0);
   return hipGetLastError();
 }
 
-hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, const PatchDesc* patches,
-                                     int first, int npatch, int max_cells, const Relax& r, hipStream_t st) {
+hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, long long oshift,
+                                      const PatchDesc* patches, int first, int npatch, int max_cells, const Relax& r,
+                                      hipStream_t st) {
   const dim3 grid((max_cells + 255) / 256, npatch);
   if (precision == 0)
-    k_fine_stream_collide<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, r);
+    k_fine_stream_collide<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, r, oshift);
   else
-    k_fine_stream_collide<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, r);
+    k_fine_stream_collide<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, r, oshift);
   return hipGetLastError();
 }
 
-hipError_t launch_fine_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
-                                  const PatchDesc* patches, int first, int npatch, int max_blocks, double rho0,
-                                  hipStream_t st) {
+hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
+                                  long long avg_stride, const PatchDesc* patches, int first, int npatch,
+                                  int max_blocks, double rho0, hipStream_t st) {
   const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
   if (precision == 0)
     k_fine_stream_avg<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride,
-                                                          patches, first, rho0);
+                                                          patches, first, rho0, ishift);
   else
     k_fine_stream_avg<float, true><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride, patches,
-                                                         first, rho0);
+                                                         first, rho0, ishift);
   return hipGetLastError();
 }
 
