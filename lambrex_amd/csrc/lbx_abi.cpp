@@ -499,7 +499,7 @@ hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStrea
 // read m's state; then Stream + ZeroInvalidComponents back into the state
 // buffer and SumFromFine into its valid cells (one fused kernel).  The state
 // stays in buf[cur].
-hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipStream_t st) {
+hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipStream_t st, bool last) {
   Level& L = sim->lev[m];
   const Level& P = sim->lev[m - 1];
   const Level& F = sim->lev[m + 1];
@@ -510,7 +510,11 @@ hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipSt
                                                P.patched ? P.d_desc : nullptr, L.d_desc, 0, np, L.max_cells,
                                                L.d_fill, L.d_mask, r, st);
   if (e == hipSuccess) e = enqueue_children(sim, m, finest, fpar, st);
-  if (e == hipSuccess)
+  if (e == hipSuccess && last)  // the state the parent averages down: its block sums in the same pass
+    e = lbx::launch_middle_stream_avg(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_avg, L.avg_stride, L.d_desc, np,
+                                      L.max_blocks, sim->rho0, F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start,
+                                      F.d_sum_contrib, st);
+  else if (e == hipSuccess)
     e = lbx::launch_fine_stream_zero(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_desc, 0, np, L.max_cells, sim->rho0,
                                      F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start, F.d_sum_contrib, st);
   return e;
@@ -522,12 +526,8 @@ hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipSt
 hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStream_t st) {
   const int c = l + 1;
   if (c == finest) return enqueue_finest_pair(sim, c, fpar, st);
-  Level& M = sim->lev[c];
-  hipError_t e = enqueue_middle_step(sim, c, finest, fpar, st);
-  if (e == hipSuccess) e = enqueue_middle_step(sim, c, finest, fpar, st);
-  if (e == hipSuccess)
-    e = lbx::launch_patch_avg(sim->precision, M.buf[M.cur], M.d_avg, M.avg_stride, M.d_desc, (int)M.boxes.size(),
-                              M.max_blocks, sim->rho0, st);
+  hipError_t e = enqueue_middle_step(sim, c, finest, fpar, st, false);
+  if (e == hipSuccess) e = enqueue_middle_step(sim, c, finest, fpar, st, true);
   return e;
 }
 

@@ -152,9 +152,14 @@ hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, con
 hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
                                   long long avg_stride, const PatchDesc* patches, int first, int npatch,
                                   int max_blocks, double rho0, hipStream_t st);
-// The same block averages of a level's state as it stands (all patches).
-hipError_t launch_patch_avg(int precision, const void* in, double* avg, long long avg_stride,
-                            const PatchDesc* patches, int npatch, int max_blocks, double rho0, hipStream_t st);
+// A middle level's last step of its parent's step, all patches: the stream of
+// launch_fine_stream_zero (the child's block averages cavg added through
+// slot / start / contrib) fused with the level's own block averages of the
+// result into avg.
+hipError_t launch_middle_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
+                                   const PatchDesc* patches, int npatch, int max_blocks, double rho0, const int* slot,
+                                   const double* cavg, long long cavg_stride, const int* start, const int* contrib,
+                                   hipStream_t st);
 // flags: per coarse cell (x fastest), bits 0-5 = x+1, x-1, y+1, y-1, z+1,
 // z-1 leave the cell's box; bit 7 = masked by the fine level.
 hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, const Lattice& L,

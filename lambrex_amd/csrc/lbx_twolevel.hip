@@ -31,8 +31,9 @@
 // child: the fill-collide takes its coarse values from patches (PC) and
 // zeroes the cells under the child's mask (MASK); k_fine_stream_zero is the
 // middle level's Stream + ZeroInvalidComponents with the child's block
-// averages added into its valid cells; and k_fine_stream_avg without its
-// stream averages a middle level's final state for its parent.
+// averages added into its valid cells; and k_fine_stream_avg<S, true> is the
+// same for the middle level's last step of its parent's step, with the block
+// averages of its result for the parent.
 //
 // Storage type S is double (FP64, bit-exact with the oracle) or float (the
 // shifted deviation d = f - w rho0 of the FP32 path).  Either way the block
@@ -201,6 +202,25 @@ __device__ __forceinline__ void zero_invalid(const PatchDesc& P, int gx, int gy,
     if (!in_valid(P, x - 2 * kC[q][0], y - 2 * kC[q][1], z - 2 * kC[q][2])) f[q] = zero_pop<S>(q, rho0);
 }
 
+// SumFromFine's addition into one valid cell of a patched level: the block
+// averages avg[q * avg_stride + contrib[j]], j in [start[u], start[u+1]),
+// added in FP64 in that order to the cell's streamed value.
+template <typename S>
+__device__ __forceinline__ void add_block_sums(S* f, int u, const double* __restrict__ avg, long long avg_stride,
+                                               const int* __restrict__ start, const int* __restrict__ contrib,
+                                               double rho0) {
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = to_f64(f[q], q, rho0);
+  for (int j = start[u]; j < start[u + 1]; ++j) {
+    const long long a = contrib[j];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] += avg[q * avg_stride + a];
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) f[q] = from_f64<S>(acc[q], q, rho0);
+}
+
 // A middle level's Stream + ZeroInvalidComponents (src/AmrSim.cpp:459-463):
 // pull over valid + 1 ghost layer, outer layer zero, then the ghost shell
 // keeps only populations that came from valid cells.  Fused with the second
@@ -223,18 +243,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
   S f[NV];
   if (fine_pull(P, in, P.base, i, gx, gy, gz, rho0, f)) {
     const int u = slot[P.fill_base + i];
-    if (u >= 0) {
-      double acc[NV];
-#pragma unroll
-      for (int q = 0; q < NV; ++q) acc[q] = to_f64(f[q], q, rho0);
-      for (int j = start[u]; j < start[u + 1]; ++j) {
-        const long long a = contrib[j];
-#pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] += avg[q * avg_stride + a];
-      }
-#pragma unroll
-      for (int q = 0; q < NV; ++q) f[q] = from_f64<S>(acc[q], q, rho0);
-    }
+    if (u >= 0) add_block_sums(f, u, avg, avg_stride, start, contrib, rho0);
   } else if (!patch_outer(P, gx, gy, gz)) {
     zero_invalid(P, gx, gy, gz, rho0, f);
   }
@@ -248,16 +257,22 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
 // added in (k, j, i) order in FP64, to avg[q * avg_stride + avg_base + block].
 // A workgroup takes 32 consecutive blocks; thread t computes child t / 32 of
 // block t % 32, so each wavefront reads whole x-pairs of its blocks.  The
-// children meet in LDS for the ordered sums.  Without STREAM the block sums
-// are taken of `in` as it stands (a middle level's state after its steps)
-// and nothing is stored.
+// children meet in LDS for the ordered sums.  With SUM (a middle level's
+// last step of its parent's step) the stream is k_fine_stream_zero's, the
+// child's block averages (slot, cavg, start, contrib) added into valid
+// cells, and the block sums are those of the level's resulting state.
 constexpr int kAvgBlocks = 32;
 
-template <typename S, bool STREAM>
+template <typename S, bool SUM = false>
 __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ in, S* __restrict__ out,
                                                          double* __restrict__ avg, long long avg_stride,
                                                          const PatchDesc* __restrict__ patches, int first,
-                                                         double rho0, long long ishift) {
+                                                         double rho0, long long ishift,
+                                                         const int* __restrict__ slot = nullptr,
+                                                         const double* __restrict__ cavg = nullptr,
+                                                         long long cavg_stride = 0,
+                                                         const int* __restrict__ start = nullptr,
+                                                         const int* __restrict__ contrib = nullptr) {
   __shared__ double part[NV][8][kAvgBlocks];
   const PatchDesc P = patches[first + blockIdx.y];
   const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
@@ -272,16 +287,17 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     const int gx = 2 * cx + di, gy = 2 * cy + dj, gz = 2 * cz + dk;
     const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
     S f[NV];
-    if constexpr (STREAM) {
-      if (!fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f) && !patch_outer(P, gx, gy, gz))
-        zero_invalid(P, gx, gy, gz, rho0, f);
-    } else {
-#pragma unroll
-      for (int q = 0; q < NV; ++q) f[q] = in[P.base + q * P.qs + i];
+    if (fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f)) {
+      if constexpr (SUM) {
+        const int su = slot[P.fill_base + i];
+        if (su >= 0) add_block_sums(f, su, cavg, cavg_stride, start, contrib, rho0);
+      }
+    } else if (!patch_outer(P, gx, gy, gz)) {
+      zero_invalid(P, gx, gy, gz, rho0, f);
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      if constexpr (STREAM) out[P.base + q * P.qs + i] = f[q];
+      out[P.base + q * P.qs + i] = f[q];
       part[q][child][lb] = to_f64(f[q], q, rho0);
     }
   }
@@ -513,15 +529,19 @@ hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, con
   return hipGetLastError();
 }
 
-hipError_t launch_patch_avg(int precision, const void* in, double* avg, long long avg_stride,
-                            const PatchDesc* patches, int npatch, int max_blocks, double rho0, hipStream_t st) {
+hipError_t launch_middle_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
+                                   const PatchDesc* patches, int npatch, int max_blocks, double rho0, const int* slot,
+                                   const double* cavg, long long cavg_stride, const int* start, const int* contrib,
+                                   hipStream_t st) {
   const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
   if (precision == 0)
-    k_fine_stream_avg<double, false><<<grid, 256, 0, st>>>((const double*)in, nullptr, avg, avg_stride, patches, 0,
-                                                           rho0, 0);
+    k_fine_stream_avg<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride,
+                                                                patches, 0, rho0, 0, slot, cavg, cavg_stride, start,
+                                                                contrib);
   else
-    k_fine_stream_avg<float, false><<<grid, 256, 0, st>>>((const float*)in, nullptr, avg, avg_stride, patches, 0,
-                                                          rho0, 0);
+    k_fine_stream_avg<float, true><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride,
+                                                               patches, 0, rho0, 0, slot, cavg, cavg_stride, start,
+                                                               contrib);
   return hipGetLastError();
 }
 
@@ -541,11 +561,11 @@ hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishif
                                   int max_blocks, double rho0, hipStream_t st) {
   const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
   if (precision == 0)
-    k_fine_stream_avg<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride,
-                                                          patches, first, rho0, ishift);
+    k_fine_stream_avg<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride, patches, first,
+                                                    rho0, ishift);
   else
-    k_fine_stream_avg<float, true><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride, patches,
-                                                         first, rho0, ishift);
+    k_fine_stream_avg<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride, patches, first,
+                                                   rho0, ishift);
   return hipGetLastError();
 }
 
