@@ -112,7 +112,10 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
     const S* __restrict__ fnow, S* __restrict__ fnext, const S* __restrict__ cnow, Lattice clat,
     const PatchDesc* __restrict__ cpatches, const PatchDesc* __restrict__ patches, int first,
     const long long* __restrict__ fill, const unsigned char* __restrict__ mask, Relax r) {
-  const PatchDesc P = patches[first + blockIdx.y];
+  // P is read in place: with a by-value copy the PC variants merged the
+  // loads of base / qs from P, patches[s] and cpatches[] into one load through
+  // a selected pointer, which kept the copy in 80 B of scratch per lane.
+  const PatchDesc& P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
