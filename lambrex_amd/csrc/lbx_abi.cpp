@@ -73,6 +73,18 @@ struct Level {
   // patched parent: touched-cell index per grown cell of the parent (-1 if
   // none), for the parent's fused stream + sum
   int* d_sum_slot = nullptr;
+  // Patches are stored in the buffers in `gdesc` order (box corners, x
+  // fastest), whatever order the caller gave the boxes in, and the finest
+  // level's substep pair runs over `groups` of consecutive ones there (about
+  // kFineGroupBytes of one buffer each), so a group is a compact set of
+  // neighbouring boxes.  gdesc holds copies of desc's entries in that order.
+  // With `regions` the array between the two substeps lives in one region of
+  // buf[2] per stream (region_span elements).
+  std::vector<PatchDesc> gdesc;
+  PatchDesc* d_gdesc = nullptr;
+  std::vector<std::pair<int, int>> groups;  // (first position in gdesc, count)
+  long long region_span = 0;
+  bool regions = false;
   double* d_avg = nullptr;       // 15 planes of avg_stride 2x2x2 block averages
   long long avg_stride = 0;
   int max_blocks = 0;            // largest (g/2)^3 of a patch
@@ -209,6 +221,7 @@ void free_level(Level& L) {
   if (L.d_sum_start) (void)hipFree(L.d_sum_start);
   if (L.d_sum_contrib) (void)hipFree(L.d_sum_contrib);
   if (L.d_sum_slot) (void)hipFree(L.d_sum_slot);
+  if (L.d_gdesc) (void)hipFree(L.d_gdesc);
   if (L.d_avg) (void)hipFree(L.d_avg);
   L = Level{};
 }
@@ -424,42 +437,20 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
   void* fnext = F.buf[1 - fpar];
   void* ftmp = F.buf[2];
   const PatchDesc* cpatches = C.patched ? C.d_desc : nullptr;
-  const int np = (int)F.boxes.size();
   // The fine patches' two substeps depend on no other patch's substeps (the
-  // ghosts are filled once, from fnow and cnow), so they run in groups of
-  // patches whose intermediate arrays stay in the MALL.
-  // Groups write disjoint patches (and block sums), so consecutive groups
-  // alternate between the context's stream and its side stream, forked from
-  // and joined back to `st` around the loop; one group's launch ramp then
-  // overlaps the other's tail.
+  // ghosts are filled once, from fnow and cnow), so they run in the level's
+  // groups of neighbouring patches, whose arrays stay in the MALL between
+  // the passes.  Groups write disjoint patches (and block sums), so
+  // consecutive groups alternate between the context's stream and its side
+  // stream, forked from and joined back to `st` around the loop; one group's
+  // launch ramp then overlaps the other's tail.  The array between the
+  // substeps is dead once its group ends: with F.regions each stream
+  // rewrites one region of buf[2] for it (the group's patches are
+  // contiguous, patch p at p.base - base(first)) while its lines are still
+  // in the MALL, instead of writing each group's back to HBM.
   lbx_ctx* ctx = sim->ctx;
-  std::vector<std::pair<int, int>> groups;  // (first patch, count)
-  for (int first = 0; first < np;) {
-    long long bytes = 0;
-    int n = 0;
-    while (first + n < np) {
-      const long long pb = F.desc[first + n].qs * lbx::NV_TOTAL * (long long)elem_size(prec);
-      if (n > 0 && bytes + pb > kFineGroupBytes) break;
-      bytes += pb;
-      ++n;
-    }
-    groups.push_back({first, n});
-    first += n;
-  }
+  const auto& groups = F.groups;
   const bool fork = groups.size() > 1;
-  // The middle array of a group (after the first substep's stream) is dead
-  // once the group ends, so each stream reuses one group-sized region of
-  // buf[2] for it: its lines are then rewritten while still in the MALL
-  // instead of being written back to HBM once per group.  Patches of a group
-  // are contiguous, group k's patch p lands at P.base - base(first) of its
-  // stream's region.  Without room for two regions, buf[2] is used in place.
-  long long span = 0;
-  for (const auto& g : groups) {
-    const PatchDesc& last = F.desc[g.first + g.second - 1];
-    span = std::max(span, last.base + last.qs * lbx::NV_TOTAL - F.desc[g.first].base);
-  }
-  const PatchDesc& tail = F.desc.back();
-  const bool regions = fork && 2 * span <= tail.base + tail.qs * lbx::NV_TOTAL;
   hipError_t e = hipSuccess;
   if (fork) {
     e = hipEventRecord(ctx->fork_ev, st);
@@ -470,16 +461,16 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
     hipStream_t gs = (k & 1) ? ctx->side : st;
     void* mid = ftmp;
     long long shift = 0;
-    if (regions) {
-      mid = static_cast<char*>(ftmp) + (k & 1) * span * elem_size(prec);
-      shift = F.desc[first].base;
+    if (F.regions) {
+      mid = static_cast<char*>(ftmp) + (k & 1) * F.region_span * elem_size(prec);
+      shift = F.gdesc[first].base;
     }
-    e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, cpatches, F.d_desc, first, n, F.max_cells,
-                                      F.d_fill, nullptr, rf, gs);
+    e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, cpatches, F.d_desc, F.d_gdesc, first, n,
+                                      F.max_cells, F.d_fill, nullptr, rf, gs);
     if (e == hipSuccess)
-      e = lbx::launch_fine_stream_collide(prec, fnext, mid, shift, F.d_desc, first, n, F.max_cells, rf, gs);
+      e = lbx::launch_fine_stream_collide(prec, fnext, mid, shift, F.d_gdesc, first, n, F.max_cells, rf, gs);
     if (e == hipSuccess)
-      e = lbx::launch_fine_stream_avg(prec, mid, shift, fnext, F.d_avg, F.avg_stride, F.d_desc, first, n,
+      e = lbx::launch_fine_stream_avg(prec, mid, shift, fnext, F.d_avg, F.avg_stride, F.d_gdesc, first, n,
                                       F.max_blocks, sim->rho0, gs);
   }
   if (fork) {  // join even after a failed launch, so a capture can end
@@ -507,8 +498,8 @@ hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipSt
   const Relax r{L.omega_s, L.omega_b, sim->rho0};
   const int np = (int)L.boxes.size();
   hipError_t e = lbx::launch_fine_fill_collide(prec, L.buf[L.cur], L.buf[1 - L.cur], P.buf[P.cur], P.lat,
-                                               P.patched ? P.d_desc : nullptr, L.d_desc, 0, np, L.max_cells,
-                                               L.d_fill, L.d_mask, r, st);
+                                               P.patched ? P.d_desc : nullptr, L.d_desc, L.d_desc, 0, np,
+                                               L.max_cells, L.d_fill, L.d_mask, r, st);
   if (e == hipSuccess) e = enqueue_children(sim, m, finest, fpar, st);
   if (e == hipSuccess && last)  // the state the parent averages down: its block sums in the same pass
     e = lbx::launch_middle_stream_avg(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_avg, L.avg_stride, L.d_desc, np,
@@ -775,11 +766,9 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
     }
     d.cells = (int)cells;
     d.qs = (cells + 63) / 64 * 64;
-    d.base = base;
     d.fill_base = fill_base;
     d.avg_base = avg_base;
     const int blocks = (d.g[0] / 2) * (d.g[1] / 2) * (d.g[2] / 2);
-    base += d.qs * lbx::NV_TOTAL;
     fill_base += cells;
     avg_base += blocks;
     F.max_cells = std::max(F.max_cells, d.cells);
@@ -791,6 +780,36 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
     return fail(ctx, LBX_E_ARG, "level %d too large", level);
   }
   F.grown_cells = fill_base;
+  // Storage order and patch groups (see Level::gdesc).
+  {
+    std::vector<int> order(boxes.size());
+    for (size_t b = 0; b < boxes.size(); ++b) order[b] = (int)b;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+      const Box& A = boxes[a];
+      const Box& B = boxes[c];
+      return std::make_tuple(A.lo[2], A.lo[1], A.lo[0]) < std::make_tuple(B.lo[2], B.lo[1], B.lo[0]);
+    });
+    for (int b : order) {
+      F.desc[b].base = base;
+      base += F.desc[b].qs * lbx::NV_TOTAL;
+    }
+    for (int b : order) F.gdesc.push_back(F.desc[b]);
+    const long long esz = (long long)elem_size(sim->precision);
+    for (size_t k = 0; k < F.gdesc.size();) {
+      long long elems = 0;
+      size_t n = 0;
+      while (k + n < F.gdesc.size()) {
+        const long long pe = F.gdesc[k + n].qs * lbx::NV_TOTAL;
+        if (n > 0 && (elems + pe) * esz > kFineGroupBytes) break;
+        elems += pe;
+        ++n;
+      }
+      F.groups.push_back({(int)k, (int)n});
+      F.region_span = std::max(F.region_span, elems);
+      k += n;
+    }
+    F.regions = F.groups.size() > 1 && 2 * F.region_span <= base;  // base: elements of one level buffer
+  }
   // Ghost fill sources (FillPatchTwoLevels with piecewise-constant
   // interpolation, src/AmrSim.cpp:373-388): the valid cell of this level
   // holding the periodic image if a box has it, else the coarse parent.
@@ -896,6 +915,7 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
     }
   }
   hipError_t e = upload(&F.d_desc, F.desc, ctx->stream);
+  if (e == hipSuccess) e = upload(&F.d_gdesc, F.gdesc, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_fill, fill, ctx->stream);
   if (e == hipSuccess && !C.patched) e = upload(&F.d_sum_cell, cell, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_start, start, ctx->stream);

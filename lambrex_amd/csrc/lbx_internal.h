@@ -131,10 +131,13 @@ struct PatchDesc {
 // -1 - ((patch << 32) | grown index).
 // mask (null = none): per grown cell of the level (fill_base + i), nonzero =
 // the valid cell lies under a finer level's mask and is zeroed, not collided.
+// own: the level's patches in launch order (patches [first, first + npatch)
+// of it run); patches: the same table by box index, for the fill sources.
 hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
                                     const Lattice& clat, const PatchDesc* cpatches, const PatchDesc* patches,
-                                    int first, int npatch, int max_cells, const long long* fill,
-                                    const unsigned char* mask, const Relax& r, hipStream_t st);
+                                    const PatchDesc* own, int first, int npatch, int max_cells,
+                                    const long long* fill, const unsigned char* mask, const Relax& r,
+                                    hipStream_t st);
 // out holds each patch at element P.base - oshift (0: a level buffer; the
 // group's first base: a group-sized scratch array).
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, long long oshift,

@@ -107,15 +107,19 @@ __device__ __forceinline__ bool in_valid(const PatchDesc& P, int x, int y, int z
 // a level array, or with PC -1 - ((coarse patch << 32) | grown index).  With
 // MASK, a valid cell whose mask byte is set is zeroed instead of collided
 // (CoarseCollide on a level that has a finer one, src/AmrSim.cpp:499-500).
+// The patch is own[first + blockIdx.y] (`own` lists the level's patches in
+// launch order); `patches` is the level's table by box index, which the
+// fill sources refer to.
 template <typename S, bool PC, bool MASK>
 __global__ __launch_bounds__(256) void k_fine_fill_collide(
     const S* __restrict__ fnow, S* __restrict__ fnext, const S* __restrict__ cnow, Lattice clat,
-    const PatchDesc* __restrict__ cpatches, const PatchDesc* __restrict__ patches, int first,
-    const long long* __restrict__ fill, const unsigned char* __restrict__ mask, Relax r) {
+    const PatchDesc* __restrict__ cpatches, const PatchDesc* __restrict__ patches,
+    const PatchDesc* __restrict__ own, int first, const long long* __restrict__ fill,
+    const unsigned char* __restrict__ mask, Relax r) {
   // P is read in place: with a by-value copy the PC variants merged the
   // loads of base / qs from P, patches[s] and cpatches[] into one load through
   // a selected pointer, which kept the copy in 80 B of scratch per lane.
-  const PatchDesc& P = patches[first + blockIdx.y];
+  const PatchDesc& P = own[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
@@ -490,33 +494,34 @@ hipError_t patch_io(int op, S* f, const PatchDesc* patches, int b, int valid_cel
 
 template <typename S>
 hipError_t fill_collide(const S* fnow, S* fnext, const S* cnow, const Lattice& clat, const PatchDesc* cpatches,
-                        const PatchDesc* patches, int first, const long long* fill, const unsigned char* mask,
-                        const Relax& r, dim3 grid, hipStream_t st) {
+                        const PatchDesc* patches, const PatchDesc* own, int first, const long long* fill,
+                        const unsigned char* mask, const Relax& r, dim3 grid, hipStream_t st) {
   if (cpatches && mask)
-    k_fine_fill_collide<S, true, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first, fill,
-                                                             mask, r);
+    k_fine_fill_collide<S, true, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own, first,
+                                                             fill, mask, r);
   else if (cpatches)
-    k_fine_fill_collide<S, true, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first,
-                                                              fill, mask, r);
+    k_fine_fill_collide<S, true, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own,
+                                                              first, fill, mask, r);
   else if (mask)
-    k_fine_fill_collide<S, false, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first,
-                                                              fill, mask, r);
+    k_fine_fill_collide<S, false, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own,
+                                                              first, fill, mask, r);
   else
-    k_fine_fill_collide<S, false, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, first,
-                                                               fill, mask, r);
+    k_fine_fill_collide<S, false, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own,
+                                                               first, fill, mask, r);
   return hipGetLastError();
 }
 
 hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext, const void* cnow,
                                     const Lattice& clat, const PatchDesc* cpatches, const PatchDesc* patches,
-                                    int first, int npatch, int max_cells, const long long* fill,
-                                    const unsigned char* mask, const Relax& r, hipStream_t st) {
+                                    const PatchDesc* own, int first, int npatch, int max_cells,
+                                    const long long* fill, const unsigned char* mask, const Relax& r,
+                                    hipStream_t st) {
   const dim3 grid((max_cells + 255) / 256, npatch);
   if (precision == 0)
-    return fill_collide((const double*)fnow, (double*)fnext, (const double*)cnow, clat, cpatches, patches, first,
-                        fill, mask, r, grid, st);
-  return fill_collide((const float*)fnow, (float*)fnext, (const float*)cnow, clat, cpatches, patches, first, fill,
-                      mask, r, grid, st);
+    return fill_collide((const double*)fnow, (double*)fnext, (const double*)cnow, clat, cpatches, patches, own,
+                        first, fill, mask, r, grid, st);
+  return fill_collide((const float*)fnow, (float*)fnext, (const float*)cnow, clat, cpatches, patches, own, first,
+                      fill, mask, r, grid, st);
 }
 
 hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
