@@ -14,8 +14,14 @@
  *   - stream: a permutation (checked against numpy.roll);
  *   - whole path: the reference's pulse goldens (tests/pulseRegression.h)
  *     under Catch2 Approx rules, and decomposition invariance (bitwise);
- *   - two-level cycle: parity unpinned (no reference test pins it), only
- *     invariants and the single-level limit are checked;
+ *   - two-level cycle: no reference test pins it.  Its pieces are pinned
+ *     one at a time against what AMReX's documented routines and the
+ *     reference's own loops fix (tests/test_oracle_pins_twolevel.py):
+ *     FillPatchTwoLevels + PC interpolation + the outer comp-0 zero on coded
+ *     fields, ZeroInvalidComponents by brute-force enumeration,
+ *     sum_fine_to_coarse by amrex_avgdown's closed form with ADD, and one
+ *     whole cycle on a fully refined domain by the coarse cells' closed form;
+ *     the ADD order of overlapping contributions stays unpinned;
  *   - deeper hierarchies: the recursion is pinned by an exact reduction (a
  *     level 1 covering the whole domain makes levels 1..n evolve bit for bit
  *     like a hierarchy one level shallower whose level 0 is that level 1);
@@ -400,7 +406,21 @@ static void swap_now_next(orc_level* L) {
 /* Two-level pieces (Rohde cycle, src/AmrSim.cpp:430-631)                    */
 /* ------------------------------------------------------------------------ */
 
+/* ORC_MUTANT plants one plausible misreading of the two-level pieces, for the
+ * test that shows each pin catches it (built only by that test, never by the
+ * Makefile):
+ *   1 sum_fine_to_coarse without the 1/8      2 ... over valid children only
+ *   3 ... overwrites instead of ADDing        4 ... children in i,j,k order
+ *   5 ghosts from coarse even with a fine image
+ *   6 parent index by truncation, not floor   7 comp 0 zeroed on both layers
+ *   8 every component zeroed on the outer layer
+ *   9 ZeroInvalid keeps pos + 2c in valid    10 ... keeps pos - c in valid */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 static int floor_div2(int i) { return (i >= 0) ? i / 2 : -((-i + 1) / 2); }
+static int parent_of(int i) { return ORC_MUTANT == 6 ? i / 2 : floor_div2(i); }
 
 /* FillPatchTwoLevels with PCInterp (src/AmrSim.cpp:373-388): every cell of
  * each grown fine box takes the fine valid value of its periodic image when
@@ -414,12 +434,12 @@ static void fill_two_levels(orc_level* C, orc_level* F, double** dst) {
         for (int k = B->lo[2] - ORC_HALO; k <= B->hi[2] + ORC_HALO; ++k) {
           int si, sj, sk;
           const int s = locate(F, i, j, k, &si, &sj, &sk);
-          if (s >= 0) {
+          if (s >= 0 && (ORC_MUTANT != 5 || in_box(B, i, j, k))) {
             for (int n = 0; n < ORC_NV; ++n)
               dst[b][fidx(B, i, j, k, n)] = F->now[s][fidx(&F->vb[s], si, sj, sk, n)];
           } else {
             int ci, cj, ck;
-            const int c = locate(C, floor_div2(i), floor_div2(j), floor_div2(k), &ci, &cj, &ck);
+            const int c = locate(C, parent_of(i), parent_of(j), parent_of(k), &ci, &cj, &ck);
             for (int n = 0; n < ORC_NV; ++n)
               dst[b][fidx(B, i, j, k, n)] = C->now[c][fidx(&C->vb[c], ci, cj, ck, n)];
           }
@@ -447,7 +467,11 @@ static void zero_outer_comp0(orc_level* F, double** f) {
     for (int i = B->lo[0] - ORC_HALO; i <= B->hi[0] + ORC_HALO; ++i)
       for (int j = B->lo[1] - ORC_HALO; j <= B->hi[1] + ORC_HALO; ++j)
         for (int k = B->lo[2] - ORC_HALO; k <= B->hi[2] + ORC_HALO; ++k)
-          if (in_outer_shell(B, i, j, k, 1)) f[b][fidx(B, i, j, k, 0)] = 0.0;
+          if (in_outer_shell(B, i, j, k, ORC_MUTANT == 7 ? 2 : 1)) {
+            f[b][fidx(B, i, j, k, 0)] = 0.0;
+            if (ORC_MUTANT == 8)
+              for (int n = 1; n < ORC_NV; ++n) f[b][fidx(B, i, j, k, n)] = 0.0;
+          }
   }
 }
 
@@ -460,9 +484,11 @@ static void zero_invalid(orc_level* L, double** f) {
       for (int j = B->lo[1] - ORC_HALO; j <= B->hi[1] + ORC_HALO; ++j)
         for (int k = B->lo[2] - ORC_HALO; k <= B->hi[2] + ORC_HALO; ++k) {
           if (!in_outer_shell(B, i, j, k, ORC_HALO)) continue;
-          for (int m = 0; m < ORC_NV; ++m)
-            if (!in_box(B, i - 2 * CV[m][0], j - 2 * CV[m][1], k - 2 * CV[m][2]))
+          for (int m = 0; m < ORC_NV; ++m) {
+            const int a = ORC_MUTANT == 9 ? -2 : ORC_MUTANT == 10 ? 1 : 2;
+            if (!in_box(B, i - a * CV[m][0], j - a * CV[m][1], k - a * CV[m][2]))
               f[b][fidx(B, i, j, k, m)] = 0.0;
+          }
         }
   }
 }
@@ -487,10 +513,17 @@ static void sum_fine_to_coarse(orc_level* F, orc_level* C, double** cdst) {
             double s = 0.0;
             for (int dk = 0; dk < 2; ++dk)
               for (int dj = 0; dj < 2; ++dj)
-                for (int di = 0; di < 2; ++di)
-                  s += F->now[b][fidx(B, 2 * ci + di, 2 * cj + dj, 2 * ck + dk, n)];
-            s *= 0.125;
-            cdst[c][fidx(&C->vb[c], ti, tj, tk, n)] += s;
+                for (int di = 0; di < 2; ++di) {
+                  int ii = 2 * ci + di, jj = 2 * cj + dj, kk = 2 * ck + dk;
+                  if (ORC_MUTANT == 4) { ii = 2 * ci + dk; kk = 2 * ck + di; }
+                  if (ORC_MUTANT == 2 && !in_box(B, ii, jj, kk)) continue;
+                  s += F->now[b][fidx(B, ii, jj, kk, n)];
+                }
+            if (ORC_MUTANT != 1) s *= 0.125;
+            if (ORC_MUTANT == 3)
+              cdst[c][fidx(&C->vb[c], ti, tj, tk, n)] = s;
+            else
+              cdst[c][fidx(&C->vb[c], ti, tj, tk, n)] += s;
           }
         }
   }
@@ -775,4 +808,53 @@ double orc_sim_time(const orc_sim* s, int level) { return s->lev[level].time; }
 int orc_sim_step(const orc_sim* s, int level) { return s->lev[level].step; }
 double orc_sim_tau(const orc_sim* s, int level, int which) {
   return which ? s->lev[level].tau_b : s->lev[level].tau_s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Test hooks: whole FABs and single stages of the cycle                     */
+/* ------------------------------------------------------------------------ */
+
+int orc_sim_nbox(const orc_sim* s, int level) { return s->lev[level].nbox; }
+
+void orc_sim_box(const orc_sim* s, int level, int box, int lohi[6]) {
+  const box3* B = &s->lev[level].vb[box];
+  for (int d = 0; d < 3; ++d) { lohi[d] = B->lo[d]; lohi[3 + d] = B->hi[d]; }
+}
+
+static double** fabs_of(orc_sim* s, int level, int which) {
+  return which ? s->lev[level].next : s->lev[level].now;
+}
+
+void orc_sim_get_grown(const orc_sim* s, int level, int which, int box, double* out) {
+  const box3* B = &s->lev[level].vb[box];
+  memcpy(out, fabs_of((orc_sim*)s, level, which)[box], fab_cells(B) * ORC_NV * sizeof(double));
+}
+
+void orc_sim_set_grown(orc_sim* s, int level, int which, int box, const double* in) {
+  const box3* B = &s->lev[level].vb[box];
+  memcpy(fabs_of(s, level, which)[box], in, fab_cells(B) * ORC_NV * sizeof(double));
+}
+
+int orc_stage(orc_sim* s, int level, int stage) {
+  if (level < 0 || level >= s->nlev) return -1;
+  orc_level* L = &s->lev[level];
+  switch (stage) {
+    case ORC_STAGE_INIT_POST_COLLISION: /* src/AmrSim.cpp:471-485 */
+      if (level == 0) {
+        fill_single_level(L, L->next, L->now, ORC_HALO, 0);
+      } else {
+        fill_two_levels(&s->lev[level - 1], L, L->next);
+        zero_outer_comp0(L, L->next);
+      }
+      return 0;
+    case ORC_STAGE_ZERO_INVALID: /* :604-617 */
+      zero_invalid(L, L->next);
+      return 0;
+    case ORC_STAGE_SUM_FROM_FINE: /* :592-602 */
+      if (level + 1 >= s->nlev) return -1;
+      sum_fine_to_coarse(&s->lev[level + 1], L, L->next);
+      return 0;
+    default:
+      return -1;
+  }
 }

@@ -88,6 +88,24 @@ double orc_sim_time(const orc_sim* s, int level);
 int orc_sim_step(const orc_sim* s, int level);
 double orc_sim_tau(const orc_sim* s, int level, int which /*0=s,1=b*/);
 
+/* Test hooks.  A box's FAB grown by ORC_HALO, [15][gz][gy][gx] x fastest,
+ * of the `now` (which = 0) or `next` (which = 1) state; and single stages of
+ * the Rohde cycle on a level's `next` state, for the pins of the two-level
+ * pieces:
+ *   ORC_STAGE_INIT_POST_COLLISION  InitPostCollision(level), :471-485
+ *   ORC_STAGE_ZERO_INVALID         ZeroInvalidComponents(level), :604-617
+ *   ORC_STAGE_SUM_FROM_FINE        SumFromFine(level) from level+1's now,
+ *                                  :592-602
+ * Returns -1 for an invalid level or stage. */
+#define ORC_STAGE_INIT_POST_COLLISION 0
+#define ORC_STAGE_ZERO_INVALID 1
+#define ORC_STAGE_SUM_FROM_FINE 2
+int orc_sim_nbox(const orc_sim* s, int level);
+void orc_sim_box(const orc_sim* s, int level, int box, int lohi[6]);
+void orc_sim_get_grown(const orc_sim* s, int level, int which, int box, double* out);
+void orc_sim_set_grown(orc_sim* s, int level, int which, int box, const double* in);
+int orc_stage(orc_sim* s, int level, int stage);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,35 +16,56 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
+def load(path):
+    """A ctypes handle on an oracle library at `path` with every entry point
+    typed (the default library, or a variant built by a test)."""
+    L = ctypes.CDLL(str(path))
+    _bind(L)
+    return L
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = _HERE / "liblbx_oracle.so"
         if not path.exists():
             raise RuntimeError(f"{path} is missing; run `make -C oracle` (or __graft_entry__.build())")
-        L = ctypes.CDLL(str(path))
-        L.orc_tables.argtypes = [_dp, _dp, _ip]
-        L.orc_collide_cell.argtypes = [_dp, ctypes.c_double, ctypes.c_double]
-        L.orc_equilibrium_cell.argtypes = [ctypes.c_double, _dp, _dp]
-        L.orc_hydro_cell.argtypes = [_dp, ctypes.POINTER(ctypes.c_double), _dp]
-        L.orc_stream_periodic.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        L.orc_sim_create.restype = ctypes.c_void_p
-        L.orc_sim_create.argtypes = [ctypes.c_int] * 4 + [_ip, ctypes.c_double, ctypes.c_double]
-        L.orc_sim_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_sim_init.argtypes = [ctypes.c_void_p, _dp, _dp]
-        L.orc_sim_add_fine.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int, _ip]
-        L.orc_sim_iterate.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_sim_bbox.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
-        L.orc_sim_get_f.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
-        L.orc_sim_set_f.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
-        L.orc_sim_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
-        L.orc_sim_time.restype = ctypes.c_double
-        L.orc_sim_time.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_sim_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_sim_tau.restype = ctypes.c_double
-        L.orc_sim_tau.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
-        _LIB = L
+        _LIB = load(path)
     return _LIB
+
+
+def _bind(L):
+    L.orc_tables.argtypes = [_dp, _dp, _ip]
+    L.orc_collide_cell.argtypes = [_dp, ctypes.c_double, ctypes.c_double]
+    L.orc_equilibrium_cell.argtypes = [ctypes.c_double, _dp, _dp]
+    L.orc_hydro_cell.argtypes = [_dp, ctypes.POINTER(ctypes.c_double), _dp]
+    L.orc_stream_periodic.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_sim_create.restype = ctypes.c_void_p
+    L.orc_sim_create.argtypes = [ctypes.c_int] * 4 + [_ip, ctypes.c_double, ctypes.c_double]
+    L.orc_sim_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_sim_init.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.orc_sim_add_fine.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int, _ip]
+    L.orc_sim_iterate.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_sim_bbox.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+    L.orc_sim_get_f.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.orc_sim_set_f.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.orc_sim_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.orc_sim_time.restype = ctypes.c_double
+    L.orc_sim_time.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_sim_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_sim_tau.restype = ctypes.c_double
+    L.orc_sim_tau.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.orc_sim_nbox.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_sim_box.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip]
+    L.orc_sim_get_grown.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_sim_set_grown.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_stage.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+
+
+STAGE_INIT_POST_COLLISION = 0
+STAGE_ZERO_INVALID = 1
+STAGE_SUM_FROM_FINE = 2
+HALO = 2
 
 
 def tables():
@@ -117,21 +138,22 @@ def boxes_to_lohi(boxes):
 class Sim:
     """One oracle simulation (level 0 periodic; optional fine level)."""
 
-    def __init__(self, nx, ny, nz, tau_s, tau_b=None, boxes=None, max_grid_size=32):
+    def __init__(self, nx, ny, nz, tau_s, tau_b=None, boxes=None, max_grid_size=32, library=None):
         if tau_b is None:
             tau_b = tau_s
         if boxes is None:
             boxes = max_size_chop((0, 0, 0), (nx - 1, ny - 1, nz - 1), max_grid_size)
         self.dims = (nx, ny, nz)
         self.boxes = boxes
+        self._L = library if library is not None else lib()
         lohi = boxes_to_lohi(boxes)
-        self._h = lib().orc_sim_create(nx, ny, nz, len(boxes), lohi, float(tau_s), float(tau_b))
+        self._h = self._L.orc_sim_create(nx, ny, nz, len(boxes), lohi, float(tau_s), float(tau_b))
         if not self._h:
             raise ValueError("oracle rejected the box layout")
 
     def close(self):
         if self._h:
-            lib().orc_sim_destroy(self._h)
+            self._L.orc_sim_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -139,21 +161,21 @@ class Sim:
 
     def init(self, rho_c, u_c):
         """rho_c: nx*ny*nz values in C order (i slowest); u_c: 3 per cell."""
-        lib().orc_sim_init(self._h, np.ascontiguousarray(rho_c, dtype=np.float64).reshape(-1),
-                           np.ascontiguousarray(u_c, dtype=np.float64).reshape(-1))
+        self._L.orc_sim_init(self._h, np.ascontiguousarray(rho_c, dtype=np.float64).reshape(-1),
+                             np.ascontiguousarray(u_c, dtype=np.float64).reshape(-1))
 
     def add_fine(self, fine_boxes, mask_boxes):
-        rc = lib().orc_sim_add_fine(self._h, len(fine_boxes), boxes_to_lohi(fine_boxes),
-                                    len(mask_boxes), boxes_to_lohi(mask_boxes))
+        rc = self._L.orc_sim_add_fine(self._h, len(fine_boxes), boxes_to_lohi(fine_boxes),
+                                      len(mask_boxes), boxes_to_lohi(mask_boxes))
         if rc != 0:
             raise ValueError(f"orc_sim_add_fine failed ({rc})")
 
     def iterate(self, n):
-        lib().orc_sim_iterate(self._h, int(n))
+        self._L.orc_sim_iterate(self._h, int(n))
 
     def bbox(self, level=0):
         bb = np.zeros(6, dtype=np.int32)
-        lib().orc_sim_bbox(self._h, level, bb)
+        self._L.orc_sim_bbox(self._h, level, bb)
         return bb
 
     def _shape(self, level):
@@ -163,24 +185,53 @@ class Sim:
     def get_f(self, level=0):
         nz, ny, nx = self._shape(level)
         f = np.zeros((15, nz, ny, nx))
-        lib().orc_sim_get_f(self._h, level, f)
+        self._L.orc_sim_get_f(self._h, level, f)
         return f
 
     def set_f(self, f, level=0):
-        lib().orc_sim_set_f(self._h, level, np.ascontiguousarray(f, dtype=np.float64))
+        self._L.orc_sim_set_f(self._h, level, np.ascontiguousarray(f, dtype=np.float64))
 
     def hydro(self, level=0):
         nz, ny, nx = self._shape(level)
         rho = np.zeros((nz, ny, nx))
         u = np.zeros((3, nz, ny, nx))
-        lib().orc_sim_hydro(self._h, level, rho, u)
+        self._L.orc_sim_hydro(self._h, level, rho, u)
         return rho, u
 
     def time(self, level=0):
-        return lib().orc_sim_time(self._h, level)
+        return self._L.orc_sim_time(self._h, level)
 
     def step(self, level=0):
-        return lib().orc_sim_step(self._h, level)
+        return self._L.orc_sim_step(self._h, level)
 
     def tau(self, level=0, which=0):
-        return lib().orc_sim_tau(self._h, level, which)
+        return self._L.orc_sim_tau(self._h, level, which)
+
+    # ---- test hooks: whole FABs (valid + 2 ghost layers) and single stages
+    def nbox(self, level=0):
+        return self._L.orc_sim_nbox(self._h, level)
+
+    def box(self, level, b):
+        lohi = np.zeros(6, dtype=np.int32)
+        self._L.orc_sim_box(self._h, level, b, lohi)
+        return lohi[:3].copy(), lohi[3:].copy()
+
+    def _grown_shape(self, level, b):
+        lo, hi = self.box(level, b)
+        n = hi - lo + 1 + 2 * HALO
+        return (15, int(n[2]), int(n[1]), int(n[0]))
+
+    def get_grown(self, level, b, which=0):
+        """Box b's FAB grown by 2 ghost layers, [15][gz][gy][gx]; which 0 =
+        the `now` state, 1 = `next`."""
+        out = np.zeros(self._grown_shape(level, b))
+        self._L.orc_sim_get_grown(self._h, level, which, b, out)
+        return out
+
+    def set_grown(self, level, b, data, which=0):
+        data = np.ascontiguousarray(data, dtype=np.float64).reshape(self._grown_shape(level, b))
+        self._L.orc_sim_set_grown(self._h, level, which, b, data)
+
+    def stage(self, level, stage):
+        if self._L.orc_stage(self._h, level, stage) != 0:
+            raise ValueError(f"stage {stage} on level {level} rejected")
