@@ -90,7 +90,8 @@ int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
  * level l holds (periodic images included), and of every valid cell, must
  * be a valid cell of level l-1 (LBX_E_ARG otherwise; AMReX builds its grids
  * that way).  Each box is stored with 2 ghost layers like the reference's
- * FABs.  Defining a level releases every finer level. */
+ * FABs.  Defining a level releases every finer level and leaves level l-1
+ * unmasked (lbx_set_level_mask sets its mask). */
 
 /* Replace the boxes of the finest level l >= 1, keeping its data
  * (RemakeLevel, src/AmrSim.cpp:715-744, with DistFnFillPatch, :359-391):
@@ -132,7 +133,8 @@ int lbx_cluster_tags_allowed(const int32_t dims[3], const uint8_t* tags, const u
                              int32_t* lohi, int32_t* n_boxes);
 
 /* Release a level's device storage and that of every finer level
- * (ClearLevel, src/AmrSim.cpp:746-751). */
+ * (ClearLevel, src/AmrSim.cpp:746-751); level-1, now the finest, is left
+ * unmasked. */
 int lbx_level_clear(lbx_sim* sim, int level);
 
 /* Relaxation rates of a level, omega = 1/(tau + 1/2) computed by the caller

@@ -3,5 +3,6 @@
 Modules:
   lbx        -- ctypes binding of include/lbx.h (liblbx.so)
   workloads  -- seeded synthetic initial states for the named configurations
-  build      -- compiles liblbx.so and the C++ AmrSim facade for gfx950
+The HIP library and the C++ AmrSim facade are built by lambrex_amd/Makefile
+(__graft_entry__.build() runs it).
 """

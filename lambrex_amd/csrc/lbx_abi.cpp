@@ -51,6 +51,13 @@ struct Level {
   Lattice lat{};         // level-array geometry (level 0)
   void* buf[3] = {nullptr, nullptr, nullptr};  // state in buf[cur]; buf[2] scratch (patched)
   int cur = 0;
+  // Level 0 of a single-level simulation between Iterate calls: buf[cur]
+  // holds the post-collision state g of the last step, and the reference's
+  // pre-collision populations are f = S(g) (one pull).  Iterate then runs
+  // only fused passes, and readers gather f through the pull (lazy
+  // materialisation); anything else that needs f in place first runs the
+  // stream-only pass (materialise()).  False: buf[cur] holds f itself.
+  bool post = false;
   double omega_s = 0.0, omega_b = 0.0;
   bool relax_set = false;
   // device box/patch descriptors
@@ -104,7 +111,12 @@ struct StepGraphs {
   }
 };
 
-constexpr int kPartials = 1024;  // blocks of the conservation reduction
+// Conservation sums: a level array is reduced by kLatticePartials
+// workgroups; a patched level by patch_chunks(n) workgroups per patch.  The
+// context's scratch holds kPartials partials (4 doubles each) + 4 results.
+constexpr int kLatticePartials = 1024;
+constexpr int kPartials = 65536;
+int patch_chunks(int npatch) { return std::max(1, std::min(64, kLatticePartials / std::max(1, npatch))); }
 
 }  // namespace
 
@@ -117,6 +129,10 @@ struct lbx_ctx {
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   double* scratch = nullptr;  // kPartials*4 partials + 4 results
+  // staging buffer of host transfers (grows on demand, kept until destroy;
+  // every call that uses it synchronises before returning)
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
   // profiling of stream-collide launches
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -256,6 +272,24 @@ hipEvent_t take_event(lbx_ctx* ctx) {
     ctx->ev_pool.push_back(e);
   }
   return ctx->ev_pool[ctx->ev_used++];
+}
+
+// The context's staging buffer, at least `bytes` long.
+hipError_t stage_buffer(lbx_ctx* ctx, size_t bytes, void** out) {
+  if (bytes > ctx->stage_bytes) {
+    if (ctx->stage) {
+      const hipError_t e = hipFree(ctx->stage);
+      ctx->stage = nullptr;
+      ctx->stage_bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    const hipError_t e = hipMalloc(&ctx->stage, want);
+    if (e != hipSuccess) return e;
+    ctx->stage_bytes = want;
+  }
+  *out = ctx->stage;
+  return hipSuccess;
 }
 
 // Fold recorded event pairs into the running total.
@@ -403,6 +437,16 @@ int launch_step(lbx_sim* sim, Level& L, lbx::StepKind kind) {
   if (e != hipSuccess) return hip_fail(ctx, e, "stream-collide launch");
   L.cur = 1 - L.cur;
   return ps.close();
+}
+
+// Bring level 0 back to the pre-collision populations f in place (the
+// stream-only pass) if it holds the post-collision state (Level::post).
+int materialise(lbx_sim* sim) {
+  Level& L = sim->lev[0];
+  if (!L.defined || !L.post) return LBX_OK;
+  if (int rc = launch_step(sim, L, lbx::STEP_STREAM)) return rc;
+  L.post = false;
+  return LBX_OK;
 }
 
 // The finest defined level (levels are defined contiguously from 0).
@@ -921,12 +965,6 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   if (e == hipSuccess) e = upload(&F.d_sum_start, start, ctx->stream);
   if (e == hipSuccess) e = upload(&F.d_sum_contrib, contrib, ctx->stream);
   if (e == hipSuccess && C.patched) e = upload(&F.d_sum_slot, slot, ctx->stream);
-  if (e == hipSuccess && !C.d_mask) {
-    if (C.patched)
-      e = upload(&C.d_mask, patch_mask(C, {}), ctx->stream);
-    else
-      e = upload(&C.d_mask, coarse_flags(C, {}), ctx->stream);
-  }
   if (!C.patched) C.push_shape = lbx::default_shape(C.lat, sim->precision);
   if (e != hipSuccess) {
     free_level(F);
@@ -936,13 +974,28 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   return LBX_OK;
 }
 
-// A level replaces any level at its index and discards the finer ones.
+// Level `level`'s mask bytes with no cell masked (a level starts unmasked;
+// on a level array they still carry the box-face bits of the coarse push).
+// The device buffer is kept if there is one, so captured steps stay valid.
+int unmask_level(lbx_sim* sim, int level) {
+  lbx_ctx* ctx = sim->ctx;
+  Level& C = sim->lev[level];
+  if (!C.defined) return LBX_OK;
+  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, {}) : coarse_flags(C, {});
+  if (!C.d_mask) LBX_HIP(ctx, hipMalloc(&C.d_mask, bytes.size()));
+  LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, bytes.data(), bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LBX_OK;
+}
+
+// A level replaces any level at its index and discards the finer ones; its
+// parent starts unmasked.
 int define_level(lbx_sim* sim, int level, const std::vector<Box>& boxes) {
   Level nf;
   if (int rc = build_level(sim, level, boxes, nf)) return rc;
   free_levels_from(sim, level);
   sim->lev[level] = nf;
-  return LBX_OK;
+  return unmask_level(sim, level - 1);
 }
 
 // RemakeLevel (src/AmrSim.cpp:715-744) of the finest level: a new BoxArray
@@ -1038,6 +1091,36 @@ int fill_from_patched(lbx_sim* sim, int level) {
   return LBX_OK;
 }
 
+// FP64 sums over a level's valid cells (lbx_conservation) with the
+// reduction's partials in `scratch` (room for max_partials partials of 4
+// doubles, then the 4 results): a level array in kLatticePartials
+// workgroups, gathering through the pull when it holds the post-collision
+// state; a patched level in one launch over all patches.  The partial
+// layout depends only on the level, so any scratch gives the same bits.
+int conservation_sums(lbx_sim* sim, int level, double* scratch, long long max_partials, double out[4]) {
+  lbx_ctx* ctx = sim->ctx;
+  Level& L = sim->lev[level];
+  double* res = scratch + 4 * max_partials;
+  if (!L.patched) {
+    if (kLatticePartials > max_partials)
+      return fail(ctx, LBX_E_ARG, "conservation needs %d partial sums, the scratch holds %lld", kLatticePartials,
+                  max_partials);
+    LBX_HIP(ctx, lbx::launch_conservation(sim->precision, L.post, L.buf[L.cur], L.lat, sim->rho0, scratch,
+                                          kLatticePartials, res, ctx->stream));
+  } else {
+    const int np = (int)L.boxes.size();
+    const int chunks = patch_chunks(np);
+    if ((long long)np * chunks > max_partials)
+      return fail(ctx, LBX_E_ARG, "conservation of level %d (%d boxes) needs %lld partial sums, the scratch holds %lld",
+                  level, np, (long long)np * chunks, max_partials);
+    LBX_HIP(ctx, lbx::launch_patch_conservation(sim->precision, L.buf[L.cur], L.d_desc, np, chunks, sim->rho0,
+                                                scratch, res, ctx->stream));
+  }
+  LBX_HIP(ctx, hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LBX_OK;
+}
+
 }  // namespace
 
 int lbx::ctx_device(const lbx_ctx* ctx) { return ctx->device; }
@@ -1086,6 +1169,7 @@ int lbx_ctx_destroy(lbx_ctx* ctx) {
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(ctx->side);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->stage) (void)hipFree(ctx->stage);
   (void)hipEventDestroy(ctx->join_ev);
   (void)hipEventDestroy(ctx->fork_ev);
   (void)hipStreamDestroy(ctx->side);
@@ -1154,7 +1238,9 @@ int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) 
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
   sim->graphs.clear();
-  return level == 0 ? define_level0(sim, boxes) : define_level(sim, level, boxes);
+  if (level == 0) return define_level0(sim, boxes);
+  if (int rc = materialise(sim)) return rc;  // the Rohde cycle keeps level 0 pre-collision
+  return define_level(sim, level, boxes);
 }
 
 int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) {
@@ -1166,6 +1252,7 @@ int lbx_level_remake(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi) 
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const std::vector<Box> boxes = read_boxes(n_boxes, lohi);
   sim->graphs.clear();
+  if (int rc = materialise(sim)) return rc;
   return remake_level(sim, level, boxes);
 }
 
@@ -1181,11 +1268,11 @@ int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint
   if (!L.defined) return fail(ctx, LBX_E_STATE, "level 0 is not defined");
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   const size_t n = (size_t)L.lat.nx * L.lat.ny * L.lat.nz;
-  DeviceTemp t;
-  LBX_HIP(ctx, hipMalloc(&t.p, n));
-  LBX_HIP(ctx, lbx::launch_tag(sim->precision, criterion, L.buf[L.cur], L.lat, sim->rho0, threshold * threshold,
-                               (unsigned char*)t.p, ctx->stream));
-  LBX_HIP(ctx, hipMemcpyAsync(tags, t.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  void* t = nullptr;
+  LBX_HIP(ctx, stage_buffer(ctx, n, &t));
+  LBX_HIP(ctx, lbx::launch_tag(sim->precision, L.post, criterion, L.buf[L.cur], L.lat, sim->rho0,
+                               threshold * threshold, (unsigned char*)t, ctx->stream));
+  LBX_HIP(ctx, hipMemcpyAsync(tags, t, n, hipMemcpyDeviceToHost, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }
@@ -1193,10 +1280,12 @@ int lbx_tag_cells(lbx_sim* sim, int level, int criterion, double threshold, uint
 int lbx_level_clear(lbx_sim* sim, int level) {
   if (int rc = check_sim(sim)) return rc;
   if (level < 0 || level >= LBX_MAX_LEVELS) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
+  LBX_HIP(sim->ctx, hipSetDevice(sim->ctx->device));
   (void)hipStreamSynchronize(sim->ctx->stream);
   sim->graphs.clear();
   free_levels_from(sim, level);
-  return LBX_OK;
+  // the new finest level has no finer level to mask for
+  return level > 0 ? unmask_level(sim, level - 1) : LBX_OK;
 }
 
 int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b) {
@@ -1244,6 +1333,7 @@ int lbx_fill_from_coarse(lbx_sim* sim, int level) {
   Level& F = sim->lev[level];
   if (!C.defined || !F.defined) return fail(ctx, LBX_E_STATE, "levels %d and %d must be defined", level - 1, level);
   LBX_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = materialise(sim)) return rc;
   if (C.patched) return fill_from_patched(sim, level);
   LBX_HIP(ctx, lbx::launch_fine_from_coarse(sim->precision, F.buf[F.cur], C.buf[C.cur], C.lat, F.d_desc,
                                             (int)F.boxes.size(), F.max_cells, sim->rho0, ctx->stream));
@@ -1261,9 +1351,11 @@ int lbx_init_equilibrium(lbx_sim* sim, int level, int box, const double* rho, co
     return fail(ctx, LBX_E_UNSUPPORTED, "level %d is filled from the coarser level", level);
   const Region R = region_of(L.boxes[box]);
   const size_t n = (size_t)L.boxes[box].cells();
-  DeviceTemp t;
-  LBX_HIP(ctx, hipMalloc(&t.p, n * 4 * sizeof(double)));
-  double* drho = (double*)t.p;
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = materialise(sim)) return rc;
+  void* t = nullptr;
+  LBX_HIP(ctx, stage_buffer(ctx, n * 4 * sizeof(double), &t));
+  double* drho = (double*)t;
   double* du = drho + n;
   LBX_HIP(ctx, hipMemcpyAsync(drho, rho, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   LBX_HIP(ctx, hipMemcpyAsync(du, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -1280,16 +1372,18 @@ int lbx_upload_f(lbx_sim* sim, int level, int box, const double* f) {
   lbx_ctx* ctx = sim->ctx;
   Level& L = sim->lev[level];
   const size_t n = (size_t)L.boxes[box].cells();
-  DeviceTemp t;
-  LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
-  LBX_HIP(ctx, hipMemcpyAsync(t.p, f, n * lbx::NV_TOTAL * sizeof(double), hipMemcpyHostToDevice,
-                              ctx->stream));
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  if (level == 0)
+    if (int rc = materialise(sim)) return rc;
+  void* t = nullptr;
+  LBX_HIP(ctx, stage_buffer(ctx, n * lbx::NV_TOTAL * sizeof(double), &t));
+  LBX_HIP(ctx, hipMemcpyAsync(t, f, n * lbx::NV_TOTAL * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   if (L.patched)
-    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 1, L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
+    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 1, L.buf[L.cur], L.d_desc, box, (int)n, (double*)t,
                                       nullptr, sim->rho0, ctx->stream));
   else
     LBX_HIP(ctx, lbx::launch_unpack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
-                                      (const double*)t.p, sim->rho0, ctx->stream));
+                                      (const double*)t, sim->rho0, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }
@@ -1301,16 +1395,16 @@ int lbx_download_f(lbx_sim* sim, int level, int box, double* f) {
   lbx_ctx* ctx = sim->ctx;
   Level& L = sim->lev[level];
   const size_t n = (size_t)L.boxes[box].cells();
-  DeviceTemp t;
-  LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  void* t = nullptr;
+  LBX_HIP(ctx, stage_buffer(ctx, n * lbx::NV_TOTAL * sizeof(double), &t));
   if (L.patched)
-    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 0, L.buf[L.cur], L.d_desc, box, (int)n, (double*)t.p,
+    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 0, L.buf[L.cur], L.d_desc, box, (int)n, (double*)t,
                                       nullptr, sim->rho0, ctx->stream));
   else
-    LBX_HIP(ctx, lbx::launch_pack_f(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
-                                    (double*)t.p, sim->rho0, ctx->stream));
-  LBX_HIP(ctx, hipMemcpyAsync(f, t.p, n * lbx::NV_TOTAL * sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
+    LBX_HIP(ctx, lbx::launch_pack_f(sim->precision, L.post, L.buf[L.cur], L.lat, region_of(L.boxes[box]),
+                                    (double*)t, sim->rho0, ctx->stream));
+  LBX_HIP(ctx, hipMemcpyAsync(f, t, n * lbx::NV_TOTAL * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }
@@ -1327,19 +1421,27 @@ int lbx_iterate(lbx_sim* sim, int nsteps) {
   if (const int finest = finest_level(sim)) {
     for (int l = 1; l <= finest; ++l)
       if (!sim->lev[l].relax_set) return fail(ctx, LBX_E_STATE, "relaxation rates of level %d are not set", l);
+    if (int rc = materialise(sim)) return rc;
     for (int s = 0; s < nsteps; ++s)
       if (int rc = launch_hierarchy_step(sim)) return rc;
     return LBX_OK;
   }
   // Reference order per step is f <- S(C(f)).  Over n steps:
-  //   f_n = S (C S)^(n-1) C f_0
-  // so: one collide-only pass, n-1 fused pull+collide passes, one
-  // stream-only pass.  Per-cell arithmetic is unchanged, so results are
-  // bit-identical to the reference order.
-  if (int rc = launch_step(sim, L, lbx::STEP_COLLIDE)) return rc;
-  for (int s = 1; s < nsteps; ++s)
+  //   f_n = S (C S)^(n-1) C f_0 = S g_n,  g_n = (C S)^(n-1) C f_0
+  // so from f in place: one collide-only pass and n-1 fused pull+collide
+  // passes leave g_n (Level::post); from g: n fused passes.  The last S is
+  // applied only when f is observed (a pull inside the readers, or the
+  // stream-only pass of materialise()).  Per-cell arithmetic is unchanged,
+  // so results are bit-identical to the reference order, and Iterate(a)
+  // then Iterate(b) is Iterate(a + b) pass for pass.
+  int s = 0;
+  if (!L.post) {
+    if (int rc = launch_step(sim, L, lbx::STEP_COLLIDE)) return rc;
+    L.post = true;
+    s = 1;
+  }
+  for (; s < nsteps; ++s)
     if (int rc = launch_step(sim, L, lbx::STEP_FUSED)) return rc;
-  if (int rc = launch_step(sim, L, lbx::STEP_STREAM)) return rc;
   return LBX_OK;
 }
 
@@ -1350,15 +1452,16 @@ int lbx_hydro(lbx_sim* sim, int level, int box, double* rho, double* u) {
   lbx_ctx* ctx = sim->ctx;
   Level& L = sim->lev[level];
   const size_t n = (size_t)L.boxes[box].cells();
-  DeviceTemp t;
-  LBX_HIP(ctx, hipMalloc(&t.p, n * 4 * sizeof(double)));
-  double* drho = (double*)t.p;
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  void* t = nullptr;
+  LBX_HIP(ctx, stage_buffer(ctx, n * 4 * sizeof(double), &t));
+  double* drho = (double*)t;
   double* du = drho + n;
   if (L.patched)
     LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 2, L.buf[L.cur], L.d_desc, box, (int)n, drho, du,
                                       sim->rho0, ctx->stream));
   else
-    LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[box]), drho,
+    LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.post, L.buf[L.cur], L.lat, region_of(L.boxes[box]), drho,
                                    du, sim->rho0, ctx->stream));
   LBX_HIP(ctx, hipMemcpyAsync(rho, drho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LBX_HIP(ctx, hipMemcpyAsync(u, du, 3 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1376,9 +1479,9 @@ int lbx_hydro_level(lbx_sim* sim, int level, double* rho, double* u) {
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   size_t total = 0;
   for (const Box& b : L.boxes) total += (size_t)b.cells();
-  DeviceTemp t;
-  LBX_HIP(ctx, hipMalloc(&t.p, total * 4 * sizeof(double)));
-  double* drho = (double*)t.p;
+  void* t = nullptr;
+  LBX_HIP(ctx, stage_buffer(ctx, total * 4 * sizeof(double), &t));
+  double* drho = (double*)t;
   double* du = drho + total;
   size_t off = 0;
   for (size_t b = 0; b < L.boxes.size(); ++b) {
@@ -1387,8 +1490,8 @@ int lbx_hydro_level(lbx_sim* sim, int level, double* rho, double* u) {
       LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 2, L.buf[L.cur], L.d_desc, (int)b, (int)n, drho + off,
                                         du + 3 * off, sim->rho0, ctx->stream));
     else
-      LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.buf[L.cur], L.lat, region_of(L.boxes[b]), drho + off,
-                                     du + 3 * off, sim->rho0, ctx->stream));
+      LBX_HIP(ctx, lbx::launch_hydro(sim->precision, L.post, L.buf[L.cur], L.lat, region_of(L.boxes[b]),
+                                     drho + off, du + 3 * off, sim->rho0, ctx->stream));
     off += n;
   }
   LBX_HIP(ctx, hipMemcpyAsync(rho, drho, total * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1402,35 +1505,8 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]) {
   if (!out) return fail(sim->ctx, LBX_E_ARG, "null buffer");
   if (level < 0 || level >= LBX_MAX_LEVELS || !sim->lev[level].defined)
     return fail(sim->ctx, LBX_E_STATE, "level %d is not defined", level);
-  lbx_ctx* ctx = sim->ctx;
-  LBX_HIP(ctx, hipSetDevice(ctx->device));
-  Level& L = sim->lev[level];
-  double* partials = ctx->scratch;
-  double* res = ctx->scratch + kPartials * 4;
-  if (!L.patched) {
-    LBX_HIP(ctx, lbx::launch_conservation(sim->precision, L.buf[L.cur], L.lat, sim->rho0, partials,
-                                          kPartials, res, ctx->stream));
-    LBX_HIP(ctx, hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return LBX_OK;
-  }
-  // Fine level: each box's valid cells packed, summed, then boxes in order.
-  for (int c = 0; c < 4; ++c) out[c] = 0.0;
-  for (size_t b = 0; b < L.boxes.size(); ++b) {
-    const long long n = L.boxes[b].cells();
-    DeviceTemp t;
-    LBX_HIP(ctx, hipMalloc(&t.p, n * lbx::NV_TOTAL * sizeof(double)));
-    LBX_HIP(ctx, lbx::launch_patch_io(sim->precision, 0, L.buf[L.cur], L.d_desc, (int)b, (int)n, (double*)t.p,
-                                      nullptr, sim->rho0, ctx->stream));
-    const Lattice packed{(int)n, 1, 1, n, lbx::NV_TOTAL * n, 0u};  // population-major, stride n
-    LBX_HIP(ctx, lbx::launch_conservation(LBX_FP64, t.p, packed, 0.0, partials, kPartials, res,
-                                          ctx->stream));
-    double part[4];
-    LBX_HIP(ctx, hipMemcpyAsync(part, res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int c = 0; c < 4; ++c) out[c] += part[c];
-  }
-  return LBX_OK;
+  LBX_HIP(sim->ctx, hipSetDevice(sim->ctx->device));
+  return conservation_sums(sim, level, sim->ctx->scratch, kPartials, out);
 }
 
 int lbx_profile_enable(lbx_ctx* ctx, int enable) {

@@ -94,20 +94,24 @@ hipError_t launch_step_f32s(StepKind kind, const float* src, float* dst, const L
 // Equilibrium of a region from device rho[nz][ny][nx], u[3][nz][ny][nx].
 hipError_t launch_equilibrium(int precision, void* f, const Lattice& L, const Region& R,
                               const double* rho, const double* u, double rho0, hipStream_t st);
+// Readers of a level array take `pull`: the array holds the post-collision
+// state g of the previous step and the populations are f = S(g), gathered
+// from the upstream cells as the stream-only pass would (see Level::post in
+// lbx_abi.cpp); without it they are read in place.
 // Copy a region between the lattice and a packed FP64 [15][n2][n1][n0] buffer.
-hipError_t launch_pack_f(int precision, const void* f, const Lattice& L, const Region& R,
+hipError_t launch_pack_f(int precision, bool pull, const void* f, const Lattice& L, const Region& R,
                          double* out, double rho0, hipStream_t st);
 hipError_t launch_unpack_f(int precision, void* f, const Lattice& L, const Region& R,
                            const double* in, double rho0, hipStream_t st);
 // Density and velocity of a region into packed FP64 buffers.
-hipError_t launch_hydro(int precision, const void* f, const Lattice& L, const Region& R,
+hipError_t launch_hydro(int precision, bool pull, const void* f, const Lattice& L, const Region& R,
                         double* rho, double* u, double rho0, hipStream_t st);
 // FP64 sums over the whole lattice: partials[nblocks*4] then out[4].
 // tag[(z*ny + y)*nx + x] = 1 where criterion 0 (|grad rho|^2) or 1 (|u|^2)
 // exceeds thr2.
-hipError_t launch_tag(int precision, int criterion, const void* f, const Lattice& L, double rho0, double thr2,
-                      unsigned char* tag, hipStream_t st);
-hipError_t launch_conservation(int precision, const void* f, const Lattice& L, double rho0,
+hipError_t launch_tag(int precision, bool pull, int criterion, const void* f, const Lattice& L, double rho0,
+                      double thr2, unsigned char* tag, hipStream_t st);
+hipError_t launch_conservation(int precision, bool pull, const void* f, const Lattice& L, double rho0,
                                double* partials, int nblocks, double* out, hipStream_t st);
 
 // A box stored on its own with 2 ghost layers (fine level), or a box of a
@@ -122,6 +126,12 @@ struct PatchDesc {
   long long fill_base;  // offset of this patch's rows in the fill table
   long long avg_base;   // offset of this patch's 2x2x2 block averages ((g/2)^3 per population)
 };
+
+// FP64 sums over the valid cells of patches [0, npatch) in one launch:
+// chunks workgroups per patch write partials[4 * (patch * chunks + chunk)],
+// then out[4] = their sum in a fixed order.
+hipError_t launch_patch_conservation(int precision, const void* f, const PatchDesc* patches, int npatch,
+                                     int chunks, double rho0, double* partials, double* out, hipStream_t st);
 
 // Level-hierarchy kernels (lbx_twolevel.hip); precision 0 = FP64 storage,
 // else FP32-shifted (rho0 in Relax or the argument).  The patch kernels run

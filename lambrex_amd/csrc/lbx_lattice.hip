@@ -186,7 +186,33 @@ __global__ __launch_bounds__(256) void k_equilibrium(S* __restrict__ f, Lattice 
   }
 }
 
-template <typename S>
+// FP64 populations of cell (x, y, z).  Without PULL they are stored at the
+// cell.  With PULL the lattice holds the post-collision state g of the
+// previous step and f = S(g) is gathered from the upstream cells exactly as
+// the stream-only pass would (periodic wrap or bounce-back walls), so a
+// reader sees the same bits as after that pass.
+template <typename S, bool PULL>
+__device__ __forceinline__ void cell_at(const S* __restrict__ f, const Lattice& L, int x, int y, int z,
+                                        double rho0, double* g) {
+  const long long selfz = (long long)z * L.zs;
+  const int selfp = y * L.nx + x;
+  S v[NV];
+  if (!PULL)
+    load_cell<S, false>(f, L, selfz, selfp, Upstream{}, v);
+  else if (L.walls)
+    load_cell_walls<S>(f, L, x, y, z, selfz, selfp, upstream(x, y, z, L), v);
+  else
+    load_cell<S, true>(f, L, selfz, selfp, upstream(x, y, z, L), v);
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    if constexpr (sizeof(S) == 8)
+      g[q] = v[q];
+    else
+      g[q] = kT.W[q] * rho0 + double(v[q]);
+  }
+}
+
+template <typename S, bool PULL>
 __global__ __launch_bounds__(256) void k_pack(const S* __restrict__ f, Lattice L, Region R,
                                               double* __restrict__ out, double rho0) {
   const int n = R.n[0] * R.n[1] * R.n[2];
@@ -194,14 +220,10 @@ __global__ __launch_bounds__(256) void k_pack(const S* __restrict__ f, Lattice L
   if (i >= n) return;
   int x, y, z;
   region_cell(R, i, x, y, z);
-  const long long self = L.at(0, y * L.nx + x, z);
+  double g[NV];
+  cell_at<S, PULL>(f, L, x, y, z, rho0, g);
 #pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    if constexpr (sizeof(S) == 8)
-      out[q * (long long)n + i] = f[self + q * L.ps];
-    else
-      out[q * (long long)n + i] = kT.W[q] * rho0 + double(f[self + q * L.ps]);
-  }
+  for (int q = 0; q < NV; ++q) out[q * (long long)n + i] = g[q];
 }
 
 template <typename S>
@@ -222,21 +244,7 @@ __global__ __launch_bounds__(256) void k_unpack(S* __restrict__ f, Lattice L, Re
   }
 }
 
-// FP64 populations of one cell from either storage.
-// self = element of population 0 of the cell; populations are ps apart.
-template <typename S>
-__device__ __forceinline__ void cell_f64(const S* __restrict__ f, long long ps, long long self,
-                                         double rho0, double* g) {
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    if constexpr (sizeof(S) == 8)
-      g[q] = f[self + q * ps];
-    else
-      g[q] = kT.W[q] * rho0 + double(f[self + q * ps]);
-  }
-}
-
-template <typename S>
+template <typename S, bool PULL>
 __global__ __launch_bounds__(256) void k_hydro(const S* __restrict__ f, Lattice L, Region R,
                                                double* __restrict__ rho, double* __restrict__ u,
                                                double rho0) {
@@ -246,7 +254,7 @@ __global__ __launch_bounds__(256) void k_hydro(const S* __restrict__ f, Lattice 
   int x, y, z;
   region_cell(R, i, x, y, z);
   double g[NV];
-  cell_f64<S>(f, L.ps, L.at(0, y * L.nx + x, z), rho0, g);
+  cell_at<S, PULL>(f, L, x, y, z, rho0, g);
   double r, u0, u1, u2;
   hydro_f64(g, r, u0, u1, u2);
   rho[i] = r;
@@ -258,7 +266,7 @@ __global__ __launch_bounds__(256) void k_hydro(const S* __restrict__ f, Lattice 
 // ErrorEst on device data: tag[c] = 1 (c = (z*ny + y)*nx + x) where
 // CRIT 0: |grad rho|^2 > thr2, central differences with periodic wrap;
 // CRIT 1: |u|^2 > thr2.  rho and u are exactly what k_hydro returns.
-template <typename S, int CRIT>
+template <typename S, int CRIT, bool PULL>
 __global__ __launch_bounds__(256) void k_tag(const S* __restrict__ f, Lattice L, double rho0, double thr2,
                                              unsigned char* __restrict__ tag) {
   const long long n = (long long)L.nx * L.ny * L.nz;
@@ -267,7 +275,7 @@ __global__ __launch_bounds__(256) void k_tag(const S* __restrict__ f, Lattice L,
   const int x = int(c % L.nx), y = int((c / L.nx) % L.ny), z = int(c / ((long long)L.nx * L.ny));
   auto hydro_at = [&](int xx, int yy, int zz, double& r, double& u0, double& u1, double& u2) {
     double g[NV];
-    cell_f64<S>(f, L.ps, L.at(0, yy * L.nx + xx, zz), rho0, g);
+    cell_at<S, PULL>(f, L, xx, yy, zz, rho0, g);
     hydro_f64(g, r, u0, u1, u2);
   };
   double s;
@@ -310,20 +318,53 @@ __device__ __forceinline__ void block_sum4(double v[4], double* out) {
     for (int c = 0; c < 4; ++c) out[c] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
 }
 
-template <typename S>
+template <typename S, bool PULL>
 __global__ __launch_bounds__(256) void k_conservation(const S* __restrict__ f, Lattice L,
                                                       double rho0, double* __restrict__ partials) {
   const long long n = (long long)L.nx * L.ny * L.nz;
+  const long long plane = (long long)L.nx * L.ny;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     double g[NV];
-    cell_f64<S>(f, L.ps, L.at_cell(0, i), rho0, g);
+    const long long pc = i % plane;
+    cell_at<S, PULL>(f, L, int(pc % L.nx), int(pc / L.nx), int(i / plane), rho0, g);
     acc[0] += mode_of<0>(g, std::make_integer_sequence<int, NV>{});
     acc[1] += mode_of<1>(g, std::make_integer_sequence<int, NV>{});
     acc[2] += mode_of<2>(g, std::make_integer_sequence<int, NV>{});
     acc[3] += mode_of<3>(g, std::make_integer_sequence<int, NV>{});
   }
   block_sum4(acc, partials + 4 * blockIdx.x);
+}
+
+// The same sums over the valid cells of patched boxes (2 ghost layers, see
+// PatchDesc), every patch in one launch: workgroup (c, b) of the grid
+// (chunks, npatch) adds the c-th strided share of patch b's valid cells, x
+// fastest, into partials[4 * (b * chunks + c)].
+template <typename S>
+__global__ __launch_bounds__(256) void k_patch_conservation(const S* __restrict__ f,
+                                                            const PatchDesc* __restrict__ patches, double rho0,
+                                                            double* __restrict__ partials) {
+  const PatchDesc& P = patches[blockIdx.y];
+  const int nx = P.hi[0] - P.lo[0] + 1, ny = P.hi[1] - P.lo[1] + 1, nz = P.hi[2] - P.lo[2] + 1;
+  const int n = nx * ny * nz;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int x = i % nx, t = i / nx, y = t % ny, z = t / ny;
+    const long long li = P.base + ((long long)(z + 2) * P.g[1] + (y + 2)) * P.g[0] + (x + 2);
+    double g[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      if constexpr (sizeof(S) == 8)
+        g[q] = f[li + q * P.qs];
+      else
+        g[q] = kT.W[q] * rho0 + double(f[li + q * P.qs]);
+    }
+    acc[0] += mode_of<0>(g, std::make_integer_sequence<int, NV>{});
+    acc[1] += mode_of<1>(g, std::make_integer_sequence<int, NV>{});
+    acc[2] += mode_of<2>(g, std::make_integer_sequence<int, NV>{});
+    acc[3] += mode_of<3>(g, std::make_integer_sequence<int, NV>{});
+  }
+  block_sum4(acc, partials + 4 * ((long long)blockIdx.y * gridDim.x + blockIdx.x));
 }
 
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partials,
@@ -424,13 +465,23 @@ hipError_t launch_equilibrium(int precision, void* f, const Lattice& L, const Re
   return hipGetLastError();
 }
 
-hipError_t launch_pack_f(int precision, const void* f, const Lattice& L, const Region& R,
+// Launch KERNEL<S, ..., PULL> for the storage precision and the pull flag.
+#define LBX_PULL_LAUNCH(KERNEL, GRID, ...)                                                              \
+  do {                                                                                                \
+    if (precision == 0 && pull)                                                                       \
+      KERNEL<double, true><<<GRID, 256, 0, st>>>((const double*)f, __VA_ARGS__);                      \
+    else if (precision == 0)                                                                          \
+      KERNEL<double, false><<<GRID, 256, 0, st>>>((const double*)f, __VA_ARGS__);                     \
+    else if (pull)                                                                                    \
+      KERNEL<float, true><<<GRID, 256, 0, st>>>((const float*)f, __VA_ARGS__);                        \
+    else                                                                                              \
+      KERNEL<float, false><<<GRID, 256, 0, st>>>((const float*)f, __VA_ARGS__);                       \
+  } while (0)
+
+hipError_t launch_pack_f(int precision, bool pull, const void* f, const Lattice& L, const Region& R,
                          double* out, double rho0, hipStream_t st) {
   const long long n = (long long)R.n[0] * R.n[1] * R.n[2];
-  if (precision == 0)
-    k_pack<double><<<blocks_for(n), 256, 0, st>>>((const double*)f, L, R, out, rho0);
-  else
-    k_pack<float><<<blocks_for(n), 256, 0, st>>>((const float*)f, L, R, out, rho0);
+  LBX_PULL_LAUNCH(k_pack, blocks_for(n), L, R, out, rho0);
   return hipGetLastError();
 }
 
@@ -444,41 +495,52 @@ hipError_t launch_unpack_f(int precision, void* f, const Lattice& L, const Regio
   return hipGetLastError();
 }
 
-hipError_t launch_hydro(int precision, const void* f, const Lattice& L, const Region& R,
+hipError_t launch_hydro(int precision, bool pull, const void* f, const Lattice& L, const Region& R,
                         double* rho, double* u, double rho0, hipStream_t st) {
   const long long n = (long long)R.n[0] * R.n[1] * R.n[2];
-  if (precision == 0)
-    k_hydro<double><<<blocks_for(n), 256, 0, st>>>((const double*)f, L, R, rho, u, rho0);
-  else
-    k_hydro<float><<<blocks_for(n), 256, 0, st>>>((const float*)f, L, R, rho, u, rho0);
+  LBX_PULL_LAUNCH(k_hydro, blocks_for(n), L, R, rho, u, rho0);
   return hipGetLastError();
 }
 
-hipError_t launch_tag(int precision, int criterion, const void* f, const Lattice& L, double rho0, double thr2,
-                      unsigned char* tag, hipStream_t st) {
+template <typename S, bool PULL>
+static void tag_launch(int criterion, const S* f, const Lattice& L, double rho0, double thr2, unsigned char* tag,
+                int blocks, hipStream_t st) {
+  if (criterion == 0)
+    k_tag<S, 0, PULL><<<blocks, 256, 0, st>>>(f, L, rho0, thr2, tag);
+  else
+    k_tag<S, 1, PULL><<<blocks, 256, 0, st>>>(f, L, rho0, thr2, tag);
+}
+
+hipError_t launch_tag(int precision, bool pull, int criterion, const void* f, const Lattice& L, double rho0,
+                      double thr2, unsigned char* tag, hipStream_t st) {
   const long long n = (long long)L.nx * L.ny * L.nz;
   const int blocks = int((n + 255) / 256);
-  if (precision == 0) {
-    if (criterion == 0)
-      k_tag<double, 0><<<blocks, 256, 0, st>>>((const double*)f, L, rho0, thr2, tag);
-    else
-      k_tag<double, 1><<<blocks, 256, 0, st>>>((const double*)f, L, rho0, thr2, tag);
-  } else {
-    if (criterion == 0)
-      k_tag<float, 0><<<blocks, 256, 0, st>>>((const float*)f, L, rho0, thr2, tag);
-    else
-      k_tag<float, 1><<<blocks, 256, 0, st>>>((const float*)f, L, rho0, thr2, tag);
-  }
+  if (precision == 0 && pull)
+    tag_launch<double, true>(criterion, (const double*)f, L, rho0, thr2, tag, blocks, st);
+  else if (precision == 0)
+    tag_launch<double, false>(criterion, (const double*)f, L, rho0, thr2, tag, blocks, st);
+  else if (pull)
+    tag_launch<float, true>(criterion, (const float*)f, L, rho0, thr2, tag, blocks, st);
+  else
+    tag_launch<float, false>(criterion, (const float*)f, L, rho0, thr2, tag, blocks, st);
   return hipGetLastError();
 }
 
-hipError_t launch_conservation(int precision, const void* f, const Lattice& L, double rho0,
+hipError_t launch_conservation(int precision, bool pull, const void* f, const Lattice& L, double rho0,
                                double* partials, int nblocks, double* out, hipStream_t st) {
-  if (precision == 0)
-    k_conservation<double><<<nblocks, 256, 0, st>>>((const double*)f, L, rho0, partials);
-  else
-    k_conservation<float><<<nblocks, 256, 0, st>>>((const float*)f, L, rho0, partials);
+  LBX_PULL_LAUNCH(k_conservation, nblocks, L, rho0, partials);
   k_reduce_partials<<<1, 256, 0, st>>>(partials, nblocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_conservation(int precision, const void* f, const PatchDesc* patches, int npatch,
+                                     int chunks, double rho0, double* partials, double* out, hipStream_t st) {
+  const dim3 grid(chunks, npatch);
+  if (precision == 0)
+    k_patch_conservation<double><<<grid, 256, 0, st>>>((const double*)f, patches, rho0, partials);
+  else
+    k_patch_conservation<float><<<grid, 256, 0, st>>>((const float*)f, patches, rho0, partials);
+  k_reduce_partials<<<1, 256, 0, st>>>(partials, chunks * npatch, out);
   return hipGetLastError();
 }
 

@@ -246,7 +246,7 @@ int queue_finish(lbx_server* s, Run& r) {
   const lbx::Lattice& L = r.lat;
   char* fin = s->arena.base + r.off + (size_t)((rq.steps + 1) & 1) * L.elems() * esize(s->precision);
   double* out = s->sums + 1024 + 4 * r.slot;
-  SRV_HIP(s, lbx::launch_conservation(s->precision, fin, L, s->rho0, s->sums, 256, out, s->stream));
+  SRV_HIP(s, lbx::launch_conservation(s->precision, false, fin, L, s->rho0, s->sums, 256, out, s->stream));
   SRV_HIP(s, hipMemcpyAsync(s->pinned + 4 * r.slot, out, 4 * sizeof(double), hipMemcpyDeviceToHost,
                             s->stream));
   SRV_HIP(s, hipEventRecord(s->slot_ev[r.slot], s->stream));
@@ -286,7 +286,7 @@ int collect(lbx_server* s, bool wait) {
     res.t_admit = r.t_admit;
     res.t_done = now_s();
     if (rq.out_rho || rq.out_u) {
-      SRV_HIP(s, lbx::launch_hydro(s->precision, fin, L, R, s->staging, s->staging + n, s->rho0, s->stream));
+      SRV_HIP(s, lbx::launch_hydro(s->precision, false, fin, L, R, s->staging, s->staging + n, s->rho0, s->stream));
       if (rq.out_rho)
         SRV_HIP(s, hipMemcpyAsync(rq.out_rho, s->staging, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
       if (rq.out_u)
@@ -295,7 +295,7 @@ int collect(lbx_server* s, bool wait) {
       SRV_HIP(s, hipStreamSynchronize(s->stream));
     }
     if (rq.out_f) {
-      SRV_HIP(s, lbx::launch_pack_f(s->precision, fin, L, R, s->staging, s->rho0, s->stream));
+      SRV_HIP(s, lbx::launch_pack_f(s->precision, false, fin, L, R, s->staging, s->rho0, s->stream));
       SRV_HIP(s, hipMemcpyAsync(rq.out_f, s->staging, lbx::NV_TOTAL * n * sizeof(double),
                                 hipMemcpyDeviceToHost, s->stream));
       SRV_HIP(s, hipStreamSynchronize(s->stream));

@@ -20,8 +20,8 @@
 //   - errors throw lambrex::Error (carrying the reference's message where it
 //     has one) instead of amrex::Abort ending the process;
 //   - AMReX's AmrCore is not available: the members callers used from it
-//     (InitFromScratch, boxArray, refRatio, maxLevel, finestLevel) are
-//     provided here, with base grids chopped like AMReX's
+//     (InitFromScratch, boxArray, DistributionMap, refRatio, maxLevel,
+//     finestLevel) are provided here, with base grids chopped like AMReX's
 //     BoxList::maxSize (max_grid_size 32 by default);
 //   - the distribution function lives on the GPU; DistFn()/SetDistFn() and
 //     Device() expose it for tests and drivers.
@@ -81,6 +81,10 @@ class AmrSim {
   // ---- members callers used from amrex::AmrCore
   void InitFromScratch(double time);
   const std::vector<lambrex::IBox>& boxArray(int level) const { return levels_.at(level).boxes; }
+  // Owner rank of each box of the level: all on this process's GPU (rank 0).
+  lambrex::DistributionMapping DistributionMap(int level) const {
+    return {std::vector<int>(levels_.at(level).boxes.size(), 0)};
+  }
   int maxLevel() const { return max_level_; }
   int finestLevel() const { return finest_level_; }
   std::vector<std::array<int, NDIMS>> refRatio() const;

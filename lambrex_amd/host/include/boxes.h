@@ -26,6 +26,17 @@ struct IBox {
   bool operator==(const IBox& o) const { return lo == o.lo && hi == o.hi; }
 };
 
+// AMReX's DistributionMapping as far as the reference's callers use it
+// (tests/catch2AMRTests.cpp:27, 70, 97 pass it on beside boxArray(l)): the
+// rank owning each box of a level.  Every box lives on the one GPU of the
+// process, rank 0.
+struct DistributionMapping {
+  std::vector<int> owner;
+  int size() const { return (int)owner.size(); }
+  int operator[](int box) const { return owner.at(box); }
+  const std::vector<int>& ProcessorMap() const { return owner; }
+};
+
 // Cells of the coarse level covering a fine box (ratio r), and back.
 IBox coarsen(const IBox& b, int r);
 IBox refine(const IBox& b, int r);

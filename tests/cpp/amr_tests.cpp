@@ -30,6 +30,12 @@ TEST_CASE("TwoLevel Initialisation") {
   REQUIRE(sim->finestLevel() == 0);
   REQUIRE_FALSE(sim->boxArray(0).empty());
   REQUIRE(sim->boxArray(1).empty());
+  // DistributionMap beside boxArray, as tests/catch2AMRTests.cpp:27 builds a
+  // TagBoxArray from the pair: one owner per box, all on this GPU (rank 0)
+  const lambrex::DistributionMapping dm = sim->DistributionMap(0);
+  REQUIRE(dm.size() == (int)sim->boxArray(0).size());
+  for (int b = 0; b < dm.size(); ++b) REQUIRE(dm[b] == 0);
+  REQUIRE(sim->DistributionMap(1).size() == 0);
 }
 
 TEST_CASE("TwoLevel FineGrids") {

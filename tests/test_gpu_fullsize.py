@@ -1,7 +1,11 @@
-"""Bitwise parity at the benchmark's own full sizes, one step each: the
-256^3 single-level workload (64 boxes of 64^3), the two-level workload
-(coarse 128^3 + fine 64x256x256) and the three-level one (+ 32x512x512),
-GPU against the oracle on the same seeded inputs.  The oracle takes a few seconds per step at these sizes on the GPU
+"""Bitwise parity at the benchmark's own full sizes: the 256^3
+single-level workload (64 boxes of 64^3) for three steps, so that the fused
+stream-collide kernel runs at its benchmark launch shape (Iterate(3) from an
+initialised state is one collide-only pass and two fused passes, and the
+readers pull the last stream), the two-level workload (coarse 128^3 + fine
+64x256x256) with both the fine-BoxArray mask and the reference's own mask, and
+the three-level one (+ 32x512x512), GPU against the oracle on the same seeded
+inputs.  The oracle takes a few seconds per step at these sizes on the GPU
 host; the smaller parity tests cover many steps."""
 import sys
 from pathlib import Path
@@ -29,32 +33,57 @@ def same_bits(a, b):
     return bool((np.ascontiguousarray(a).view(np.uint64) == np.ascontiguousarray(b).view(np.uint64)).all())
 
 
-def test_256_cubed_one_step_bitwise(ctx):
+def run_256(ctx, steps, precision=lbx.LBX_FP64):
     dims = (256, 256, 256)
     boxes = workloads.chop(dims, 64)
     rho, u = workloads.random_state(dims, seed=2024, rho_sigma=1e-3, u_amp=0.01)
-    sim = lbx.Sim(ctx, dims)
+    sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
     try:
         sim.define_level(0, boxes)
         sim.set_tau(0, 0.6)
         sim.init_from_fields(rho, u)
-        sim.iterate(1)
-        fg = sim.get_f(0)
+        sim.iterate(steps)
+        return dims, boxes, rho, u, sim.get_f(0), sim.get_hydro(0)
     finally:
         sim.close()
+
+
+def test_256_cubed_three_steps_bitwise(ctx):
+    """The headline workload through the fused kernel at its benchmark launch
+    shape (512-lane workgroups over whole x-rows, one resident per CU):
+    every population after 3 steps equals the oracle's, bit for bit."""
+    dims, boxes, rho, u, fg, _ = run_256(ctx, 3)
     o = orc.Sim(*dims, 0.6, boxes=lists(boxes))
     o.init(*c_order(rho, u))
     del rho, u
-    o.iterate(1)
+    o.iterate(3)
     fo = o.get_f(0)
     o.close()
     assert same_bits(fo, fg)
 
 
-def test_two_level_workload_one_coarse_step_bitwise(ctx):
+def test_256_cubed_fp32_shifted_three_steps_vs_fp64(ctx):
+    """FP32-shifted storage on the same 256^3 state and launch path: density
+    within 1e-6 relative of FP64 after 3 steps (the precision target)."""
+    _, _, _, _, _, (rho64, u64) = run_256(ctx, 3)
+    _, _, _, _, _, (rho32, u32) = run_256(ctx, 3, precision=lbx.LBX_FP32_SHIFTED)
+    rel = float(np.max(np.abs(rho32 - rho64) / rho64))
+    du = float(np.max(np.abs(u32 - u64)) / np.max(np.abs(u64)))
+    print(f"[fp32 256^3 x3] max rel rho err {rel:.3e}, max |du|/max|u| {du:.3e}", flush=True)
+    assert rel <= 1e-6
+    assert du <= 1e-4
+
+
+@pytest.mark.parametrize("mask_kind", ["fine", "reference"])
+def test_two_level_workload_one_coarse_step_bitwise(ctx, mask_kind):
+    """mask_kind "fine": the benchmark's mask (coarse cells under the fine
+    level); "reference": MakeFineMask's own, the coarse BoxArray passed as the
+    fine one (src/AmrSim.cpp:419-428), which masks coarsen(coarse boxes, 2)."""
     sys.path.insert(0, str(ROOT))
     import bench
     dims, cboxes, fboxes, mask, rho, u, _ = bench.two_level_geometry()
+    if mask_kind == "reference":
+        mask = list(cboxes)
     sim = lbx.Sim(ctx, dims)
     try:
         sim.define_level(0, cboxes)

@@ -276,3 +276,46 @@ def test_autotuned_shapes_do_not_change_results(ctx):
         ctx.set_autotune(False)
     assert_bitwise(f_tuned, f_default, "autotuned vs default shape")
     assert bx * by <= 512 and bx in (48, 32) and occ in (0, 1, 2, 3, 4, 6) and ms > 0
+
+
+def test_lazy_state_between_calls_bitwise(ctx):
+    """Between Iterate calls the device keeps the post-collision state and
+    readers pull the last stream: reads after every call, a box upload while
+    the state is post-collision (which first streams it in place) and the
+    walls-free pull of the conservation sums must all agree with the oracle,
+    which keeps the reference's pre-collision state."""
+    dims = (20, 18, 16)
+    boxes = workloads.chop(dims, 8)
+    rho, u = workloads.random_state(dims, seed=31, rho_sigma=1e-2, u_amp=0.03)
+    sim = lbx.Sim(ctx, dims)
+    sim.define_level(0, boxes)
+    sim.set_tau(0, 0.75)
+    sim.init_from_fields(rho, u)
+    o = orc.Sim(*dims, 0.75, boxes=[(list(a), list(b)) for a, b in boxes])
+    o.init(*c_order_fields(rho, u))
+    rng = np.random.default_rng(5)
+    for chunk in (2, 1, 3):
+        sim.iterate(chunk)
+        o.iterate(chunk)
+        fo = o.get_f()
+        ro, uo = o.hydro()
+        rg, ug = sim.get_hydro()
+        assert_bitwise(sim.get_f(), fo, f"f after a chunk of {chunk}")
+        assert_bitwise(rg, ro, "rho")
+        assert_bitwise(ug, uo, "u")
+        # the sums of the pulled populations equal those of the oracle's f
+        mass = sim.conservation()[0]
+        assert abs(mass - ro.sum()) <= 1e-12 * abs(mass)
+        # overwrite box 1 while the device state is post-collision
+        (lo, hi) = boxes[1]
+        sub = fo[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+        new = sub * (1.0 + 1e-3 * rng.standard_normal(sub.shape))
+        sim.upload_f(0, 1, new)
+        fo[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = new
+        o.set_f(fo)
+    sim.iterate(0)
+    sim.iterate(2)
+    o.iterate(2)
+    assert_bitwise(sim.get_f(), o.get_f(), "f after the uploads and 2 more steps")
+    sim.close()
+    o.close()

@@ -360,3 +360,40 @@ def test_fine_regions_meeting_across_the_periodic_x_face(ctx):
             sim.close()
         assert_bitwise(fo0, fg0, f"coarse f ({mask} mask)")
         assert_bitwise(fo1, fg1, f"fine f ({mask} mask)")
+
+
+def test_redefined_fine_level_starts_with_unmasked_parent(ctx):
+    """Define level 1 at one place and mask level 0 for it, clear it, define
+    it somewhere else and set no mask: level 0 must step unmasked (a level
+    starts unmasked), as the oracle does for the new fine level alone.  The
+    old mask would zero coarse cells under the first region every step."""
+    dims = (16, 8, 8)
+    rho, u = workloads.random_state(dims, seed=41, rho_sigma=1e-2, u_amp=0.02)
+    cboxes = workloads.chop(dims, 8)
+    first = fine_boxes_in((0, 0, 0), (7, 15, 15), 8)
+    second = fine_boxes_in((16, 0, 0), (23, 15, 15), 8)
+    o = orc.Sim(*dims, 0.8, boxes=[(list(a), list(b)) for a, b in cboxes])
+    o.init(*c_order(rho, u))
+    o.add_fine([(list(a), list(b)) for a, b in second], [])
+    o.iterate(2)
+    fo0, fo1 = o.get_f(0), o.get_f(1)
+    o.close()
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, 0.8)
+        sim.init_from_fields(rho, u)
+        for boxes, masked in ((first, True), (second, False)):
+            sim.define_level(1, boxes)
+            sim.set_tau(1, 2 * (0.8 - 0.5) + 0.5)
+            sim.fill_from_coarse(1)
+            if masked:
+                sim.set_fine_mask(boxes)
+                sim.clear_level(1)
+        sim.iterate(2)
+        fg0 = sim.get_f(0)
+        fg1, _ = sim.get_f_bbox(1)
+    finally:
+        sim.close()
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
