@@ -8,9 +8,11 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <fstream>
 #include <limits>
 #include <sstream>
+#include <string>
 
 namespace {
 constexpr int kRefRatio = 2;  // ratio 2 on every level (src/AmrSim.cpp:766)
@@ -637,4 +639,234 @@ std::array<double, 4> AmrSim::Conservation(int level) const {
   std::array<double, 4> out{};
   Check(lbx_conservation(sim_, level, out.data()));
   return out;
+}
+
+namespace {
+constexpr const char* kCheckpointTag = "LAMBReX-MI355X-Checkpoint-V1";
+
+std::string hexd(double v) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%a", v);
+  return buf;
+}
+
+// Whitespace-separated tokens of a checkpoint Header.
+struct HeaderReader {
+  std::istringstream in;
+  std::string path;
+  std::string word() {
+    std::string w;
+    if (!(in >> w)) throw lambrex::Error("checkpoint " + path + ": truncated header");
+    return w;
+  }
+  void expect(const char* key) {
+    const std::string w = word();
+    if (w != key) throw lambrex::Error("checkpoint " + path + ": expected '" + key + "', found '" + w + "'");
+  }
+  long long integer() {
+    const std::string w = word();
+    char* end = nullptr;
+    const long long v = std::strtoll(w.c_str(), &end, 10);
+    if (end == w.c_str() || *end) throw lambrex::Error("checkpoint " + path + ": bad integer '" + w + "'");
+    return v;
+  }
+  double real() {
+    const std::string w = word();
+    char* end = nullptr;
+    const double v = std::strtod(w.c_str(), &end);
+    if (end == w.c_str() || *end) throw lambrex::Error("checkpoint " + path + ": bad number '" + w + "'");
+    return v;
+  }
+  lambrex::IBox box() {
+    lambrex::IBox b;
+    for (int d = 0; d < 3; ++d) b.lo[d] = (int)integer();
+    for (int d = 0; d < 3; ++d) b.hi[d] = (int)integer();
+    return b;
+  }
+};
+
+void write_doubles(std::ofstream& f, const double* p, size_t n) {
+  f.write(reinterpret_cast<const char*>(p), (std::streamsize)(n * sizeof(double)));
+}
+
+void read_doubles(std::ifstream& f, double* p, size_t n, const std::string& path) {
+  f.read(reinterpret_cast<char*>(p), (std::streamsize)(n * sizeof(double)));
+  if (!f) throw lambrex::Error("checkpoint " + path + ": file too short");
+}
+}  // namespace
+
+void AmrSim::WriteCheckpoint(const std::string& dir) const {
+  if (!sim_) throw lambrex::Error("WriteCheckpoint before InitFromScratch");
+  make_dir(dir);
+  std::ostringstream h;
+  h << kCheckpointTag << '\n'
+    << "dims " << NX << ' ' << NY << ' ' << NZ << '\n'
+    << "periodicity " << PERIODICITY[0] << ' ' << PERIODICITY[1] << ' ' << PERIODICITY[2] << '\n'
+    << "max_level " << max_level_ << '\n'
+    << "finest_level " << finest_level_ << '\n'
+    << "precision " << precision_ << ' ' << hexd(rho0_) << '\n'
+    << "max_grid_size " << max_grid_size_ << '\n'
+    << "mask_from_fine_grids " << (mask_from_fine_grids_ ? 1 : 0) << '\n';
+  for (int l = 0; l <= max_level_; ++l) {
+    const lambrex::IBox& t = static_tags_.at(l);
+    h << "static_tag " << l << ' ' << t.lo[0] << ' ' << t.lo[1] << ' ' << t.lo[2] << ' ' << t.hi[0] << ' '
+      << t.hi[1] << ' ' << t.hi[2] << '\n';
+  }
+  for (int l = 0; l <= max_level_; ++l)
+    h << "tau " << l << ' ' << hexd(tau_s_.at(l)) << ' ' << hexd(tau_b_.at(l)) << ' ' << hexd(mass_.at(l)) << '\n';
+  for (int l = 0; l <= finest_level_; ++l) {
+    const LevelHost& L = levels_.at(l);
+    h << "level " << l << " boxes " << L.boxes.size() << " time " << hexd(L.time) << " dt " << hexd(L.dt)
+      << " step " << L.step << " cur " << L.cur << '\n';
+    for (const lambrex::IBox& b : L.boxes)
+      h << "box " << b.lo[0] << ' ' << b.lo[1] << ' ' << b.lo[2] << ' ' << b.hi[0] << ' ' << b.hi[1] << ' '
+        << b.hi[2] << '\n';
+  }
+  h << "end\n";
+  for (int l = 0; l <= finest_level_; ++l) {
+    const LevelHost& L = levels_.at(l);
+    const std::string ldir = dir + "/Level_" + std::to_string(l);
+    make_dir(ldir);
+    std::ofstream df(ldir + "/DistFn", std::ios::binary);
+    std::ofstream hf(ldir + "/Hydro", std::ios::binary);
+    for (size_t b = 0; b < L.boxes.size(); ++b) {
+      const std::vector<double> f = DistFn(l, (int)b);
+      write_doubles(df, f.data(), f.size());
+      const size_t n = (size_t)L.boxes[b].numPts();
+      for (int s = 0; s < 2; ++s) {
+        if (L.rho[s].size() == L.boxes.size() && L.rho[s][b].size() == n)
+          write_doubles(hf, L.rho[s][b].data(), n);
+        else
+          write_doubles(hf, std::vector<double>(n, std::numeric_limits<double>::quiet_NaN()).data(), n);
+      }
+      write_doubles(hf, L.vel[b].data(), 3 * n);
+    }
+    if (!df || !hf) throw lambrex::Error("cannot write the data of " + ldir);
+  }
+  std::ofstream hout(dir + "/Header");
+  hout << h.str();
+  if (!hout) throw lambrex::Error("cannot write " + dir + "/Header");
+}
+
+void AmrSim::ReadCheckpoint(const std::string& dir) {
+  HeaderReader r;
+  r.path = dir + "/Header";
+  {
+    std::ifstream hin(r.path);
+    if (!hin) throw lambrex::Error("cannot read " + r.path);
+    std::ostringstream all;
+    all << hin.rdbuf();
+    r.in.str(all.str());
+  }
+  if (r.word() != kCheckpointTag) throw lambrex::Error("checkpoint " + r.path + ": not a checkpoint of this format");
+  r.expect("dims");
+  const long long nx = r.integer(), ny = r.integer(), nz = r.integer();
+  if (nx != NX || ny != NY || nz != NZ)
+    throw lambrex::Error("ReadCheckpoint: the checkpoint's domain differs from this AmrSim's");
+  r.expect("periodicity");
+  for (int d = 0; d < NDIMS; ++d)
+    if (r.integer() != PERIODICITY[d])
+      throw lambrex::Error("ReadCheckpoint: the checkpoint's periodicity differs from this AmrSim's");
+  r.expect("max_level");
+  if (r.integer() != max_level_) throw lambrex::Error("ReadCheckpoint: the checkpoint's max_level differs");
+  r.expect("finest_level");
+  const long long finest = r.integer();
+  if (finest < 0 || finest > max_level_) throw lambrex::Error("ReadCheckpoint: bad finest level");
+  r.expect("precision");
+  const long long precision = r.integer();
+  const double rho0 = r.real();
+  if (precision != LBX_FP64 && precision != LBX_FP32_SHIFTED)
+    throw lambrex::Error("ReadCheckpoint: unknown precision");
+  r.expect("max_grid_size");
+  const long long mgs = r.integer();
+  r.expect("mask_from_fine_grids");
+  const bool mask_fine = r.integer() != 0;
+  std::vector<lambrex::IBox> tags(max_level_ + 1);
+  for (int l = 0; l <= max_level_; ++l) {
+    r.expect("static_tag");
+    if (r.integer() != l) throw lambrex::Error("checkpoint " + r.path + ": static tags out of order");
+    tags[l] = r.box();
+  }
+  std::vector<double> ts(max_level_ + 1), tb(max_level_ + 1), ms(max_level_ + 1);
+  for (int l = 0; l <= max_level_; ++l) {
+    r.expect("tau");
+    if (r.integer() != l) throw lambrex::Error("checkpoint " + r.path + ": tau lines out of order");
+    ts[l] = r.real();
+    tb[l] = r.real();
+    ms[l] = r.real();
+  }
+  std::vector<LevelHost> lv(max_level_ + 1);
+  for (int l = 0; l <= finest; ++l) {
+    r.expect("level");
+    if (r.integer() != l) throw lambrex::Error("checkpoint " + r.path + ": levels out of order");
+    r.expect("boxes");
+    const long long nb = r.integer();
+    if (nb < 1 || nb > (1 << 24)) throw lambrex::Error("checkpoint " + r.path + ": bad box count");
+    LevelHost& L = lv[l];
+    r.expect("time");
+    L.time = r.real();
+    r.expect("dt");
+    L.dt = r.real();
+    r.expect("step");
+    L.step = (int)r.integer();
+    r.expect("cur");
+    L.cur = r.integer() ? 1 : 0;
+    for (long long b = 0; b < nb; ++b) {
+      r.expect("box");
+      L.boxes.push_back(r.box());
+      if (!L.boxes.back().ok()) throw lambrex::Error("checkpoint " + r.path + ": empty box");
+    }
+  }
+  r.expect("end");
+
+  // A fresh device simulation in the checkpoint's precision, every level
+  // defined before any data goes up (defining a level resets its parent's
+  // mask), then the masks, the populations and the host fields.
+  if (sim_) {
+    Check(lbx_sim_destroy(sim_));
+    sim_ = nullptr;
+  }
+  precision_ = (int)precision;
+  rho0_ = rho0;
+  max_grid_size_ = (int)mgs;
+  mask_from_fine_grids_ = mask_fine;
+  static_tags_ = tags;
+  tau_s_ = ts;
+  tau_b_ = tb;
+  mass_ = ms;
+  for (auto& L : levels_) L = LevelHost{};
+  finest_level_ = 0;
+  const int32_t dims[3] = {NX, NY, NZ};
+  const int32_t per[3] = {PERIODICITY[0], PERIODICITY[1], PERIODICITY[2]};
+  Check(lbx_sim_create(lambrex::Context(), dims, per, precision_, rho0_, &sim_));
+  for (int l = 0; l <= finest; ++l) {
+    const std::vector<int32_t> lohi = lambrex::to_lohi(lv[l].boxes);
+    Check(lbx_level_define(sim_, l, (int)lv[l].boxes.size(), lohi.data()));
+    Check(lbx_set_relax(sim_, l, 1.0 / (tau_s_.at(l) + 0.5), 1.0 / (tau_b_.at(l) + 0.5)));
+    levels_.at(l) = lv[l];
+    finest_level_ = l;
+  }
+  for (int l = 0; l < finest_level_; ++l) MakeFineMask(l);
+  for (int l = 0; l <= finest_level_; ++l) {
+    LevelHost& L = levels_.at(l);
+    const std::string ldir = dir + "/Level_" + std::to_string(l);
+    std::ifstream df(ldir + "/DistFn", std::ios::binary);
+    std::ifstream hf(ldir + "/Hydro", std::ios::binary);
+    if (!df || !hf) throw lambrex::Error("cannot read the data of " + ldir);
+    for (int s = 0; s < 2; ++s) L.rho[s].assign(L.boxes.size(), {});
+    L.vel.assign(L.boxes.size(), {});
+    for (size_t b = 0; b < L.boxes.size(); ++b) {
+      const size_t n = (size_t)L.boxes[b].numPts();
+      std::vector<double> f((size_t)NMODES * n);
+      read_doubles(df, f.data(), f.size(), ldir + "/DistFn");
+      Check(lbx_upload_f(sim_, l, (int)b, f.data()));
+      for (int s = 0; s < 2; ++s) {
+        L.rho[s][b].resize(n);
+        read_doubles(hf, L.rho[s][b].data(), n, ldir + "/Hydro");
+      }
+      L.vel[b].resize(3 * n);
+      read_doubles(hf, L.vel[b].data(), 3 * n, ldir + "/Hydro");
+    }
+  }
+  box_hint_.clear();
 }

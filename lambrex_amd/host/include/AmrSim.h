@@ -114,6 +114,21 @@ class AmrSim {
   // little-endian IEEE doubles, x fastest.  The problem domain is the
   // reference's RealBox [0,1]^3.
   void WritePlotfile(const std::string& dir);
+  // Checkpoint and restart (the reference has no I/O).  WriteCheckpoint
+  // writes to `dir`: a text Header (format tag, domain, periodicity,
+  // max/finest level, precision and rho0, max_grid_size, mask choice,
+  // static refinement boxes, and per level its boxes, time, dt, step,
+  // tau_s, tau_b, mass and density-swap parity, doubles as C hexfloats), and
+  // per level Level_<l>/DistFn (every box's populations of the current
+  // state, [15][nz][ny][nx] little-endian FP64, boxes in order) and
+  // Level_<l>/Hydro (both density arrays and the velocity of every box).
+  // ReadCheckpoint takes the place of InitFromScratch on an AmrSim made with
+  // the same domain and periodicity: it rebuilds every level and its masks
+  // and uploads the populations, after which Iterate continues as the
+  // original would have, bit for bit (FP64; FP32-shifted storage restores
+  // the stored deviations exactly wherever f - w rho0 is representable).
+  void WriteCheckpoint(const std::string& dir) const;
+  void ReadCheckpoint(const std::string& dir);
   lbx_sim* Device() const { return sim_; }
   // Populations of the current state of one box, [15][nz][ny][nx] FP64.
   std::vector<double> DistFn(int level, int box) const;

@@ -37,6 +37,16 @@ def test_two_level_amr_cases():
     assert "0 failed" in out
 
 
+def test_checkpoint_restart_is_bitwise():
+    """WriteCheckpoint after Iterate(a), a fresh AmrSim's ReadCheckpoint and
+    Iterate(b) equal Iterate(a + b) bit for bit (one, two and three levels,
+    both masks, walls, FP32-shifted)."""
+    rc, out = run("checkpoint_restart", timeout=300)
+    print(out)
+    assert rc == 0, out
+    assert "0 failed" in out
+
+
 def test_amr_pulse_prints_the_reference_goldens():
     """examples/amr_pulse.cpp re-run through the facade: every printed value
     (precision 6, the format the reference's golden header was made with)
