@@ -192,6 +192,19 @@ int lbx_hydro_level(lbx_sim* sim, int level, double* rho, double* u);
  * src/AmrSim.cpp:312,318 is never used); deterministic order. */
 int lbx_conservation(lbx_sim* sim, int level, double out[4]);
 
+/* The same sums as lbx_conservation, bit for bit, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * allocates nothing: the partial sums live in a fixed device array of the
+ * library (65536 partials of 4 doubles, 2 MiB per GPU, held until the
+ * process ends), and a level that needs more partials (a patched level of
+ * more than 65536 boxes) is rejected with LBX_E_ARG.  d_out is device memory
+ * of at least 4 doubles owned by the caller; the sums are written there in
+ * stream order on the context's stream, and the call returns without
+ * waiting (lbx_sync waits).  Every context on one GPU shares that array:
+ * calls that overlap on one GPU (from two contexts, or two threads) race. */
+int lbx_conservation_noalloc(lbx_sim* sim, int level, double* d_out);
+
 /* Timing of the stream-collide kernel with HIP events recorded on the
  * context's stream around every launch while enabled.  *ms is the summed
  * kernel time, *launches the number of timed launches since the last

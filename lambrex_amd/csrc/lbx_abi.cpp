@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <cstdio>
 #include <cstring>
@@ -128,7 +129,7 @@ struct lbx_ctx {
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
-  double* scratch = nullptr;  // kPartials*4 partials + 4 results
+  double* sums = nullptr;  // conservation sums: kPartials*4 partials + 4 results
   // staging buffer of host transfers (grows on demand, kept until destroy;
   // every call that uses it synchronises before returning)
   void* stage = nullptr;
@@ -1091,34 +1092,65 @@ int fill_from_patched(lbx_sim* sim, int level) {
   return LBX_OK;
 }
 
-// FP64 sums over a level's valid cells (lbx_conservation) with the
-// reduction's partials in `scratch` (room for max_partials partials of 4
-// doubles, then the 4 results): a level array in kLatticePartials
-// workgroups, gathering through the pull when it holds the post-collision
-// state; a patched level in one launch over all patches.  The partial
-// layout depends only on the level, so any scratch gives the same bits.
-int conservation_sums(lbx_sim* sim, int level, double* scratch, long long max_partials, double out[4]) {
+// FP64 sums over a level's valid cells, enqueued on the context's stream
+// with the reduction's partials in `partials` (room for max_partials
+// partials of 4 doubles) and the 4 results written to `res` (device
+// memory): a level array in kLatticePartials workgroups, gathering through
+// the pull when it holds the post-collision state; a patched level in one
+// launch over all patches.  The partial layout depends only on the level, so
+// any partials buffer gives the same bits.
+int enqueue_conservation(lbx_sim* sim, int level, double* partials, long long max_partials, double* res) {
   lbx_ctx* ctx = sim->ctx;
   Level& L = sim->lev[level];
-  double* res = scratch + 4 * max_partials;
   if (!L.patched) {
     if (kLatticePartials > max_partials)
-      return fail(ctx, LBX_E_ARG, "conservation needs %d partial sums, the scratch holds %lld", kLatticePartials,
+      return fail(ctx, LBX_E_ARG, "conservation needs %d partial sums, the buffer holds %lld", kLatticePartials,
                   max_partials);
-    LBX_HIP(ctx, lbx::launch_conservation(sim->precision, L.post, L.buf[L.cur], L.lat, sim->rho0, scratch,
+    LBX_HIP(ctx, lbx::launch_conservation(sim->precision, L.post, L.buf[L.cur], L.lat, sim->rho0, partials,
                                           kLatticePartials, res, ctx->stream));
   } else {
     const int np = (int)L.boxes.size();
     const int chunks = patch_chunks(np);
     if ((long long)np * chunks > max_partials)
-      return fail(ctx, LBX_E_ARG, "conservation of level %d (%d boxes) needs %lld partial sums, the scratch holds %lld",
+      return fail(ctx, LBX_E_ARG, "conservation of level %d (%d boxes) needs %lld partial sums, the buffer holds %lld",
                   level, np, (long long)np * chunks, max_partials);
     LBX_HIP(ctx, lbx::launch_patch_conservation(sim->precision, L.buf[L.cur], L.d_desc, np, chunks, sim->rho0,
-                                                scratch, res, ctx->stream));
+                                                partials, res, ctx->stream));
   }
+  return LBX_OK;
+}
+
+// lbx_conservation: partials and results in the context's own buffer, then
+// a synchronous copy of the results to the host.
+int conservation_sums(lbx_sim* sim, int level, double* buf, long long max_partials, double out[4]) {
+  lbx_ctx* ctx = sim->ctx;
+  double* res = buf + 4 * max_partials;
+  if (int rc = enqueue_conservation(sim, level, buf, max_partials, res)) return rc;
   LBX_HIP(ctx, hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
+}
+
+// Device memory of lbx_conservation_noalloc: the reduction's partials live in
+// this array, one copy per GPU, held until the process ends.  Host code never
+// passes `scratch` itself to a kernel (in host code the name is a stand-in,
+// not a device address); it uses the address hipGetSymbolAddress returns for
+// the device, looked up once per device at context creation.
+constexpr long long kNoAllocPartials = 65536;  // 2 MiB of partial sums
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocPartials * 4];
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's current device must be `device`.
+hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[device] = (double*)p;
+  return e;
 }
 
 }  // namespace
@@ -1148,7 +1180,8 @@ int lbx_ctx_create(int device, lbx_ctx** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipMalloc(&ctx->scratch, (kPartials * 4 + 4) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&ctx->sums, (kPartials * 4 + 4) * sizeof(double));
+  if (e == hipSuccess) e = scratch_setup(device);
   if (e != hipSuccess) {
     hip_fail(nullptr, e, "context set-up");
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
@@ -1168,7 +1201,7 @@ int lbx_ctx_destroy(lbx_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(ctx->side);
-  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->sums) (void)hipFree(ctx->sums);
   if (ctx->stage) (void)hipFree(ctx->stage);
   (void)hipEventDestroy(ctx->join_ev);
   (void)hipEventDestroy(ctx->fork_ev);
@@ -1506,7 +1539,23 @@ int lbx_conservation(lbx_sim* sim, int level, double out[4]) {
   if (level < 0 || level >= LBX_MAX_LEVELS || !sim->lev[level].defined)
     return fail(sim->ctx, LBX_E_STATE, "level %d is not defined", level);
   LBX_HIP(sim->ctx, hipSetDevice(sim->ctx->device));
-  return conservation_sums(sim, level, sim->ctx->scratch, kPartials, out);
+  return conservation_sums(sim, level, sim->ctx->sums, kPartials, out);
+}
+
+int lbx_conservation_noalloc(lbx_sim* sim, int level, double* d_out) {
+  if (int rc = check_sim(sim)) return rc;
+  lbx_ctx* ctx = sim->ctx;
+  if (!d_out) return fail(ctx, LBX_E_ARG, "null output");
+  if (level < 0 || level >= LBX_MAX_LEVELS || !sim->lev[level].defined)
+    return fail(ctx, LBX_E_STATE, "level %d is not defined", level);
+  LBX_HIP(ctx, hipSetDevice(ctx->device));
+  double* partials = nullptr;
+  if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    partials = g_scratch_ptr[ctx->device];
+  }
+  if (!partials) return fail(ctx, LBX_E_STATE, "no scratch address for device %d", ctx->device);
+  return enqueue_conservation(sim, level, partials, kNoAllocPartials, d_out);
 }
 
 int lbx_profile_enable(lbx_ctx* ctx, int enable) {

@@ -55,6 +55,7 @@ SIGNATURES = {
     "lbx_iterate": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_hydro": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "lbx_conservation": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "lbx_conservation_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_void_p]),
     "lbx_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "lbx_bench_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, _dp]),
@@ -355,6 +356,11 @@ class Sim:
         out = np.zeros(4)
         self.ctx.check(self._lib.lbx_conservation(self.h, level, _ptr(out, ctypes.c_double)))
         return out
+
+    def conservation_noalloc(self, level, d_out):
+        """lbx_conservation_noalloc: the 4 sums written in stream order to
+        d_out, a device address of at least 4 doubles; no allocation, no wait."""
+        self.ctx.check(self._lib.lbx_conservation_noalloc(self.h, level, ctypes.c_void_p(d_out)))
 
     # --- whole-level helpers (level 0 tiles the domain)
     def level_array(self, level, per_box, ncomp):
