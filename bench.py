@@ -4,7 +4,12 @@ Default workload: 256^3 periodic domain decomposed into 64 boxes of 64^3,
 FP64, tau 0.6, rho = 1 + 1e-3 N(0,1), u_a ~ U(-0.01, 0.01) (seed 2024).
 A step is one lattice-Boltzmann time step over the whole domain.  The timed
 region is one Iterate(K) call (K steps) with the state resident in HBM,
-bracketed by stream synchronisations.
+bracketed by stream synchronisations.  Before it, untimed: a clock warm-up
+(steps of the same simulation for at least --clock-warmup-ms of wall time, so
+that the GPU's clocks have left their idle state: after idling, the first
+~25 launches of the fused kernel run up to 9 % slower; the FP32-shifted run
+takes the same number of warm-up steps as the FP64 one it is compared
+with), then the W warm-up steps.
 
 Prints one JSON line.  `roofline` is computed for the fused stream-collide
 kernel from its average duration, measured with HIP events on the library's
@@ -33,14 +38,39 @@ TRAFFIC_FILE = ROOT / "profiles" / "traffic.json"
 COPY_BYTES = 2 << 30  # 2 GiB read + 2 GiB written per copy dispatch
 
 
+CLOCK_WARMUP_MS = 200.0
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup, keep_fields=False, periodic=(1, 1, 1)):
-    """Iterate(steps) after Iterate(warmup), timed; with keep_fields the final
-    density and velocity come back as result["fields"].  A 0 in periodic puts
-    halfway bounce-back walls on both faces normal to that axis."""
+def clock_warmup(ctx, sim, ms, chunk=10, steps=None):
+    """Untimed steps of `sim` until `ms` of wall time have passed (at least one
+    chunk), so the GPU's clocks leave their idle state before anything is
+    timed; or exactly `steps` steps, so that a run compared with an earlier
+    one reaches the same simulated time.  Returns the steps taken."""
+    if steps is not None:
+        sim.iterate(steps)
+        ctx.sync()
+        return steps
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        sim.iterate(chunk)
+        ctx.sync()
+        done += chunk
+        if (time.perf_counter() - t0) * 1e3 >= ms:
+            return done
+
+
+def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup, keep_fields=False, periodic=(1, 1, 1),
+            clock_steps=None):
+    """Iterate(steps) after the clock warm-up and Iterate(warmup), timed; with
+    keep_fields the final density and velocity come back as result["fields"].
+    A 0 in periodic puts halfway bounce-back walls on both faces normal to
+    that axis.  clock_steps fixes the clock warm-up's step count (to compare
+    with an earlier run); result["clock_steps"] is the count taken."""
     boxes = workloads.chop(dims, mgs)
     fields = None
     sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0, periodic=periodic)
@@ -49,6 +79,7 @@ def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup, keep_fields=F
         sim.set_tau(0, tau)
         sim.init_from_fields(rho, u)
         c0 = sim.conservation()
+        clock_steps = clock_warmup(ctx, sim, CLOCK_WARMUP_MS, steps=clock_steps)
         if warmup:
             sim.iterate(warmup)
         ctx.sync()
@@ -77,6 +108,7 @@ def measure(ctx, dims, mgs, tau, rho, u, precision, steps, warmup, keep_fields=F
         "kernel_gbs": achieved,
         "mass_drift_rel": abs(c1[0] - c0[0]) / abs(c0[0]),
         "fields": fields,
+        "clock_steps": clock_steps,
     }
 
 
@@ -212,7 +244,9 @@ def two_level_geometry():
 def two_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
     """The two-level configuration on the GPU (one coarse step = one replay
     of the captured HIP graph).  Also returns the final density and velocity
-    of both levels (for the FP32 vs FP64 comparison)."""
+    of both levels (for the FP32 vs FP64 comparison).  Its timed region
+    (hundreds of coarse steps) is long enough that the GPU's clock ramp after
+    idling does not matter, so there is no clock warm-up."""
     dims, cboxes, fboxes, mask, rho, u, lu_per_step = two_level_geometry()
     sim = lbx.Sim(ctx, dims, precision=precision, rho0=1.0)
     try:
@@ -365,6 +399,7 @@ def cpu_model():
 
 
 def main():
+    global CLOCK_WARMUP_MS
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
@@ -378,6 +413,9 @@ def main():
                     help="time every workgroup shape first and use the fastest (default: the default shape)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the copy kernel (counter calibration)")
+    ap.add_argument("--clock-warmup-ms", type=float, default=CLOCK_WARMUP_MS,
+                    help="untimed steps before the W warm-up steps of each single-level measurement, for at "
+                         "least this much wall time (lets the GPU clocks leave their idle state)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-two-level-steps", type=int, default=1,
                     help="coarse steps of the two-level CPU baseline (0 skips it)")
@@ -393,6 +431,7 @@ def main():
     ap.add_argument("--walls-steps", type=int, default=200,
                     help="steps of the FP64 measurement with bounce-back walls on both z faces (0 skips it)")
     args = ap.parse_args()
+    CLOCK_WARMUP_MS = args.clock_warmup_ms
 
     if args.copy_only:
         with lbx.Context(0) as ctx:
@@ -508,7 +547,7 @@ def main():
         if not args.no_fp32:
             log("[bench] FP32-shifted, same workload")
             r32 = measure(ctx, dims, args.max_grid_size, tau, rho, u, lbx.LBX_FP32_SHIFTED,
-                          args.steps, args.warmup, keep_fields=True)
+                          args.steps, args.warmup, keep_fields=True, clock_steps=r64["clock_steps"])
             (r0, u0), (s0, v0) = r64.pop("fields"), r32.pop("fields")
             r32["max_rel_rho_err_vs_fp64"] = float(np.max(np.abs(s0 - r0) / r0))
             r32["max_du_over_max_u_vs_fp64"] = float(np.max(np.abs(v0 - u0)) / np.max(np.abs(u0)))
@@ -530,6 +569,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "clock_warmup_ms": args.clock_warmup_ms,
+        "clock_warmup_steps": r64["clock_steps"],
         "ms_per_step": round(r64["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
