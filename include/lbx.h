@@ -234,14 +234,19 @@ int lbx_bench_copy(lbx_ctx* ctx, size_t bytes, int reps, double* gbs);
 
 /* ---------------------------------------------------------------------
  * Batched serving of independent runs.  No reference counterpart: the
- * reference runs one AmrSim at a time.  Each request is a periodic
- * single-level run (its FP64 populations must fit the server's 256 MB
- * staging buffer: at most 2,236,962 cells; 128^3 fits) with its own tau,
- * initial state and step count.  Requests are admitted in arrival order
- * while the server's device arena has room; all admitted runs advance
- * together, one patch-list kernel launch per step over the work items of
- * every active run.  Each run computes exactly what lbx_iterate computes for
- * it alone, bit for bit. */
+ * reference runs one AmrSim at a time.  A request is a periodic run with
+ * its own tau, initial state and step count: a single-level run (its FP64
+ * populations must fit the server's 256 MB staging buffer: at most
+ * 2,236,962 cells; 128^3 fits), or, with fine boxes, a two-level run
+ * (ratio 2) stepped by the Rohde cycle (src/AmrSim.cpp:430-469).
+ * Single-level requests are admitted while the server's device arena has
+ * room and advance together, one patch-list kernel launch per step over the
+ * work items of every active run; a refined run is admitted while the
+ * device has memory for its levels, is held in a simulation of its own on
+ * the context (one captured level-0 step per server step, on the context's
+ * stream, overlapping the patch-list launches on the server's stream) and
+ * is released when delivered.  Each run computes exactly what lbx_iterate
+ * computes for it alone, bit for bit. */
 
 #define LBX_INIT_PLANE_PULSE 0  /* rho = 1 + amplitude on plane `plane` normal to `axis`, u = 0 */
 #define LBX_INIT_FIELDS 1       /* rho [nz][ny][nx] and u [3][nz][ny][nx] given (copied at submit) */
@@ -260,11 +265,22 @@ typedef struct {
   double* out_rho;         /* optional, filled at completion: [nz][ny][nx] */
   double* out_u;           /* optional: [3][nz][ny][nx] */
   double* out_f;           /* optional: FP64 populations [15][nz][ny][nx] */
+  /* Refined runs (n_fine_boxes > 0): level 1 (ratio 2) over fine_lohi, 6
+   * int32 per box (inclusive lo, hi in the fine index space 2*dims), aligned
+   * to the ratio; tau of level 1 rescaled as ComputeDt does
+   * (src/AmrSim.cpp:313-314); level 0 masked under level 1 (the fine-grid
+   * mask).  Level 0 is box_lohi's n_boxes boxes (one box over the domain if
+   * n_boxes = 0; single-level runs take none).  mass/momentum and the out_*
+   * fields are level 0's.  Box lists are copied at submit. */
+  int32_t n_boxes;
+  const int32_t* box_lohi;
+  int32_t n_fine_boxes;
+  const int32_t* fine_lohi;
 } lbx_request;
 
 typedef struct {
   uint64_t id;
-  int32_t status;
+  int32_t status;          /* LBX_OK, or why a refined run could not be set up */
   int32_t steps;
   int64_t cells;
   double mass;             /* FP64 sum of rho at completion */

@@ -1156,6 +1156,7 @@ hipError_t scratch_setup(int device) {
 }  // namespace
 
 int lbx::ctx_device(const lbx_ctx* ctx) { return ctx->device; }
+hipStream_t lbx::ctx_stream(const lbx_ctx* ctx) { return ctx->stream; }
 
 extern "C" {
 

@@ -12,8 +12,9 @@ namespace lbx {
 
 constexpr int NV_TOTAL = 15;
 
-// Device ordinal of a context (lbx_abi.cpp).
-int ctx_device(const lbx_ctx* ctx);  // populations per cell (D3Q15)
+// Device ordinal and HIP stream of a context (lbx_abi.cpp).
+int ctx_device(const lbx_ctx* ctx);
+hipStream_t ctx_stream(const lbx_ctx* ctx);
 
 #if defined(__HIPCC__)
 #define LBX_IHD __host__ __device__ __forceinline__

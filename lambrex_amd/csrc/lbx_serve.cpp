@@ -16,6 +16,16 @@
 // pass and copied to pinned host memory) and any requested fields are
 // handed to the caller and its arena space is released; nothing in the
 // steady state waits for the GPU except the in-flight bound.
+//
+// A request with fine boxes is a refined run: a two-level hierarchy (ratio
+// 2, the fine-grid mask) held in a simulation of its own, made through the
+// C ABI on the server's context, whose level-0 steps (one captured Rohde
+// cycle each) go to the context's stream while the patch-list launches of
+// the single-level runs go to the server's stream, so the two overlap on the
+// device.  It takes one coarse step per global step, is admitted against
+// the device's free memory (its levels are allocated when it is admitted
+// and released when it is delivered), and its result is what lbx_iterate
+// computes for the same hierarchy alone, bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,6 +49,8 @@ namespace {
 
 constexpr int kInFlight = 4;
 constexpr size_t kAlign = 256;
+// device memory left free when a refined run is admitted
+constexpr size_t kRefinedReserve = (size_t)512 << 20;
 
 double now_s() {
   using namespace std::chrono;
@@ -96,8 +108,11 @@ struct Request {
   uint64_t id;
   lbx_request req;
   std::vector<double> rho, u;  // copies of caller fields (LBX_INIT_FIELDS)
+  std::vector<int32_t> boxes, fine;  // copies of the level-0 / level-1 boxes
   double t_submit;
 };
+
+bool refined(const lbx_request& q) { return q.n_fine_boxes > 0; }
 
 struct Run {
   Request r;
@@ -109,6 +124,9 @@ struct Run {
   long long last_step = 0;
   int slot = -1;          // per-run result slot (device sums, pinned copy, event)
   bool finish_queued = false;
+  lbx_sim* sim = nullptr;  // refined run: its own hierarchy
+  int steps_done = 0;      // refined run: level-0 steps enqueued
+  size_t device_bytes = 0;  // refined run: device memory it was admitted against
 };
 
 }  // namespace
@@ -138,8 +156,11 @@ struct lbx_server {
   long long step = 0;  // next global step to launch
   hipEvent_t ev[kInFlight] = {};
   long long ev_step[kInFlight] = {-1, -1, -1, -1};
+  hipEvent_t rev[kInFlight] = {};  // refined runs' steps on the context's stream
+  long long rev_step[kInFlight] = {-1, -1, -1, -1};
   std::string err;
   int64_t lattice_updates = 0;
+  size_t refined_bytes = 0;  // device memory held by active refined runs
 };
 
 namespace {
@@ -163,6 +184,97 @@ int sfail(lbx_server* s, int code, const char* fmt, ...) {
 size_t esize(int precision) { return precision == LBX_FP64 ? 8 : 4; }
 
 long long cells_of(const lbx_request& q) { return (long long)q.dims[0] * q.dims[1] * q.dims[2]; }
+
+// Device memory a refined run's simulation allocates: two level-0 arrays
+// and their flag bytes, four buffers of grown fine patches (2 ghost layers,
+// population stride rounded to 64) with their fill table, mask bytes and
+// block averages, and the parent's touched-cell slots; 1/8 more for the
+// smaller tables.
+size_t refined_bytes_estimate(const Request& q, int precision) {
+  const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2]);
+  const long long coarse_cells = (long long)L.nx * L.ny * L.nz;
+  size_t bytes = 2 * (size_t)L.elems() * esize(precision) + (size_t)coarse_cells * 5;
+  for (size_t b = 0; b + 5 < q.fine.size(); b += 6) {
+    long long grown = 1, blocks = 1;
+    for (int d = 0; d < 3; ++d) {
+      const long long g = q.fine[b + 3 + d] - q.fine[b + d] + 1 + 4;
+      grown *= g;
+      blocks *= g / 2;
+    }
+    const long long qs = (grown + 63) / 64 * 64;
+    bytes += (size_t)(4 * qs * lbx::NV_TOTAL) * esize(precision) + (size_t)grown * 9 +
+             (size_t)blocks * lbx::NV_TOTAL * sizeof(double);
+  }
+  return bytes + bytes / 8;
+}
+
+// Level-0 fields of box [lo, hi] of a refined run, x fastest, from the
+// request: the plane pulse (rho = 1 + amplitude on the plane, u = 0) or the
+// caller's whole-domain fields.
+void box_fields(const Request& q, const int32_t* lohi, std::vector<double>& rho, std::vector<double>& u) {
+  const lbx_request& rq = q.req;
+  const int n0 = lohi[3] - lohi[0] + 1, n1 = lohi[4] - lohi[1] + 1, n2 = lohi[5] - lohi[2] + 1;
+  const long long n = (long long)n0 * n1 * n2, dom = (long long)rq.dims[0] * rq.dims[1] * rq.dims[2];
+  rho.assign((size_t)n, 1.0);
+  u.assign((size_t)(3 * n), 0.0);
+  for (int z = 0; z < n2; ++z)
+    for (int y = 0; y < n1; ++y)
+      for (int x = 0; x < n0; ++x) {
+        const int p[3] = {lohi[0] + x, lohi[1] + y, lohi[2] + z};
+        const long long i = ((long long)z * n1 + y) * n0 + x;
+        const long long g = ((long long)p[2] * rq.dims[1] + p[1]) * rq.dims[0] + p[0];
+        if (rq.init_kind == LBX_INIT_PLANE_PULSE) {
+          if (p[rq.axis] == rq.plane) rho[i] = 1.0 + rq.amplitude;
+        } else {
+          rho[i] = q.rho[g];
+          for (int a = 0; a < 3; ++a) u[a * n + i] = q.u[a * dom + g];
+        }
+      }
+}
+
+// The refined run's simulation on the server's context: level 0 from its
+// boxes (one box over the domain if none were given) at equilibrium, level 1
+// filled from it by piecewise-constant interpolation with tau rescaled as
+// ComputeDt does (src/AmrSim.cpp:297-322), and level 0 masked under level 1.
+int start_refined(lbx_server* s, Run& r) {
+  const Request& q = r.r;
+  const lbx_request& rq = q.req;
+  const int32_t per[3] = {1, 1, 1};
+  auto abi = [s](int rc) {
+    if (rc != LBX_OK) sfail(s, rc, "refined run: %s", lbx_last_error(s->ctx));
+    return rc;
+  };
+  if (int rc = abi(lbx_sim_create(s->ctx, rq.dims, per, s->precision, s->rho0, &r.sim))) return rc;
+  std::vector<int32_t> boxes = q.boxes;
+  if (boxes.empty()) boxes = {0, 0, 0, rq.dims[0] - 1, rq.dims[1] - 1, rq.dims[2] - 1};
+  const int nb = (int)boxes.size() / 6, nf = (int)q.fine.size() / 6;
+  if (int rc = abi(lbx_level_define(r.sim, 0, nb, boxes.data()))) return rc;
+  if (int rc = abi(lbx_set_relax(r.sim, 0, 1.0 / (rq.tau_s + 0.5), 1.0 / (rq.tau_b + 0.5)))) return rc;
+  std::vector<double> rho, u;
+  for (int b = 0; b < nb; ++b) {
+    box_fields(q, &boxes[6 * b], rho, u);
+    if (int rc = abi(lbx_init_equilibrium(r.sim, 0, b, rho.data(), u.data()))) return rc;
+  }
+  const double tf_s = 2 * (rq.tau_s - 0.5) + 0.5, tf_b = 2 * (rq.tau_b - 0.5) + 0.5;
+  if (int rc = abi(lbx_level_define(r.sim, 1, nf, q.fine.data()))) return rc;
+  if (int rc = abi(lbx_set_relax(r.sim, 1, 1.0 / (tf_s + 0.5), 1.0 / (tf_b + 0.5)))) return rc;
+  if (int rc = abi(lbx_fill_from_coarse(r.sim, 1))) return rc;
+  return abi(lbx_set_fine_mask(r.sim, nf, q.fine.data()));
+}
+
+// A refined run that could not be set up is delivered at once with the
+// error status; nothing of it stays on the device.
+void reject_refined(lbx_server* s, Run& r, int rc) {
+  if (r.sim) lbx_sim_destroy(r.sim);
+  r.sim = nullptr;
+  lbx_result res{};
+  res.id = r.r.id;
+  res.status = rc;
+  res.t_submit = r.r.t_submit;
+  res.t_admit = res.t_done = now_s();
+  s->done.push_back(res);
+  s->free_slots.push_back(r.slot);
+}
 
 // Admission: move queued requests into the active set while they fit, in
 // arrival order (FIFO, stopping at the first that does not fit) or in order
@@ -194,6 +306,32 @@ int admit(lbx_server* s) {
     const bool fifo = s->policy != LBX_SERVE_SHORTEST_FIRST;
     if (fifo ? s->queue.empty() : k >= order.size()) break;
     Request& q = fifo ? s->queue.front() : s->queue[order[k]];
+    if (refined(q.req)) {
+      const size_t need = refined_bytes_estimate(q, s->precision);
+      size_t fr = 0, tot = 0;
+      SRV_HIP(s, hipMemGetInfo(&fr, &tot));
+      if (need + kRefinedReserve > fr) {
+        if (fifo) break;
+        continue;
+      }
+      Run r;
+      r.slot = s->free_slots.back();
+      s->free_slots.pop_back();
+      r.r = std::move(q);
+      if (fifo)
+        s->queue.pop_front();
+      else
+        q.id = 0;
+      r.t_admit = now_s();
+      if (int rc = start_refined(s, r)) {
+        reject_refined(s, r, rc);
+        continue;
+      }
+      r.device_bytes = need;
+      s->refined_bytes += need;
+      s->active.push_back(std::move(r));
+      continue;
+    }
     const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2]);
     const size_t bytes = (size_t)2 * L.elems() * esize(s->precision);
     size_t off = 0;
@@ -254,6 +392,75 @@ int queue_finish(lbx_server* s, Run& r) {
   return LBX_OK;
 }
 
+// A finished refined run: level-0 sums and fields (optional outputs are
+// level 0's, over the whole domain like a single-level run's), then its
+// simulation is released.
+int deliver_refined(lbx_server* s, Run& r) {
+  const lbx_request& rq = r.r.req;
+  auto abi = [s](int rc) {
+    if (rc != LBX_OK) sfail(s, rc, "refined run: %s", lbx_last_error(s->ctx));
+    return rc;
+  };
+  lbx_result res{};
+  res.id = r.r.id;
+  res.status = LBX_OK;
+  res.steps = rq.steps;
+  const long long n = cells_of(rq);
+  res.cells = n;
+  double sums[4];
+  if (int rc = abi(lbx_conservation(r.sim, 0, sums))) return rc;
+  res.mass = sums[0];
+  for (int a = 0; a < 3; ++a) res.momentum[a] = sums[1 + a];
+  std::vector<int32_t> boxes = r.r.boxes;
+  if (boxes.empty()) boxes = {0, 0, 0, rq.dims[0] - 1, rq.dims[1] - 1, rq.dims[2] - 1};
+  const int nb = (int)boxes.size() / 6;
+  // box b's cell (x, y, z) of its [n2][n1][n0] block -> whole-domain index
+  auto scatter = [&](const int32_t* lohi, const double* src, double* dst, int ncomp, long long box_n) {
+    const int n0 = lohi[3] - lohi[0] + 1, n1 = lohi[4] - lohi[1] + 1, n2 = lohi[5] - lohi[2] + 1;
+    for (int c = 0; c < ncomp; ++c)
+      for (int z = 0; z < n2; ++z)
+        for (int y = 0; y < n1; ++y)
+          for (int x = 0; x < n0; ++x)
+            dst[c * n + ((long long)(lohi[2] + z) * rq.dims[1] + lohi[1] + y) * rq.dims[0] + lohi[0] + x] =
+                src[c * box_n + ((long long)z * n1 + y) * n0 + x];
+  };
+  if (rq.out_rho || rq.out_u) {
+    std::vector<double> rho((size_t)n), u((size_t)(3 * n));
+    if (int rc = abi(lbx_hydro_level(r.sim, 0, rho.data(), u.data()))) return rc;
+    long long off = 0;
+    for (int b = 0; b < nb; ++b) {
+      const int32_t* lohi = &boxes[6 * b];
+      const long long bn = (long long)(lohi[3] - lohi[0] + 1) * (lohi[4] - lohi[1] + 1) * (lohi[5] - lohi[2] + 1);
+      if (rq.out_rho) scatter(lohi, rho.data() + off, rq.out_rho, 1, bn);
+      if (rq.out_u) scatter(lohi, u.data() + 3 * off, rq.out_u, 3, bn);
+      off += bn;
+    }
+  }
+  if (rq.out_f) {
+    std::vector<double> f;
+    for (int b = 0; b < nb; ++b) {
+      const int32_t* lohi = &boxes[6 * b];
+      const long long bn = (long long)(lohi[3] - lohi[0] + 1) * (lohi[4] - lohi[1] + 1) * (lohi[5] - lohi[2] + 1);
+      f.resize((size_t)(lbx::NV_TOTAL * bn));
+      if (int rc = abi(lbx_download_f(r.sim, 0, b, f.data()))) return rc;
+      scatter(lohi, f.data(), rq.out_f, lbx::NV_TOTAL, bn);
+    }
+  }
+  long long fine_cells = 0;
+  for (size_t b = 0; b + 5 < r.r.fine.size(); b += 6)
+    fine_cells += (long long)(r.r.fine[b + 3] - r.r.fine[b] + 1) * (r.r.fine[b + 4] - r.r.fine[b + 1] + 1) *
+                  (r.r.fine[b + 5] - r.r.fine[b + 2] + 1);
+  s->lattice_updates += (n + 2 * fine_cells) * (int64_t)rq.steps;
+  lbx_sim_destroy(r.sim);
+  r.sim = nullptr;
+  s->refined_bytes -= r.device_bytes;
+  res.t_submit = r.r.t_submit;
+  res.t_admit = r.t_admit;
+  res.t_done = now_s();
+  s->done.push_back(res);
+  return LBX_OK;
+}
+
 // Deliver runs whose sums have reached the host; optional field outputs are
 // copied synchronously (the run's arena space is still intact), then the
 // arena space and the slot are released.
@@ -270,6 +477,12 @@ int collect(lbx_server* s, bool wait) {
       continue;
     }
     if (q != hipSuccess) return sfail(s, LBX_E_HIP, "run completion: %s", hipGetErrorString(q));
+    if (r.sim) {
+      if (int rc = deliver_refined(s, r)) return rc;
+      s->free_slots.push_back(r.slot);
+      s->active.erase(s->active.begin() + i);
+      continue;
+    }
     const lbx_request& rq = r.r.req;
     const long long n = cells_of(rq);
     const lbx::Lattice& L = r.lat;
@@ -310,10 +523,33 @@ int collect(lbx_server* s, bool wait) {
 }
 
 int launch_step(lbx_server* s) {
+  // refined runs: one level-0 step each on the context's stream, at most
+  // kInFlight steps ahead of the GPU
+  bool stepped = false;
+  for (Run& r : s->active) {
+    if (!r.sim || r.launched_last) continue;
+    if (!stepped) {
+      const int k = int(s->step % kInFlight);
+      if (s->rev_step[k] >= 0) SRV_HIP(s, hipEventSynchronize(s->rev[k]));
+    }
+    if (int rc = lbx_iterate(r.sim, 1)) return sfail(s, rc, "refined run: %s", lbx_last_error(s->ctx));
+    stepped = true;
+    if (++r.steps_done == r.r.req.steps) {
+      r.launched_last = true;
+      r.last_step = s->step;
+      SRV_HIP(s, hipEventRecord(s->slot_ev[r.slot], lbx::ctx_stream(s->ctx)));
+      r.finish_queued = true;
+    }
+  }
+  if (stepped) {
+    const int k = int(s->step % kInFlight);
+    SRV_HIP(s, hipEventRecord(s->rev[k], lbx::ctx_stream(s->ctx)));
+    s->rev_step[k] = s->step;
+  }
   ServeBatch b{};
   int items = 0;
   for (Run& r : s->active) {
-    if (r.launched_last) continue;
+    if (r.sim || r.launched_last) continue;
     ServeRun& d = b.run[b.nruns++];
     d.buf[0] = s->arena.base + r.off;
     d.buf[1] = s->arena.base + r.off + (size_t)r.lat.elems() * esize(s->precision);
@@ -332,7 +568,10 @@ int launch_step(lbx_server* s) {
     }
   }
   b.total_items = items;
-  if (b.nruns == 0) return LBX_OK;
+  if (b.nruns == 0) {
+    if (stepped) ++s->step;
+    return LBX_OK;
+  }
   // reuse the event slot of step - kInFlight once the GPU has passed it
   const int k = int(s->step % kInFlight);
   if (s->ev_step[k] >= 0) SRV_HIP(s, hipEventSynchronize(s->ev[k]));
@@ -387,6 +626,7 @@ int lbx_serve_create(lbx_ctx* ctx, int precision, double rho0, size_t arena_byte
     e = hipEventCreateWithFlags(&s->slot_ev[k], hipEventDisableTiming);
   for (int k = lbx::kServeMaxRuns - 1; k >= 0; --k) s->free_slots.push_back(k);
   for (int k = 0; k < kInFlight && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming);
+  for (int k = 0; k < kInFlight && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&s->rev[k], hipEventDisableTiming);
   if (e != hipSuccess) {
     lbx_serve_destroy(s);
     return LBX_E_NOMEM;
@@ -410,8 +650,12 @@ int lbx_serve_set_policy(lbx_server* s, int policy, double max_wait_s) {
 int lbx_serve_destroy(lbx_server* s) {
   if (!s) return LBX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (int k = 0; k < kInFlight; ++k)
+  for (Run& r : s->active)
+    if (r.sim) lbx_sim_destroy(r.sim);
+  for (int k = 0; k < kInFlight; ++k) {
     if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    if (s->rev[k]) (void)hipEventDestroy(s->rev[k]);
+  }
   if (s->arena_mem) (void)hipFree(s->arena_mem);
   if (s->staging) (void)hipFree(s->staging);
   if (s->sums) (void)hipFree(s->sums);
@@ -427,18 +671,26 @@ const char* lbx_serve_last_error(const lbx_server* s) { return s ? s->err.c_str(
 
 int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
   if (!s || !req) return LBX_E_ARG;
-  const long long max_cells = (long long)(s->staging_bytes / (lbx::NV_TOTAL * sizeof(double)));
+  const bool ref = req->n_fine_boxes > 0;
+  // single-level runs pass their fields through the staging buffer
+  const long long max_cells =
+      ref ? (1ll << 31) : (long long)(s->staging_bytes / (lbx::NV_TOTAL * sizeof(double)));
   long long n = 1;
   for (int d = 0; d < 3; ++d) {
     if (req->dims[d] < 1) return sfail(s, LBX_E_ARG, "dims must be positive");
     n *= req->dims[d];  // <= max_cells after every factor, so no overflow
     if (n > max_cells) return sfail(s, LBX_E_ARG, "run exceeds the serving staging size (%lld cells)", max_cells);
   }
+  if (req->n_fine_boxes < 0 || req->n_boxes < 0) return sfail(s, LBX_E_ARG, "negative box count");
+  if ((ref && !req->fine_lohi) || (req->n_boxes > 0 && !req->box_lohi))
+    return sfail(s, LBX_E_ARG, "box count without a box list");
+  if (req->n_boxes > 0 && !ref)
+    return sfail(s, LBX_E_ARG, "level-0 boxes are taken for refined runs only (single-level runs are one lattice)");
   if (req->steps < 1) return sfail(s, LBX_E_ARG, "steps must be >= 1");
   if (!(req->tau_s > -0.5) || !(req->tau_b > -0.5) || !std::isfinite(req->tau_s) || !std::isfinite(req->tau_b))
     return sfail(s, LBX_E_ARG, "tau must be finite and > -1/2");
   const lbx::Lattice L = lbx::make_lattice(req->dims[0], req->dims[1], req->dims[2]);
-  if ((size_t)2 * L.elems() * esize(s->precision) > s->arena.size)
+  if (!ref && (size_t)2 * L.elems() * esize(s->precision) > s->arena.size)
     return sfail(s, LBX_E_NOMEM, "run does not fit the serving arena");
   if (req->init_kind != LBX_INIT_PLANE_PULSE && req->init_kind != LBX_INIT_FIELDS)
     return sfail(s, LBX_E_ARG, "unknown init kind %d", req->init_kind);
@@ -457,6 +709,10 @@ int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
   }
   q.req.rho = nullptr;
   q.req.u = nullptr;
+  if (req->n_boxes > 0) q.boxes.assign(req->box_lohi, req->box_lohi + 6 * (size_t)req->n_boxes);
+  if (ref) q.fine.assign(req->fine_lohi, req->fine_lohi + 6 * (size_t)req->n_fine_boxes);
+  q.req.box_lohi = nullptr;
+  q.req.fine_lohi = nullptr;
   q.t_submit = now_s();
   if (id) *id = q.id;
   s->queue.push_back(std::move(q));

@@ -70,7 +70,9 @@ class Request(ctypes.Structure):
                 ("tau_s", ctypes.c_double), ("tau_b", ctypes.c_double),
                 ("init_kind", ctypes.c_int32), ("axis", ctypes.c_int32), ("plane", ctypes.c_int32),
                 ("amplitude", ctypes.c_double),
-                ("rho", _dp), ("u", _dp), ("out_rho", _dp), ("out_u", _dp), ("out_f", _dp)]
+                ("rho", _dp), ("u", _dp), ("out_rho", _dp), ("out_u", _dp), ("out_f", _dp),
+                ("n_boxes", ctypes.c_int32), ("box_lohi", _i32p),
+                ("n_fine_boxes", ctypes.c_int32), ("fine_lohi", _i32p)]
 
 
 class Result(ctypes.Structure):
@@ -444,9 +446,11 @@ class Server:
             raise LbxError(rc, self._lib.lbx_serve_last_error(self.h).decode())
 
     def submit(self, dims, steps, tau_s, tau_b=None, amplitude=0.0, axis=0, plane=None,
-               rho=None, u=None, want_rho=False, want_f=False):
+               rho=None, u=None, want_rho=False, want_f=False, boxes=None, fine_boxes=None):
         """Queue one run; returns (id, outputs) where outputs holds the numpy
-        arrays the server fills at completion."""
+        arrays the server fills at completion.  fine_boxes (fine index space)
+        makes it a two-level run over level-0 `boxes` (default: one box);
+        outputs are then level 0's."""
         nx, ny, nz = (int(d) for d in dims)
         r = Request()
         r.dims[:] = (nx, ny, nz)
@@ -474,6 +478,14 @@ class Server:
         if want_f:
             outs["f"] = np.empty((15, nz, ny, nx))
             r.out_f = _ptr(outs["f"], ctypes.c_double)
+        if boxes:
+            keep["boxes"] = _lohi(boxes)
+            r.n_boxes = len(boxes)
+            r.box_lohi = _ptr(keep["boxes"], ctypes.c_int32)
+        if fine_boxes:
+            keep["fine"] = _lohi(fine_boxes)
+            r.n_fine_boxes = len(fine_boxes)
+            r.fine_lohi = _ptr(keep["fine"], ctypes.c_int32)
         rid = ctypes.c_uint64()
         self._check(self._lib.lbx_serve_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
         keep.update(outs)

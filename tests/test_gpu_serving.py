@@ -1,6 +1,6 @@
 """Batched serving (lbx_serve_*): each run's result is the same alone and in a
-mix (bitwise), FP64 serving matches the oracle, admission respects the
-arena, and bad requests are refused."""
+mix (bitwise), refined (two-level) runs included, FP64 serving matches the
+oracle, admission respects the arena, and bad requests are refused."""
 import numpy as np
 import pytest
 
@@ -53,6 +53,96 @@ def test_mix_equals_each_run_alone(ctx, precision):
         assert bitwise_equal(o["f"], f_alone), (dims, steps)
         assert res[rid]["steps"] == steps and res[rid]["cells"] == np.prod(dims)
         assert abs(res[rid]["mass"] - o["rho"].sum()) <= 1e-9 * res[rid]["mass"]
+
+
+def fine_boxes_in(lo, hi, edge):
+    """Fine boxes (fine index space) tiling fine cells lo..hi in cubes of `edge`."""
+    out = []
+    for z in range(lo[2], hi[2] + 1, edge):
+        for y in range(lo[1], hi[1] + 1, edge):
+            for x in range(lo[0], hi[0] + 1, edge):
+                out.append(((x, y, z), (min(x + edge, hi[0] + 1) - 1, min(y + edge, hi[1] + 1) - 1,
+                                        min(z + edge, hi[2] + 1) - 1)))
+    return out
+
+
+REFINED = [  # dims, level-0 max grid size (0: one box), fine region (fine index space), steps, tau, amplitude
+    ((32, 24, 16), 8, ((16, 8, 8), (47, 39, 23)), 6, 0.8, 1e-2),
+    ((24, 24, 24), 0, ((8, 8, 8), (39, 31, 39)), 9, 0.7, 3e-3),
+]
+
+
+def alone_refined(ctx, dims, mgs, region, steps, tau, amp, precision):
+    """The same two-level run through lbx_sim + lbx_iterate: level-0 f and sums."""
+    boxes = workloads.chop(dims, mgs) if mgs else [((0, 0, 0), tuple(d - 1 for d in dims))]
+    fine = fine_boxes_in(*region, 16)
+    sim = lbx.Sim(ctx, dims, precision=precision)
+    try:
+        sim.define_level(0, boxes)
+        sim.set_tau(0, tau)
+        rho, u = workloads.plane_pulse(dims, amp, axis=0, plane=dims[0] // 2)
+        sim.init_from_fields(rho, u)
+        sim.define_level(1, fine)
+        sim.set_tau(1, 2 * (tau - 0.5) + 0.5)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(fine)
+        sim.iterate(steps)
+        return sim.get_f(0), sim.conservation(0), sim.get_hydro(0)[0]
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("precision", [lbx.LBX_FP32_SHIFTED, lbx.LBX_FP64])
+def test_refined_runs_in_a_mix_equal_each_run_alone(ctx, precision):
+    """Two-level runs served beside single-level ones: each refined run's
+    level-0 populations, density and sums equal the same hierarchy run
+    alone through lbx_iterate, bit for bit; the single-level runs still
+    equal theirs."""
+    srv = lbx.Server(ctx, precision=precision, arena_bytes=1 << 30)
+    singles, refined = {}, {}
+    for k, (dims, steps, tau, amp) in enumerate(MIX[:4]):
+        rid, o = srv.submit(dims, steps, tau, amplitude=amp, axis=0, want_f=True)
+        singles[rid] = (dims, steps, tau, amp, o)
+        if k < len(REFINED):
+            dims_r, mgs, region, steps_r, tau_r, amp_r = REFINED[k]
+            boxes = workloads.chop(dims_r, mgs) if mgs else None
+            rid, o = srv.submit(dims_r, steps_r, tau_r, amplitude=amp_r, axis=0, want_f=True, want_rho=True,
+                                boxes=boxes, fine_boxes=fine_boxes_in(*region, 16))
+            refined[rid] = (REFINED[k], o)
+    srv.drain()
+    res = {r["id"]: r for r in srv.poll()}
+    stats = srv.stats()
+    srv.close()
+    assert sorted(res) == sorted(list(singles) + list(refined))
+    assert all(r["status"] == lbx.LBX_OK for r in res.values())
+    for rid, (dims, steps, tau, amp, o) in singles.items():
+        assert bitwise_equal(o["f"], alone_plain(ctx, dims, steps, tau, amp, precision)), (dims, steps)
+    lu = sum(np.prod(d) * st for d, st, _, _, _ in singles.values())
+    for rid, (case, o) in refined.items():
+        dims, mgs, region, steps, tau, amp = case
+        f0, sums, rho0 = alone_refined(ctx, dims, mgs, region, steps, tau, amp, precision)
+        assert bitwise_equal(o["f"], f0), case
+        assert bitwise_equal(o["rho"], rho0), case
+        assert bitwise_equal(np.array([res[rid]["mass"], *res[rid]["momentum"]]), sums), case
+        assert res[rid]["steps"] == steps and res[rid]["cells"] == np.prod(dims)
+        fine_cells = np.prod([h - l + 1 for l, h in zip(*region)])
+        lu += (np.prod(dims) + 2 * fine_cells) * steps
+    assert stats["lattice_updates"] == lu
+
+
+def test_refined_request_with_bad_boxes_is_delivered_as_an_error(ctx):
+    """A fine box that is not aligned to ratio 2 fails when the run is set
+    up: the request comes back with an error status and the server goes on
+    with the others."""
+    srv = lbx.Server(ctx, precision=lbx.LBX_FP64, arena_bytes=64 << 20)
+    bad, _ = srv.submit((16, 16, 16), 3, 0.8, amplitude=1e-3, fine_boxes=[((1, 0, 0), (8, 7, 7))])
+    good, o = srv.submit((16, 16, 16), 3, 0.8, amplitude=1e-3, want_f=True)
+    srv.drain()
+    res = {r["id"]: r for r in srv.poll()}
+    srv.close()
+    assert res[bad]["status"] == lbx.LBX_E_ARG
+    assert res[good]["status"] == lbx.LBX_OK
+    assert bitwise_equal(o["f"], alone_plain(ctx, (16, 16, 16), 3, 0.8, 1e-3, lbx.LBX_FP64))
 
 
 def test_fp64_serving_matches_oracle(ctx):
