@@ -14,9 +14,11 @@
 //                         now, piecewise constant) + zero comp 0 on the outer
 //                         ghost layer + collide valid fine cells
 //   k_fine_stream_collide Stream(1) (valid + 1 ghost layer; outer layer of the
-//                         fresh array defined as 0) + collide valid cells
+//                         fresh array defined as 0, not stored) + collide
+//                         valid cells
 //   k_fine_stream_avg     Stream(1) + ZeroInvalidComponents(1), and the 1/8
-//                         weighted sum of every 2x2x2 block of the result
+//                         weighted sum of every 2x2x2 block of the result;
+//                         valid cells stored (the next fill rewrites ghosts)
 //   k_coarse_push         CoarseCollide(0) (masked cells -> 0) + Stream(0) in
 //                         push form; pulls across a box face read the
 //                         pre-collision ghost copies, as the reference's
@@ -180,9 +182,39 @@ __device__ __forceinline__ bool fine_pull(const PatchDesc& P, const S* __restric
   return patch_valid(P, gx, gy, gz);
 }
 
+// fine_pull for an input whose outer ghost layer is not stored (the first
+// substep's output, k_fine_stream_collide): a population whose upstream cell
+// lies on the outer layer takes that layer's defined value, zero, without a
+// load.
+template <typename S>
+__device__ __forceinline__ bool fine_pull_inner(const PatchDesc& P, const S* __restrict__ in, long long base,
+                                                int i, int gx, int gy, int gz, double rho0, S* f) {
+  if (patch_outer(P, gx, gy, gz)) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) f[q] = zero_pop<S>(q, rho0);
+    return false;
+  }
+  const int sx = 1, sy = P.g[0], sz = P.g[0] * P.g[1];
+  const bool valid = patch_valid(P, gx, gy, gz);
+  if (valid) {  // every upstream cell is valid or on the first ghost layer
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      f[q] = in[base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
+  } else {
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+      f[q] = patch_outer(P, gx - kC[q][0], gy - kC[q][1], gz - kC[q][2])
+                 ? zero_pop<S>(q, rho0)
+                 : in[base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
+  }
+  return valid;
+}
+
 // First fine substep's stream + the second substep's collision: pull, then
 // collide valid cells.  `out` holds the patch at element P.base - oshift (a
-// group-sized scratch array when oshift is the group's first base).
+// group-sized scratch array when oshift is the group's first base).  The
+// outer ghost layer of the result is zero by definition (Stream's fresh
+// array) and is not stored: the next pass reads it as zero.
 template <typename S>
 __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict__ in, S* __restrict__ out,
                                                              const PatchDesc* __restrict__ patches, int first,
@@ -192,6 +224,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
+  if (patch_outer(P, gx, gy, gz)) return;
   S f[NV];
   if (fine_pull(P, in, P.base, i, gx, gy, gz, r.rho0, f)) collide_s(f, r);
   const long long ob = P.base - oshift;
@@ -268,6 +301,10 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
 // last step of its parent's step) the stream is k_fine_stream_zero's, the
 // child's block averages (slot, cavg, start, contrib) added into valid
 // cells, and the block sums are those of the level's resulting state.
+// Without SUM (the finest level's second substep) the input's outer ghost
+// layer is not stored and reads as zero, and only valid cells are stored:
+// the ghost cells of the level's state are rewritten by the next
+// fill-collide before anything reads them.
 constexpr int kAvgBlocks = 32;
 
 template <typename S, bool SUM = false>
@@ -294,7 +331,9 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     const int gx = 2 * cx + di, gy = 2 * cy + dj, gz = 2 * cz + dk;
     const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
     S f[NV];
-    if (fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f)) {
+    const bool valid = SUM ? fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f)
+                           : fine_pull_inner(P, in, P.base - ishift, i, gx, gy, gz, rho0, f);
+    if (valid) {
       if constexpr (SUM) {
         const int su = slot[P.fill_base + i];
         if (su >= 0) add_block_sums(f, su, cavg, cavg_stride, start, contrib, rho0);
@@ -304,7 +343,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      out[P.base + q * P.qs + i] = f[q];
+      if (SUM || valid) out[P.base + q * P.qs + i] = f[q];
       part[q][child][lb] = to_f64(f[q], q, rho0);
     }
   }
