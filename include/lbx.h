@@ -137,6 +137,19 @@ int lbx_cluster_tags_allowed(const int32_t dims[3], const uint8_t* tags, const u
  * unmasked. */
 int lbx_level_clear(lbx_sim* sim, int level);
 
+/* Refinement ratio of level 1 to level 0: 2 (the default, the reference's
+ * constructor fixes it, src/AmrSim.cpp:766), 3 or 4; set before level 1 is
+ * defined.  Level 1's boxes are then given in the index space ratio * dims,
+ * aligned to the ratio; their parents are floor(pos / ratio), the mask
+ * boxes are coarsened by the ratio, the average-down runs over ratio^3
+ * blocks times 1/ratio^3 over each box coarsened and grown by 2 / ratio
+ * coarse cells (AMReX's sum_fine_to_coarse), and level 1 takes `ratio`
+ * substeps per level-0 step (the reference writes ratio 2's two substeps
+ * out, src/AmrSim.cpp:446-452, 621-624).  The caller sets level 1's tau as
+ * ComputeDt does at that ratio (src/AmrSim.cpp:308-315).  Levels below
+ * level 1 need ratio 2 (LBX_E_UNSUPPORTED otherwise). */
+int lbx_set_ref_ratio(lbx_sim* sim, int ratio);
+
 /* Relaxation rates of a level, omega = 1/(tau + 1/2) computed by the caller
  * exactly as src/AmrSim.cpp:125-126. */
 int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b);

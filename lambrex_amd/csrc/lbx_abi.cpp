@@ -48,6 +48,7 @@ struct Box {
 struct Level {
   bool defined = false;
   bool patched = false;  // per-box ghosted storage (levels >= 1)
+  int ratio = 2;         // refinement ratio to the parent (patched levels)
   std::vector<Box> boxes;
   Lattice lat{};         // level-array geometry (level 0)
   void* buf[3] = {nullptr, nullptr, nullptr};  // state in buf[cur]; buf[2] scratch (patched)
@@ -152,6 +153,9 @@ struct lbx_sim {
   unsigned walls = 0;  // bit d: bounce-back walls normal to axis d
   int precision = LBX_FP64;
   double rho0 = 1.0;
+  // refinement ratio of level 1 to level 0 (2, 3 or 4; deeper levels only
+  // at 2)
+  int ratio = 2;
   Level lev[LBX_MAX_LEVELS];
   StepGraphs graphs;
   bool use_graphs = true;
@@ -216,7 +220,8 @@ int wrap(int v, int n) {
   const int r = v % n;
   return r < 0 ? r + n : r;
 }
-int floor_div2(int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+int floor_div(int v, int r) { return v >= 0 ? v / r : -((-v + r - 1) / r); }
+int floor_div2(int v) { return floor_div(v, 2); }
 
 template <typename T>
 hipError_t upload(T** dptr, const std::vector<T>& h, hipStream_t st) {
@@ -468,7 +473,8 @@ void free_levels_from(lbx_sim* sim, int level) {
 // arrays (≈ 2 x 2 x 48 MB) then stay in the 256 MB MALL.
 constexpr long long kFineGroupBytes = 48ll << 20;
 
-// The finest level's two substeps, from its parent's current state
+// The finest level's substeps (two at ratio 2, r at ratio r), from its
+// parent's current state
 // (level 0's `now`, or a middle level's state, which stays put until its own
 // stream).  Its block averages land in F.d_avg; fpar is the buffer holding
 // its state and flips.
@@ -510,13 +516,34 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
       mid = static_cast<char*>(ftmp) + (k & 1) * F.region_span * elem_size(prec);
       shift = F.gdesc[first].base;
     }
-    e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, cpatches, F.d_desc, F.d_gdesc, first, n,
-                                      F.max_cells, F.d_fill, nullptr, rf, gs);
-    if (e == hipSuccess)
-      e = lbx::launch_fine_stream_collide(prec, fnext, mid, shift, F.d_gdesc, first, n, F.max_cells, rf, gs);
-    if (e == hipSuccess)
-      e = lbx::launch_fine_stream_avg(prec, mid, shift, fnext, F.d_avg, F.avg_stride, F.d_gdesc, first, n,
-                                      F.max_blocks, sim->rho0, gs);
+    if (F.ratio == 2) {
+      e = lbx::launch_fine_fill_collide(prec, fnow, fnext, cnow, C.lat, cpatches, F.d_desc, F.d_gdesc, first, n,
+                                        F.max_cells, F.d_fill, nullptr, rf, gs);
+      if (e == hipSuccess)
+        e = lbx::launch_fine_stream_collide(prec, fnext, mid, shift, F.d_gdesc, first, n, F.max_cells, rf, gs);
+      if (e == hipSuccess)
+        e = lbx::launch_fine_stream_avg(prec, mid, shift, fnext, F.d_avg, F.avg_stride, F.d_gdesc, first, n,
+                                        F.max_blocks, sim->rho0, gs);
+    } else {
+      // ratio r > 2: r substeps, each a collide then a stream, chained
+      // through buf[2] (whole, no regions) and fnext so that the last one
+      // reads buf[2]: the fill-collide writes stage 1, the k-th
+      // stream-collide reads stage k and writes stage k+1, stage k in buf[2]
+      // when r - k is even; then the last stream into fnext and the
+      // average-down of its valid cells.
+      const int r = F.ratio;
+      auto stage = [&](int k) { return ((r - k) % 2 == 0) ? mid : fnext; };
+      e = lbx::launch_fine_fill_collide(prec, fnow, stage(1), cnow, C.lat, cpatches, F.d_desc, F.d_gdesc, first, n,
+                                        F.max_cells, F.d_fill, nullptr, rf, gs);
+      for (int k = 1; k < r && e == hipSuccess; ++k)
+        e = lbx::launch_fine_stream_collide(prec, stage(k), stage(k + 1), 0, F.d_gdesc, first, n, F.max_cells, rf, gs,
+                                            k > 1);
+      if (e == hipSuccess)
+        e = lbx::launch_fine_stream_valid(prec, mid, fnext, F.d_gdesc, first, n, F.max_cells, sim->rho0, gs);
+      if (e == hipSuccess)
+        e = lbx::launch_fine_avg_valid(prec, fnext, F.d_avg, F.avg_stride, F.d_gdesc, first, n, F.max_blocks, r,
+                                       sim->rho0, gs);
+    }
   }
   if (fork) {  // join even after a failed launch, so a capture can end
     const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
@@ -683,8 +710,9 @@ int define_level0(lbx_sim* sim, const std::vector<Box>& boxes) {
 // leaves the cell's level-0 box (the coarse push then streams the
 // pre-collision value, as the reference's pull from stale ghost copies
 // does); bit 7 = the cell lies under one of the mask boxes (fine index
-// space) and is zeroed instead of collided.
-std::vector<unsigned char> coarse_flags(const Level& C, const std::vector<Box>& mask) {
+// space, coarsened by the refinement ratio r) and is zeroed instead of
+// collided.
+std::vector<unsigned char> coarse_flags(const Level& C, const std::vector<Box>& mask, int r) {
   const Lattice& L = C.lat;
   std::vector<unsigned char> fl((size_t)L.nx * L.ny * L.nz, 0);
   auto at = [&](int x, int y, int z) -> unsigned char& { return fl[((size_t)z * L.ny + y) * L.nx + x]; };
@@ -695,9 +723,9 @@ std::vector<unsigned char> coarse_flags(const Level& C, const std::vector<Box>& 
           at(x, y, z) = (unsigned char)((x == b.hi[0] ? 1 : 0) | (x == b.lo[0] ? 2 : 0) | (y == b.hi[1] ? 4 : 0) |
                                         (y == b.lo[1] ? 8 : 0) | (z == b.hi[2] ? 16 : 0) | (z == b.lo[2] ? 32 : 0));
   for (const Box& b : mask)
-    for (int z = std::max(0, floor_div2(b.lo[2])); z <= std::min(L.nz - 1, floor_div2(b.hi[2])); ++z)
-      for (int y = std::max(0, floor_div2(b.lo[1])); y <= std::min(L.ny - 1, floor_div2(b.hi[1])); ++y)
-        for (int x = std::max(0, floor_div2(b.lo[0])); x <= std::min(L.nx - 1, floor_div2(b.hi[0])); ++x)
+    for (int z = std::max(0, floor_div(b.lo[2], r)); z <= std::min(L.nz - 1, floor_div(b.hi[2], r)); ++z)
+      for (int y = std::max(0, floor_div(b.lo[1], r)); y <= std::min(L.ny - 1, floor_div(b.hi[1], r)); ++y)
+        for (int x = std::max(0, floor_div(b.lo[0], r)); x <= std::min(L.nx - 1, floor_div(b.hi[0], r)); ++x)
           at(x, y, z) |= 0x80;
   return fl;
 }
@@ -764,18 +792,23 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   Level& C = sim->lev[level - 1];
   if (!C.defined) return fail(ctx, LBX_E_STATE, "define level %d before level %d", level - 1, level);
   if (sim->walls) return fail(ctx, LBX_E_UNSUPPORTED, "refined levels need a fully periodic domain");
+  if (level >= 2 && sim->ratio != 2)
+    return fail(ctx, LBX_E_UNSUPPORTED, "levels below level 1 need refinement ratio 2 (this simulation has %d)",
+                sim->ratio);
+  const int r = level == 1 ? sim->ratio : 2;
+  const long long scale = level == 1 ? r : (1ll << level);
   int fdom[3];
   for (int d = 0; d < 3; ++d) {
-    if (((long long)sim->dims[d] << level) > (1ll << 30))
+    if ((long long)sim->dims[d] * scale > (1ll << 30))
       return fail(ctx, LBX_E_ARG, "level %d's index space exceeds 2^30 cells per direction", level);
-    fdom[d] = sim->dims[d] << level;
+    fdom[d] = (int)(sim->dims[d] * scale);
   }
   for (size_t b = 0; b < boxes.size(); ++b) {
     if (!box_ok(boxes[b], fdom))
       return fail(ctx, LBX_E_ARG, "level-%d box %zu lies outside the refined domain or is empty", level, b);
     for (int d = 0; d < 3; ++d)
-      if (boxes[b].lo[d] % 2 != 0 || boxes[b].hi[d] % 2 != 1)
-        return fail(ctx, LBX_E_ARG, "level-%d box %zu is not aligned to refinement ratio 2", level, b);
+      if (boxes[b].lo[d] % r != 0 || (boxes[b].hi[d] + 1) % r != 0)
+        return fail(ctx, LBX_E_ARG, "level-%d box %zu is not aligned to refinement ratio %d", level, b, r);
   }
   if (any_overlap(boxes)) return fail(ctx, LBX_E_ARG, "level-%d boxes overlap", level);
   if (boxes.size() > 65535)  // one grid row of workgroups per patch
@@ -785,9 +818,9 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   int chint = 0;
   if (C.patched)
     for (size_t b = 0; b < boxes.size(); ++b)
-      for (int z = boxes[b].lo[2] / 2; z <= boxes[b].hi[2] / 2; ++z)
-        for (int y = boxes[b].lo[1] / 2; y <= boxes[b].hi[1] / 2; ++y)
-          for (int x = boxes[b].lo[0] / 2; x <= boxes[b].hi[0] / 2; ++x) {
+      for (int z = boxes[b].lo[2] / r; z <= boxes[b].hi[2] / r; ++z)
+        for (int y = boxes[b].lo[1] / r; y <= boxes[b].hi[1] / r; ++y)
+          for (int x = boxes[b].lo[0] / r; x <= boxes[b].hi[0] / r; ++x) {
             long long ref;
             if (!cell_ref(C, x, y, z, &ref, chint))
               return fail(ctx, LBX_E_ARG, "level-%d box %zu is not properly nested in level %d", level, b,
@@ -796,6 +829,11 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   F = Level{};
   F.boxes = boxes;
   F.patched = true;
+  F.ratio = r;
+  // Average-down blocks per patch: the box coarsened by r and grown by
+  // ng = 2 / r coarse cells (AMReX's sum_fine_to_coarse: the fine ghosts
+  // over the ratio), c[d] blocks per direction.
+  const int ng = 2 / r;
   long long base = 0, fill_base = 0, avg_base = 0;
   for (const Box& b : boxes) {
     PatchDesc d{};
@@ -813,7 +851,7 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
     d.qs = (cells + 63) / 64 * 64;
     d.fill_base = fill_base;
     d.avg_base = avg_base;
-    const int blocks = (d.g[0] / 2) * (d.g[1] / 2) * (d.g[2] / 2);
+    const int blocks = (b.n(0) / r + 2 * ng) * (b.n(1) / r + 2 * ng) * (b.n(2) / r + 2 * ng);
     fill_base += cells;
     avg_base += blocks;
     F.max_cells = std::max(F.max_cells, d.cells);
@@ -853,7 +891,9 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
       F.region_span = std::max(F.region_span, elems);
       k += n;
     }
-    F.regions = F.groups.size() > 1 && 2 * F.region_span <= base;  // base: elements of one level buffer
+    // per-stream regions of buf[2] for the array between the two substeps;
+    // at a ratio above 2 the substeps chain through the whole buf[2]
+    F.regions = r == 2 && F.groups.size() > 1 && 2 * F.region_span <= base;  // base: one level buffer's elements
   }
   // Ghost fill sources (FillPatchTwoLevels with piecewise-constant
   // interpolation, src/AmrSim.cpp:373-388): the valid cell of this level
@@ -884,7 +924,7 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
             d = ((long long)src << 32) | grown_index(F.desc[src], wx, wy, wz);
           } else {
             long long ref;
-            if (!cell_ref(C, wx / 2, wy / 2, wz / 2, &ref, chint)) {
+            if (!cell_ref(C, wx / r, wy / r, wz / r, &ref, chint)) {
               F = Level{};
               return fail(ctx, LBX_E_ARG, "level-%d box %zu is not properly nested in level %d", level, b,
                           level - 1);
@@ -904,16 +944,16 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
     long long cell;  // parent cell reference
     int avg;         // block-average index
   };
-  const int cdom[3] = {sim->dims[0] << (level - 1), sim->dims[1] << (level - 1), sim->dims[2] << (level - 1)};
+  const int cdom[3] = {fdom[0] / r, fdom[1] / r, fdom[2] / r};
   std::vector<Contrib> list;
   for (size_t b = 0; b < boxes.size(); ++b) {
     const Box& B = boxes[b];
     const PatchDesc& P = F.desc[b];
-    const int c0 = P.g[0] / 2, c1 = P.g[1] / 2;
-    const int i0 = floor_div2(B.lo[0]) - 1, j0 = floor_div2(B.lo[1]) - 1, k0 = floor_div2(B.lo[2]) - 1;
-    for (int ci = i0; ci <= floor_div2(B.hi[0]) + 1; ++ci)
-      for (int cj = j0; cj <= floor_div2(B.hi[1]) + 1; ++cj)
-        for (int ck = k0; ck <= floor_div2(B.hi[2]) + 1; ++ck) {
+    const int c0 = B.n(0) / r + 2 * ng, c1 = B.n(1) / r + 2 * ng;
+    const int i0 = floor_div(B.lo[0], r) - ng, j0 = floor_div(B.lo[1], r) - ng, k0 = floor_div(B.lo[2], r) - ng;
+    for (int ci = i0; ci <= floor_div(B.hi[0], r) + ng; ++ci)
+      for (int cj = j0; cj <= floor_div(B.hi[1], r) + ng; ++cj)
+        for (int ck = k0; ck <= floor_div(B.hi[2], r) + ng; ++ck) {
           long long ref;
           if (!cell_ref(C, wrap(ci, cdom[0]), wrap(cj, cdom[1]), wrap(ck, cdom[2]), &ref, chint)) continue;
           const long long blk = ((long long)(ck - k0) * c1 + (cj - j0)) * c0 + (ci - i0);
@@ -982,7 +1022,7 @@ int unmask_level(lbx_sim* sim, int level) {
   lbx_ctx* ctx = sim->ctx;
   Level& C = sim->lev[level];
   if (!C.defined) return LBX_OK;
-  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, {}) : coarse_flags(C, {});
+  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, {}) : coarse_flags(C, {}, sim->ratio);
   if (!C.d_mask) LBX_HIP(ctx, hipMalloc(&C.d_mask, bytes.size()));
   LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, bytes.data(), bytes.size(), hipMemcpyHostToDevice, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1023,7 +1063,7 @@ int remake_level(lbx_sim* sim, int level, const std::vector<Box>& boxes) {
       for (int y = P.lo[1]; y <= P.hi[1]; ++y)
         for (int x = P.lo[0]; x <= P.hi[0]; ++x) {
           long long ref = 0;
-          (void)cell_ref(C, x / 2, y / 2, z / 2, &ref, chint);  // nesting checked by build_level
+          (void)cell_ref(C, x / nf.ratio, y / nf.ratio, z / nf.ratio, &ref, chint);  // nesting checked by build_level
           src[P.fill_base + grown_index(P, x, y, z)] = -1 - ref;
         }
     for (size_t o = 0; o < F.boxes.size(); ++o) {
@@ -1077,7 +1117,7 @@ int fill_from_patched(lbx_sim* sim, int level) {
       for (int y = P.lo[1]; y <= P.hi[1]; ++y)
         for (int x = P.lo[0]; x <= P.hi[0]; ++x) {
           long long ref;
-          if (!cell_ref(C, x / 2, y / 2, z / 2, &ref, chint))
+          if (!cell_ref(C, x / F.ratio, y / F.ratio, z / F.ratio, &ref, chint))
             return fail(ctx, LBX_E_STATE, "level %d is not nested in level %d", level, level - 1);
           src[P.fill_base + grown_index(P, x, y, z)] = -1 - ref;
         }
@@ -1322,6 +1362,15 @@ int lbx_level_clear(lbx_sim* sim, int level) {
   return level > 0 ? unmask_level(sim, level - 1) : LBX_OK;
 }
 
+int lbx_set_ref_ratio(lbx_sim* sim, int ratio) {
+  if (int rc = check_sim(sim)) return rc;
+  if (ratio < 2 || ratio > 4) return fail(sim->ctx, LBX_E_ARG, "refinement ratio %d is not 2, 3 or 4", ratio);
+  if (sim->lev[1].defined)
+    return fail(sim->ctx, LBX_E_STATE, "the refinement ratio is set before level 1 is defined");
+  sim->ratio = ratio;
+  return LBX_OK;
+}
+
 int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b) {
   if (int rc = check_sim(sim)) return rc;
   if (level < 0 || level >= LBX_MAX_LEVELS) return fail(sim->ctx, LBX_E_ARG, "level %d out of range", level);
@@ -1342,7 +1391,7 @@ int lbx_set_level_mask(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi
   if (!C.defined) return fail(ctx, LBX_E_STATE, "level %d is not defined", level);
   if (n_boxes < 0 || (n_boxes > 0 && !lohi_fine)) return fail(ctx, LBX_E_ARG, "bad mask boxes");
   const std::vector<Box> mask = read_boxes(n_boxes, lohi_fine);
-  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, mask) : coarse_flags(C, mask);
+  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, mask) : coarse_flags(C, mask, sim->ratio);
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (!C.d_mask) {
@@ -1370,7 +1419,7 @@ int lbx_fill_from_coarse(lbx_sim* sim, int level) {
   if (int rc = materialise(sim)) return rc;
   if (C.patched) return fill_from_patched(sim, level);
   LBX_HIP(ctx, lbx::launch_fine_from_coarse(sim->precision, F.buf[F.cur], C.buf[C.cur], C.lat, F.d_desc,
-                                            (int)F.boxes.size(), F.max_cells, sim->rho0, ctx->stream));
+                                            (int)F.boxes.size(), F.max_cells, sim->rho0, F.ratio, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return LBX_OK;
 }

@@ -151,9 +151,20 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
                                     hipStream_t st);
 // out holds each patch at element P.base - oshift (0: a level buffer; the
 // group's first base: a group-sized scratch array).
+// inner: `in` is itself such an output (outer layer not stored, read as
+// zero), for the later substeps at a refinement ratio above 2.
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, long long oshift,
                                       const PatchDesc* patches, int first, int npatch, int max_cells, const Relax& r,
-                                      hipStream_t st);
+                                      hipStream_t st, bool inner = false);
+// The finest level's last substep at a refinement ratio above 2: stream of
+// the valid cells (input outer layer read as zero) into out ...
+hipError_t launch_fine_stream_valid(int precision, const void* in, void* out, const PatchDesc* patches, int first,
+                                    int npatch, int max_cells, double rho0, hipStream_t st);
+// ... and the average-down of the result over ratio^3 blocks of valid cells
+// (children in (k, j, i) order, times 1/ratio^3) into avg.
+hipError_t launch_fine_avg_valid(int precision, const void* f, double* avg, long long avg_stride,
+                                 const PatchDesc* patches, int first, int npatch, int max_blocks, int ratio,
+                                 double rho0, hipStream_t st);
 // Stream + ZeroInvalidComponents of a middle level (no collide), with the
 // child's block averages added into valid cells: slot[fill_base + i] is the
 // touched-cell index of grown cell i (-1 if none), start/contrib as in
@@ -185,7 +196,7 @@ hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stri
                             const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
                             hipStream_t st);
 hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, const Lattice& clat,
-                                   const PatchDesc* patches, int npatch, int max_cells, double rho0,
+                                   const PatchDesc* patches, int npatch, int max_cells, double rho0, int ratio,
                                    hipStream_t st);
 // Valid cells of new patches from old patches or the coarse parent
 // (src per new grown cell: (old patch << 32) | old grown index, or a coarse

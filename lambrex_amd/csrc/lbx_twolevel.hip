@@ -214,8 +214,10 @@ __device__ __forceinline__ bool fine_pull_inner(const PatchDesc& P, const S* __r
 // collide valid cells.  `out` holds the patch at element P.base - oshift (a
 // group-sized scratch array when oshift is the group's first base).  The
 // outer ghost layer of the result is zero by definition (Stream's fresh
-// array) and is not stored: the next pass reads it as zero.
-template <typename S>
+// array) and is not stored: the next pass reads it as zero.  With INNER the
+// input is itself such a result (a later substep at a refinement ratio above
+// 2), whose outer layer is read as zero.
+template <typename S, bool INNER = false>
 __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict__ in, S* __restrict__ out,
                                                              const PatchDesc* __restrict__ patches, int first,
                                                              Relax r, long long oshift) {
@@ -226,7 +228,9 @@ __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict
   patch_coords(P, i, gx, gy, gz);
   if (patch_outer(P, gx, gy, gz)) return;
   S f[NV];
-  if (fine_pull(P, in, P.base, i, gx, gy, gz, r.rho0, f)) collide_s(f, r);
+  const bool valid = INNER ? fine_pull_inner(P, in, P.base, i, gx, gy, gz, r.rho0, f)
+                           : fine_pull(P, in, P.base, i, gx, gy, gz, r.rho0, f);
+  if (valid) collide_s(f, r);
   const long long ob = P.base - oshift;
 #pragma unroll
   for (int q = 0; q < NV; ++q) out[ob + q * P.qs + i] = f[q];
@@ -423,20 +427,68 @@ __global__ __launch_bounds__(256) void k_sum_apply(const double* __restrict__ av
   for (int q = 0; q < NV; ++q) cnext[c + q * stride] = from_f64<S>(acc[q], q, rho0);
 }
 
+// The finest level's last substep at a refinement ratio r above 2: the
+// stream of its valid cells (the input's outer layer read as zero) into
+// `out`.  Its ghost cells need no ZeroInvalidComponents: at such a ratio the
+// average-down reads valid cells only (nGrow = 2 / r = 0) and the next
+// fill-collide rewrites the ghosts.
+template <typename S>
+__global__ __launch_bounds__(256) void k_fine_stream_valid(const S* __restrict__ in, S* __restrict__ out,
+                                                           const PatchDesc* __restrict__ patches, int first,
+                                                           double rho0) {
+  const PatchDesc& P = patches[first + blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.cells) return;
+  int gx, gy, gz;
+  patch_coords(P, i, gx, gy, gz);
+  if (!patch_valid(P, gx, gy, gz)) return;
+  S f[NV];
+  fine_pull_inner(P, in, P.base, i, gx, gy, gz, rho0, f);
+#pragma unroll
+  for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
+}
+
+// The first half of SumFromFine at a refinement ratio r above 2
+// (amrex_avgdown over the valid box only, nGrow = 2 / r = 0): per r x r x r
+// block of valid cells, the FP64 sum of its children in (k, j, i) order
+// times 1/r^3, to avg[q * avg_stride + P.avg_base + block] (blocks x
+// fastest over the coarsened valid box).
+template <typename S>
+__global__ __launch_bounds__(256) void k_fine_avg_valid(const S* __restrict__ f, double* __restrict__ avg,
+                                                        long long avg_stride, const PatchDesc* __restrict__ patches,
+                                                        int first, int r, double rho0) {
+  const PatchDesc& P = patches[first + blockIdx.y];
+  const int c0 = (P.hi[0] - P.lo[0] + 1) / r, c1 = (P.hi[1] - P.lo[1] + 1) / r, c2 = (P.hi[2] - P.lo[2] + 1) / r;
+  const int u = blockIdx.x * 256 + threadIdx.x;
+  if (u >= c0 * c1 * c2) return;
+  const int cx = u % c0, t = u / c0, cy = t % c1, cz = t / c1;
+  const double volfrac = 1.0 / (double)(r * r * r);
+  const int sy = P.g[0], sz = P.g[0] * P.g[1];
+  const long long corner = (long long)(2 + r * cz) * sz + (long long)(2 + r * cy) * sy + (2 + r * cx);
+  for (int q = 0; q < NV; ++q) {
+    double sum = 0.0;
+    for (int dk = 0; dk < r; ++dk)
+      for (int dj = 0; dj < r; ++dj)
+        for (int di = 0; di < r; ++di)
+          sum += to_f64(f[P.base + q * P.qs + corner + (long long)dk * sz + dj * sy + di], q, rho0);
+    avg[q * avg_stride + P.avg_base + u] = sum * volfrac;
+  }
+}
+
 // DistFnFillFromCoarse (src/AmrSim.cpp:393-411): valid fine cells take the
 // coarse value of their parent; ghosts are zeroed (they are rewritten before
 // they are read).
 template <typename S>
 __global__ __launch_bounds__(256) void k_fine_from_coarse(S* __restrict__ fnow, const S* __restrict__ cnow,
                                                           Lattice clat, const PatchDesc* __restrict__ patches,
-                                                          double rho0) {
+                                                          double rho0, int ratio) {
   const PatchDesc P = patches[blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
   if (patch_valid(P, gx, gy, gz)) {
-    const int x = (P.lo[0] - 2 + gx) >> 1, y = (P.lo[1] - 2 + gy) >> 1, z = (P.lo[2] - 2 + gz) >> 1;
+    const int x = (P.lo[0] - 2 + gx) / ratio, y = (P.lo[1] - 2 + gy) / ratio, z = (P.lo[2] - 2 + gz) / ratio;
     const long long c = clat.at(0, y * clat.nx + x, z);
 #pragma unroll
     for (int q = 0; q < NV; ++q) fnow[P.base + q * P.qs + i] = cnow[c + q * clat.ps];
@@ -594,10 +646,16 @@ hipError_t launch_middle_stream_avg(int precision, const void* in, void* out, do
 
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, long long oshift,
                                       const PatchDesc* patches, int first, int npatch, int max_cells, const Relax& r,
-                                      hipStream_t st) {
+                                      hipStream_t st, bool inner) {
   const dim3 grid((max_cells + 255) / 256, npatch);
-  if (precision == 0)
+  if (precision == 0 && inner)
+    k_fine_stream_collide<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, r,
+                                                              oshift);
+  else if (precision == 0)
     k_fine_stream_collide<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, r, oshift);
+  else if (inner)
+    k_fine_stream_collide<float, true><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, r,
+                                                             oshift);
   else
     k_fine_stream_collide<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, r, oshift);
   return hipGetLastError();
@@ -642,13 +700,34 @@ hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stri
 }
 
 hipError_t launch_fine_from_coarse(int precision, void* fnow, const void* cnow, const Lattice& clat,
-                                   const PatchDesc* patches, int npatch, int max_cells, double rho0,
+                                   const PatchDesc* patches, int npatch, int max_cells, double rho0, int ratio,
                                    hipStream_t st) {
   const dim3 grid((max_cells + 255) / 256, npatch);
   if (precision == 0)
-    k_fine_from_coarse<double><<<grid, 256, 0, st>>>((double*)fnow, (const double*)cnow, clat, patches, rho0);
+    k_fine_from_coarse<double><<<grid, 256, 0, st>>>((double*)fnow, (const double*)cnow, clat, patches, rho0, ratio);
   else
-    k_fine_from_coarse<float><<<grid, 256, 0, st>>>((float*)fnow, (const float*)cnow, clat, patches, rho0);
+    k_fine_from_coarse<float><<<grid, 256, 0, st>>>((float*)fnow, (const float*)cnow, clat, patches, rho0, ratio);
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_stream_valid(int precision, const void* in, void* out, const PatchDesc* patches, int first,
+                                    int npatch, int max_cells, double rho0, hipStream_t st) {
+  const dim3 grid((max_cells + 255) / 256, npatch);
+  if (precision == 0)
+    k_fine_stream_valid<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, rho0);
+  else
+    k_fine_stream_valid<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, rho0);
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_avg_valid(int precision, const void* f, double* avg, long long avg_stride,
+                                 const PatchDesc* patches, int first, int npatch, int max_blocks, int ratio,
+                                 double rho0, hipStream_t st) {
+  const dim3 grid((max_blocks + 255) / 256, npatch);
+  if (precision == 0)
+    k_fine_avg_valid<double><<<grid, 256, 0, st>>>((const double*)f, avg, avg_stride, patches, first, ratio, rho0);
+  else
+    k_fine_avg_valid<float><<<grid, 256, 0, st>>>((const float*)f, avg, avg_stride, patches, first, ratio, rho0);
   return hipGetLastError();
 }
 

@@ -15,8 +15,21 @@
 #include <string>
 
 namespace {
-constexpr int kRefRatio = 2;  // ratio 2 on every level (src/AmrSim.cpp:766)
+// ratio 2 on every level (src/AmrSim.cpp:766) unless SetRefRatio changes
+// level 1's
+constexpr int kRefRatio = 2;
 }
+
+void AmrSim::SetRefRatio(int ratio) {
+  if (sim_) throw lambrex::Error("SetRefRatio must be called before InitFromScratch");
+  if (ratio < 2 || ratio > 4) throw lambrex::Error("SetRefRatio: the ratio must be 2, 3 or 4");
+  if (ratio != kRefRatio && max_level_ > 1)
+    throw lambrex::Error("SetRefRatio: a ratio other than 2 needs max_level <= 1");
+  ref_ratio_ = ratio;
+}
+
+// Ratio between level `level` and level `level` + 1.
+int AmrSim::Ratio(int level) const { return level == 0 ? ref_ratio_ : kRefRatio; }
 
 AmrSim::AmrSim(int nx, int ny, int nz, int max_ref_level,
                const std::array<int, NDIMS>& periodicity, double tau_s, double tau_b)
@@ -70,12 +83,14 @@ void AmrSim::SetInitialVelocity(double u_init) { initial_velocity.assign(3 * NUM
 void AmrSim::SetInitialVelocity(std::vector<double> u_init) { initial_velocity = std::move(u_init); }
 
 std::vector<std::array<int, NDIMS>> AmrSim::refRatio() const {
-  return std::vector<std::array<int, NDIMS>>(max_level_ + 1, {kRefRatio, kRefRatio, kRefRatio});
+  std::vector<std::array<int, NDIMS>> out;
+  for (int l = 0; l <= max_level_; ++l) out.push_back({Ratio(l), Ratio(l), Ratio(l)});
+  return out;
 }
 
 std::array<int, NDIMS> AmrSim::refRatio(int level) const {
   if (level < 0 || level > max_level_) throw lambrex::Error("refRatio: level out of range");
-  return {kRefRatio, kRefRatio, kRefRatio};
+  return {Ratio(level), Ratio(level), Ratio(level)};
 }
 
 // src/AmrSim.cpp:297-322
@@ -83,7 +98,7 @@ void AmrSim::ComputeDt(int level) {
   LevelHost& L = levels_.at(level);
   if (level) {
     const LevelHost& C = levels_.at(level - 1);
-    const int r = kRefRatio;
+    const int r = Ratio(level - 1);
     L.dt = C.dt / r;
     mass_.at(level) = mass_.at(level - 1) / r;
     tau_s_.at(level) = r * (tau_s_.at(level - 1) - 0.5) + 0.5;
@@ -176,6 +191,7 @@ void AmrSim::InitFromScratch(double time) {
     const int32_t dims[3] = {NX, NY, NZ};
     const int32_t per[3] = {PERIODICITY[0], PERIODICITY[1], PERIODICITY[2]};
     Check(lbx_sim_create(lambrex::Context(), dims, per, precision_, rho0_, &sim_));
+    if (ref_ratio_ != kRefRatio) Check(lbx_set_ref_ratio(sim_, ref_ratio_));
   }
   lambrex::IBox domain;
   domain.lo = {0, 0, 0};
@@ -244,15 +260,17 @@ void AmrSim::Iterate(int nsteps) {
     if (nsteps % 2) C.cur = 1 - C.cur;  // UpdateNow swaps Density too
   } else {
     // UpdateDistribution (:619-631): a level between 0 and the finest takes
-    // one step per RohdeCycle, 2^l cycles per level-0 step; the finest level
-    // takes two steps per cycle of its parent.
+    // one step per RohdeCycle, 2^l cycles per level-0 step (levels below 1
+    // exist at ratio 2 only); the finest level takes ratio steps per cycle
+    // of its parent (two at ratio 2, as the reference writes out).
     for (int t = 0; t < nsteps; ++t)
       for (int l = 0; l <= finest_level_; ++l) {
         LevelHost& L = levels_.at(l);
         if (l > 0 && l == finest_level_) {
+          const int r = Ratio(l - 1);  // substeps per cycle of its parent
           for (int c = 0; c < (1 << (l - 1)); ++c) {
-            L.time += 2 * L.dt;
-            L.step += 2;
+            L.time += r * L.dt;
+            L.step += r;
           }
         } else {
           for (int c = 0; c < (1 << l); ++c) {
@@ -299,6 +317,7 @@ void AmrSim::MakeNewLevelFromCoarse(int level, double time, const std::vector<la
 void AmrSim::Regrid(int criterion, double threshold, int blocking, int buffer) {
   if (!sim_) throw lambrex::Error("Regrid before InitFromScratch");
   if (max_level_ < 1) throw lambrex::Error("Regrid: max_level must be >= 1");
+  if (ref_ratio_ != kRefRatio) throw lambrex::Error("Regrid: tag clustering refines by ratio 2 only");
   for (auto& t : static_tags_) t = lambrex::IBox{};
   for (int l = 0; l < max_level_ && l <= finest_level_; ++l) {
     const std::vector<lambrex::IBox> fba = RefineFromState(l, criterion, threshold, blocking, buffer);
@@ -457,9 +476,9 @@ void AmrSim::WritePlotfile(const std::string& dir) {
   h << "HyperCLaw-V1.1\n4\ndensity\nx_velocity\ny_velocity\nz_velocity\n3\n" << time << '\n'
     << finest_level_ << '\n';
   h << "0 0 0 \n1 1 1 \n";
-  for (int l = 0; l < finest_level_; ++l) h << kRefRatio << ' ';
+  for (int l = 0; l < finest_level_; ++l) h << Ratio(l) << ' ';
   h << '\n';
-  for (int l = 0, r = 1; l < nlev; ++l, r *= kRefRatio) {
+  for (int l = 0, r = 1; l < nlev; r *= Ratio(l), ++l) {
     lambrex::IBox d;
     d.lo = {0, 0, 0};
     d.hi = {r * NX - 1, r * NY - 1, r * NZ - 1};
@@ -468,10 +487,10 @@ void AmrSim::WritePlotfile(const std::string& dir) {
   h << '\n';
   for (int l = 0; l < nlev; ++l) h << levels_.at(l).step << ' ';
   h << '\n';
-  for (int l = 0, r = 1; l < nlev; ++l, r *= kRefRatio)
+  for (int l = 0, r = 1; l < nlev; r *= Ratio(l), ++l)
     h << 1.0 / (r * NX) << ' ' << 1.0 / (r * NY) << ' ' << 1.0 / (r * NZ) << " \n";
   h << "0\n0\n";
-  for (int l = 0, r = 1; l < nlev; ++l, r *= kRefRatio) {
+  for (int l = 0, r = 1; l < nlev; r *= Ratio(l), ++l) {
     const LevelHost& L = levels_.at(l);
     const int ext[3] = {r * NX, r * NY, r * NZ};
     h << l << ' ' << L.boxes.size() << ' ' << time << '\n' << L.step << '\n';
@@ -594,8 +613,14 @@ void AmrSim::RegridFrom(int base) {
       if (finest_level_ > l) ClearLevel(l + 1);
       return;
     }
-    const std::vector<lambrex::IBox> fba =
-        lambrex::max_size_chop(lambrex::refine(tag, kRefRatio), max_grid_size_);
+    std::vector<lambrex::IBox> fba;
+    if (Ratio(l) == kRefRatio) {
+      fba = lambrex::max_size_chop(lambrex::refine(tag, kRefRatio), max_grid_size_);
+    } else {  // boxes aligned to the ratio: chop in the coarse index space, then refine
+      const int r = Ratio(l);
+      for (const lambrex::IBox& b : lambrex::max_size_chop(tag, std::max(1, max_grid_size_ / r)))
+        fba.push_back(lambrex::refine(b, r));
+    }
     // AmrCore::regrid: a new level is made from the coarse one, an existing
     // level with a different BoxArray is remade (keeping its data; its finer
     // levels are rebuilt above it), an unchanged one is left alone.
@@ -706,7 +731,8 @@ void AmrSim::WriteCheckpoint(const std::string& dir) const {
     << "finest_level " << finest_level_ << '\n'
     << "precision " << precision_ << ' ' << hexd(rho0_) << '\n'
     << "max_grid_size " << max_grid_size_ << '\n'
-    << "mask_from_fine_grids " << (mask_from_fine_grids_ ? 1 : 0) << '\n';
+    << "mask_from_fine_grids " << (mask_from_fine_grids_ ? 1 : 0) << '\n'
+    << "ref_ratio " << ref_ratio_ << '\n';
   for (int l = 0; l <= max_level_; ++l) {
     const lambrex::IBox& t = static_tags_.at(l);
     h << "static_tag " << l << ' ' << t.lo[0] << ' ' << t.lo[1] << ' ' << t.lo[2] << ' ' << t.hi[0] << ' '
@@ -781,6 +807,10 @@ void AmrSim::ReadCheckpoint(const std::string& dir) {
   const long long mgs = r.integer();
   r.expect("mask_from_fine_grids");
   const bool mask_fine = r.integer() != 0;
+  r.expect("ref_ratio");
+  const long long ratio = r.integer();
+  if (ratio < 2 || ratio > 4 || (ratio != kRefRatio && max_level_ > 1))
+    throw lambrex::Error("ReadCheckpoint: bad refinement ratio");
   std::vector<lambrex::IBox> tags(max_level_ + 1);
   for (int l = 0; l <= max_level_; ++l) {
     r.expect("static_tag");
@@ -828,6 +858,7 @@ void AmrSim::ReadCheckpoint(const std::string& dir) {
   }
   precision_ = (int)precision;
   rho0_ = rho0;
+  ref_ratio_ = (int)ratio;
   max_grid_size_ = (int)mgs;
   mask_from_fine_grids_ = mask_fine;
   static_tags_ = tags;
@@ -839,6 +870,7 @@ void AmrSim::ReadCheckpoint(const std::string& dir) {
   const int32_t dims[3] = {NX, NY, NZ};
   const int32_t per[3] = {PERIODICITY[0], PERIODICITY[1], PERIODICITY[2]};
   Check(lbx_sim_create(lambrex::Context(), dims, per, precision_, rho0_, &sim_));
+  if (ref_ratio_ != kRefRatio) Check(lbx_set_ref_ratio(sim_, ref_ratio_));
   for (int l = 0; l <= finest; ++l) {
     const std::vector<int32_t> lohi = lambrex::to_lohi(lv[l].boxes);
     Check(lbx_level_define(sim_, l, (int)lv[l].boxes.size(), lohi.data()));

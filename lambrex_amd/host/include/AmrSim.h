@@ -6,8 +6,9 @@
 //     periodicity flag is set; where the reference aborts on a non-periodic
 //     direction (src/AmrSim.cpp:788-797), this build puts halfway
 //     bounce-back walls on its faces (single-level simulations only);
-//   - up to LBX_MAX_LEVELS levels (max_ref_level < LBX_MAX_LEVELS), ratio 2,
-//     advanced by the reference's recursive Rohde cycle;
+//   - up to LBX_MAX_LEVELS levels (max_ref_level < LBX_MAX_LEVELS), ratio 2
+//     (or 3 or 4 between levels 0 and 1 of a two-level simulation,
+//     SetRefRatio), advanced by the reference's recursive Rohde cycle;
 //   - initial density / velocity vectors in C order, i slowest, velocity
 //     component fastest (include/AmrSim.h:79-83);
 //   - omega = 1/(tau + 1/2) (src/AmrSim.cpp:125-126);
@@ -95,6 +96,12 @@ class AmrSim {
   void SetMaxGridSize(int max_grid_size);
   // LBX_FP64 (default) or LBX_FP32_SHIFTED with reference density rho0.
   void SetPrecision(int precision, double rho0);
+  // Refinement ratio of level 1 to level 0: 2 (the reference's, fixed in its
+  // constructor, src/AmrSim.cpp:766), 3 or 4; a ratio other than 2 needs
+  // max_level <= 1.  Before InitFromScratch.  Level 1 then takes `ratio`
+  // substeps per level-0 step (dt / ratio, tau rescaled by ComputeDt at that
+  // ratio) and Regrid is unavailable (its clustering refines by 2).
+  void SetRefRatio(int ratio);
   // Mask the coarse collide with the fine BoxArray instead of the coarse one
   // the reference passes to makeFineMask (src/AmrSim.cpp:419-428).
   void SetFineMaskFromFineGrids(bool on);
@@ -118,7 +125,8 @@ class AmrSim {
   // writes to `dir`: a text Header (format tag, domain, periodicity,
   // max/finest level, precision and rho0, max_grid_size, mask choice,
   // static refinement boxes, and per level its boxes, time, dt, step,
-  // tau_s, tau_b, mass and density-swap parity, doubles as C hexfloats), and
+  // tau_s, tau_b, mass and density-swap parity, doubles as C hexfloats; the
+  // refinement ratio), and
   // per level Level_<l>/DistFn (every box's populations of the current
   // state, [15][nz][ny][nx] little-endian FP64, boxes in order) and
   // Level_<l>/Hydro (both density arrays and the velocity of every box).
@@ -158,6 +166,7 @@ class AmrSim {
   };
 
   void ComputeDt(int level);
+  int Ratio(int level) const;
   void MakeNewLevelFromScratch(int level, double time, const std::vector<lambrex::IBox>& ba);
   void MakeNewLevelFromCoarse(int level, double time, const std::vector<lambrex::IBox>& ba);
   void RemakeLevel(int level, double time, const std::vector<lambrex::IBox>& ba);
@@ -179,6 +188,7 @@ class AmrSim {
   int max_level_;
   int finest_level_ = 0;
   int max_grid_size_ = 32;
+  int ref_ratio_ = 2;
   int precision_ = LBX_FP64;
   double rho0_ = 1.0;
   bool mask_from_fine_grids_ = false;

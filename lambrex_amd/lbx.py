@@ -45,6 +45,7 @@ SIGNATURES = {
                                         ctypes.c_int32, _i32p, _i32p]),
     "lbx_cluster_tags_allowed": (ctypes.c_int, [_i32p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i32p, _i32p]),
+    "lbx_set_ref_ratio": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lbx_set_relax": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     "lbx_set_fine_mask": (ctypes.c_int, [_vp, ctypes.c_int, _i32p]),
     "lbx_set_level_mask": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i32p]),
@@ -275,6 +276,10 @@ class Sim:
         self.ctx.check(self._lib.lbx_tag_cells(self.h, level, int(criterion), float(threshold),
                                                tags.ctypes.data_as(ctypes.c_void_p)))
         return tags
+
+    def set_ref_ratio(self, ratio):
+        """Refinement ratio (2, 3 or 4) of level 1 to level 0, before level 1 is defined."""
+        self.ctx.check(self._lib.lbx_set_ref_ratio(self.h, int(ratio)))
 
     def set_relax(self, level, omega_s, omega_b):
         self.ctx.check(self._lib.lbx_set_relax(self.h, level, float(omega_s), float(omega_b)))

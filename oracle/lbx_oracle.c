@@ -262,6 +262,7 @@ typedef struct {
   double** next;
   double** tmp;
   int dom[3];      /* periodic level domain [0,dom) */
+  int ratio;       /* refinement ratio to the parent level (level 0: 1) */
   double tau_s, tau_b, dt, time;
   int step;
   int* mask;       /* per coarse box valid cell: 1 = covered (FINE_VAL) */
@@ -272,6 +273,7 @@ typedef struct {
 struct orc_sim {
   int n[3];
   int nlev;
+  int ratio;       /* ratio of every level added next (refRatio, src/AmrSim.cpp:766: 2) */
   orc_level lev[ORC_MAX_LEVELS];
 };
 
@@ -414,19 +416,24 @@ static void swap_now_next(orc_level* L) {
  *   5 ghosts from coarse even with a fine image
  *   6 parent index by truncation, not floor   7 comp 0 zeroed on both layers
  *   8 every component zeroed on the outer layer
- *   9 ZeroInvalid keeps pos + 2c in valid    10 ... keeps pos - c in valid */
+ *   9 ZeroInvalid keeps pos + 2c in valid    10 ... keeps pos - c in valid
+ *  11 average-down over the valid boxes only (nGrow 0) at every ratio
+ *  12 average-down volume fraction 1/8 at every ratio
+ *  13 the finest level takes 2 substeps at every ratio */
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
 #endif
 
-static int floor_div2(int i) { return (i >= 0) ? i / 2 : -((-i + 1) / 2); }
-static int parent_of(int i) { return ORC_MUTANT == 6 ? i / 2 : floor_div2(i); }
+static int floor_div(int i, int r) { return (i >= 0) ? i / r : -((-i + r - 1) / r); }
+static int parent_of(int i, int r) { return ORC_MUTANT == 6 ? i / r : floor_div(i, r); }
 
 /* FillPatchTwoLevels with PCInterp (src/AmrSim.cpp:373-388): every cell of
  * each grown fine box takes the fine valid value of its periodic image when
  * some fine box holds it, else the coarse `now` value of the coarse cell
- * that contains it (piecewise-constant, ratio 2, coarse periodic wrap). */
+ * that contains it (piecewise-constant over the refinement ratio, coarse
+ * periodic wrap). */
 static void fill_two_levels(orc_level* C, orc_level* F, double** dst) {
+  const int r = F->ratio;
   for (int b = 0; b < F->nbox; ++b) {
     const box3* B = &F->vb[b];
     for (int i = B->lo[0] - ORC_HALO; i <= B->hi[0] + ORC_HALO; ++i)
@@ -439,7 +446,7 @@ static void fill_two_levels(orc_level* C, orc_level* F, double** dst) {
               dst[b][fidx(B, i, j, k, n)] = F->now[s][fidx(&F->vb[s], si, sj, sk, n)];
           } else {
             int ci, cj, ck;
-            const int c = locate(C, parent_of(i), parent_of(j), parent_of(k), &ci, &cj, &ck);
+            const int c = locate(C, parent_of(i, r), parent_of(j, r), parent_of(k, r), &ci, &cj, &ck);
             for (int n = 0; n < ORC_NV; ++n)
               dst[b][fidx(B, i, j, k, n)] = C->now[c][fidx(&C->vb[c], ci, cj, ck, n)];
           }
@@ -493,16 +500,20 @@ static void zero_invalid(orc_level* L, double** f) {
   }
 }
 
-/* sum_fine_to_coarse (src/AmrSim.cpp:592-602): each fine box grown by 2
- * fine ghosts is averaged 2x2x2 onto its coarsened box grown by 1 coarse
- * ghost (sum in k,j,i order of the fine children, then times 1/8), and the
- * result is ADDED into every coarse valid cell that holds the periodic image
- * of that coarse position.  Boxes are added in box order. */
+/* sum_fine_to_coarse (src/AmrSim.cpp:592-602, AMReX's sum_fine_to_coarse
+ * with ratio r): each fine box grown by its 2 fine ghosts is averaged over
+ * r x r x r blocks (amrex_avgdown: sum in k,j,i order of the fine children,
+ * then times the volume fraction 1/r^3) onto its coarsened box grown by
+ * nGrow = 2 / r coarse cells (1 for ratio 2, 0 above), and the result is
+ * ADDED into every coarse valid cell that holds the periodic image of that
+ * coarse position.  Boxes are added in box order. */
 static void sum_fine_to_coarse(orc_level* F, orc_level* C, double** cdst) {
+  const int r = F->ratio, ng = ORC_MUTANT == 11 ? 0 : ORC_HALO / r;
+  const double volfrac = ORC_MUTANT == 12 ? 0.125 : 1.0 / (double)(r * r * r);
   for (int b = 0; b < F->nbox; ++b) {
     const box3* B = &F->vb[b];
-    const int clo[3] = {floor_div2(B->lo[0]) - 1, floor_div2(B->lo[1]) - 1, floor_div2(B->lo[2]) - 1};
-    const int chi[3] = {floor_div2(B->hi[0]) + 1, floor_div2(B->hi[1]) + 1, floor_div2(B->hi[2]) + 1};
+    const int clo[3] = {floor_div(B->lo[0], r) - ng, floor_div(B->lo[1], r) - ng, floor_div(B->lo[2], r) - ng};
+    const int chi[3] = {floor_div(B->hi[0], r) + ng, floor_div(B->hi[1], r) + ng, floor_div(B->hi[2], r) + ng};
     for (int ci = clo[0]; ci <= chi[0]; ++ci)
       for (int cj = clo[1]; cj <= chi[1]; ++cj)
         for (int ck = clo[2]; ck <= chi[2]; ++ck) {
@@ -511,15 +522,15 @@ static void sum_fine_to_coarse(orc_level* F, orc_level* C, double** cdst) {
           if (c < 0) continue;
           for (int n = 0; n < ORC_NV; ++n) {
             double s = 0.0;
-            for (int dk = 0; dk < 2; ++dk)
-              for (int dj = 0; dj < 2; ++dj)
-                for (int di = 0; di < 2; ++di) {
-                  int ii = 2 * ci + di, jj = 2 * cj + dj, kk = 2 * ck + dk;
-                  if (ORC_MUTANT == 4) { ii = 2 * ci + dk; kk = 2 * ck + di; }
+            for (int dk = 0; dk < r; ++dk)
+              for (int dj = 0; dj < r; ++dj)
+                for (int di = 0; di < r; ++di) {
+                  int ii = r * ci + di, jj = r * cj + dj, kk = r * ck + dk;
+                  if (ORC_MUTANT == 4) { ii = r * ci + dk; kk = r * ck + di; }
                   if (ORC_MUTANT == 2 && !in_box(B, ii, jj, kk)) continue;
                   s += F->now[b][fidx(B, ii, jj, kk, n)];
                 }
-            if (ORC_MUTANT != 1) s *= 0.125;
+            if (ORC_MUTANT != 1) s *= volfrac;
             if (ORC_MUTANT == 3)
               cdst[c][fidx(&C->vb[c], ti, tj, tk, n)] = s;
             else
@@ -559,6 +570,8 @@ orc_sim* orc_sim_create(int nx, int ny, int nz, int nboxes, const int* lohi,
   L->tau_s = tau_s;
   L->tau_b = tau_b;
   L->dt = 1.0; /* ComputeDt level 0, src/AmrSim.cpp:316-319 */
+  L->ratio = 1;
+  s->ratio = 2;
   level_alloc(L);
   return s;
 }
@@ -606,19 +619,21 @@ static void iterate_single(orc_sim* s) {
 
 /* The finest level's part of RohdeCycle (src/AmrSim.cpp:445-452):
  * InitPostCollision (two-level fill, comp 0 zeroed on the outer shell), then
- * FineCollide, Stream, FineCollide, Stream with no ghost refresh between,
- * ZeroInvalidComponents, and UpdateDistribution's two steps and swap
- * (src/AmrSim.cpp:619-631). */
+ * FineCollide + Stream once per fine substep with no ghost refresh between,
+ * ZeroInvalidComponents, and UpdateDistribution's steps and swap
+ * (src/AmrSim.cpp:619-631).  The reference writes the two substeps and
+ * UpdateDistribution's 2 out for its ratio 2; at ratio r the finest level
+ * takes r substeps (dt_fine = dt_coarse / r), which is the same for r = 2. */
 static void finest_pair(orc_level* C, orc_level* F) {
   fill_two_levels(C, F, F->next);
   zero_outer_comp0(F, F->next);
-  collide_level(F, F->next, 0);
-  stream_level(F, &F->next);
-  collide_level(F, F->next, 0);
-  stream_level(F, &F->next);
+  for (int sub = 0; sub < (ORC_MUTANT == 13 ? 2 : F->ratio); ++sub) {
+    collide_level(F, F->next, 0);
+    stream_level(F, &F->next);
+  }
   zero_invalid(F, F->next);
-  F->time += 2 * F->dt;
-  F->step += 2;
+  F->time += F->ratio * F->dt;
+  F->step += F->ratio;
   swap_now_next(F);
 }
 
@@ -643,8 +658,7 @@ static void rohde_cycle(orc_sim* s, int l) {
   if (l + 1 == s->nlev - 1) {
     finest_pair(L, F);
   } else {
-    rohde_cycle(s, l + 1);
-    rohde_cycle(s, l + 1);
+    for (int iter = 0; iter < F->ratio; ++iter) rohde_cycle(s, l + 1);  /* REF_RATIO, :453-455 */
   }
   stream_level(L, &L->next);
   sum_fine_to_coarse(F, L, L->next);
@@ -653,6 +667,12 @@ static void rohde_cycle(orc_sim* s, int l) {
   L->time += L->dt;
   L->step += 1;
   swap_now_next(L);
+}
+
+int orc_sim_set_ratio(orc_sim* s, int ratio) {
+  if (ratio < 2 || ratio > 4) return -1;
+  s->ratio = ratio;
+  return 0;
 }
 
 void orc_sim_iterate(orc_sim* s, int nsteps) {
@@ -670,7 +690,9 @@ int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
   memset(F, 0, sizeof(*F));
   F->nbox = nboxes;
   F->vb = (box3*)calloc(nboxes, sizeof(box3));
-  for (int d = 0; d < 3; ++d) F->dom[d] = 2 * C->dom[d];
+  const int r = s->ratio;
+  F->ratio = r;
+  for (int d = 0; d < 3; ++d) F->dom[d] = r * C->dom[d];
   for (int b = 0; b < nboxes; ++b)
     for (int d = 0; d < 3; ++d) {
       F->vb[b].lo[d] = lohi_fine[6 * b + d];
@@ -679,10 +701,10 @@ int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
         free(F->vb); memset(F, 0, sizeof(*F)); return -2;
       }
     }
-  /* ComputeDt(1) (src/AmrSim.cpp:308-315), ratio 2 */
-  F->dt = C->dt / 2;
-  F->tau_s = 2 * (C->tau_s - 0.5) + 0.5;
-  F->tau_b = 2 * (C->tau_b - 0.5) + 0.5;
+  /* ComputeDt(1) (src/AmrSim.cpp:308-315) at ratio r */
+  F->dt = C->dt / r;
+  F->tau_s = r * (C->tau_s - 0.5) + 0.5;
+  F->tau_b = r * (C->tau_b - 0.5) + 0.5;
   F->time = C->time;
   F->step = 0;
   level_alloc(F);
@@ -697,7 +719,7 @@ int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
         for (int k = B->lo[2] - ORC_HALO; k <= B->hi[2] + ORC_HALO; ++k) {
           int si, sj, sk;
           if (!in_box(B, i, j, k) && locate(F, i, j, k, &si, &sj, &sk) >= 0) continue;
-          if (locate(C, floor_div2(i), floor_div2(j), floor_div2(k), &si, &sj, &sk) < 0) {
+          if (locate(C, floor_div(i, r), floor_div(j, r), floor_div(k, r), &si, &sj, &sk) < 0) {
             level_free(F);
             return -3;
           }
@@ -713,13 +735,13 @@ int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
       for (int j = B->lo[1]; j <= B->hi[1]; ++j)
         for (int k = B->lo[2]; k <= B->hi[2]; ++k) {
           int ci, cj, ck;
-          const int c = locate(C, floor_div2(i), floor_div2(j), floor_div2(k), &ci, &cj, &ck);
+          const int c = locate(C, floor_div(i, r), floor_div(j, r), floor_div(k, r), &ci, &cj, &ck);
           for (int n = 0; n < ORC_NV; ++n)
             F->now[b][fidx(B, i, j, k, n)] = C->now[c][fidx(&C->vb[c], ci, cj, ck, n)];
         }
   }
   /* Fine mask on coarse valid cells: FINE_VAL where the coarse cell lies in
-   * coarsen(mask box, 2) for some mask box (src/AmrSim.cpp:419-428). */
+   * coarsen(mask box, r) for some mask box (src/AmrSim.cpp:419-428). */
   size_t total = 0;
   free(C->mask);
   free(C->mask_off);
@@ -736,9 +758,9 @@ int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine, int nmask,
         for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
           int hit = 0;
           for (int m = 0; m < nmask && !hit; ++m) {
-            const int* r = &mask_lohi_fine[6 * m];
-            if (i >= floor_div2(r[0]) && i <= floor_div2(r[3]) && j >= floor_div2(r[1]) &&
-                j <= floor_div2(r[4]) && k >= floor_div2(r[2]) && k <= floor_div2(r[5]))
+            const int* mb = &mask_lohi_fine[6 * m];
+            if (i >= floor_div(mb[0], r) && i <= floor_div(mb[3], r) && j >= floor_div(mb[1], r) &&
+                j <= floor_div(mb[4], r) && k >= floor_div(mb[2], r) && k <= floor_div(mb[5], r))
               hit = 1;
           }
           C->mask[C->mask_off[b] + (((size_t)(k - B->lo[2]) * bnx(B, 1) + (j - B->lo[1])) * bnx(B, 0) +

@@ -64,16 +64,22 @@ void orc_sim_destroy(orc_sim* s);
  * equilibrium distribution on level 0.  src/AmrSim.cpp:138-295, 845-936. */
 int orc_sim_init(orc_sim* s, const double* rho_c, const double* u_c);
 
-/* Add a new finest level (refinement ratio 2, at most ORC_MAX_LEVELS
+/* Add a new finest level (refinement ratio 2 unless set below, at most ORC_MAX_LEVELS
  * levels) made of the given boxes in its own index space, filled from the
  * current finest level by piecewise-constant interpolation
  * (src/AmrSim.cpp:689-713).  mask_lohi lists the boxes (new level's index
- * space) whose coarsened(by 2) cover is masked in the parent's collide; the
+ * space) whose coarsened (by the ratio) cover is masked in the parent's collide; the
  * reference passes the parent's own BoxArray (src/AmrSim.cpp:419-428).
  * Returns -3 if a grown new box needs a parent cell outside the parent
  * level's valid region (not properly nested). */
 int orc_sim_add_fine(orc_sim* s, int nboxes, const int* lohi_fine,
                      int nmask, const int* mask_lohi_fine);
+/* Refinement ratio (2, 3 or 4) of the levels added after this call; the
+ * reference's constructor fixes 2 (src/AmrSim.cpp:766).  The new level's
+ * boxes, domain, parents, dt, tau, mask and average-down use it, and it
+ * takes `ratio` substeps per step of its parent (2 written out in the
+ * reference, src/AmrSim.cpp:446-452, 621-624).  Returns -1 otherwise. */
+int orc_sim_set_ratio(orc_sim* s, int ratio);
 
 void orc_sim_iterate(orc_sim* s, int nsteps);
 

@@ -45,6 +45,7 @@ def _bind(L):
     L.orc_sim_destroy.argtypes = [ctypes.c_void_p]
     L.orc_sim_init.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.orc_sim_add_fine.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int, _ip]
+    L.orc_sim_set_ratio.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.orc_sim_iterate.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.orc_sim_bbox.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
     L.orc_sim_get_f.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
@@ -169,6 +170,11 @@ class Sim:
                                       len(mask_boxes), boxes_to_lohi(mask_boxes))
         if rc != 0:
             raise ValueError(f"orc_sim_add_fine failed ({rc})")
+
+    def set_ratio(self, ratio):
+        """Refinement ratio (2, 3 or 4) of the levels added after this call."""
+        if self._L.orc_sim_set_ratio(self._h, int(ratio)) != 0:
+            raise ValueError(f"unsupported refinement ratio {ratio}")
 
     def iterate(self, n):
         self._L.orc_sim_iterate(self._h, int(n))

@@ -317,6 +317,42 @@ TEST_CASE("FP32-shifted facade run tracks FP64") {
   REQUIRE(max_err <= 1e-6);
 }
 
+// Refinement ratio 3 between levels 0 and 1 (SetRefRatio; the reference
+// fixes 2): ComputeDt at that ratio, boxes aligned to it, three fine steps
+// per coarse step, conservation of the fine level's arithmetic (finite
+// state, masses of both levels reported).
+TEST_CASE("TwoLevel Ratio3") {
+  AmrSim sim(24, 24, 12, 1, {1, 1, 1}, 0.7, 0.7);
+  REQUIRE_THROWS(sim.SetRefRatio(5));
+  sim.SetRefRatio(3);
+  sim.SetInitialDensity(1.0);
+  sim.SetInitialVelocity(0.01);
+  sim.InitFromScratch(0.0);
+  REQUIRE_THROWS(sim.SetRefRatio(2));  // after InitFromScratch
+  REQUIRE(sim.refRatio(0)[0] == 3);
+  sim.SetStaticRefinement(0, {6, 6, 3}, {14, 14, 8});
+  REQUIRE(sim.finestLevel() == 1);
+  for (const lambrex::IBox& b : sim.boxArray(1))
+    for (int d = 0; d < NDIMS; ++d) {
+      REQUIRE(b.lo[d] % 3 == 0);
+      REQUIRE((b.hi[d] + 1) % 3 == 0);
+    }
+  const auto ext = sim.GetExtent(1);
+  REQUIRE(ext.first[0] == 18);
+  REQUIRE(ext.second[0] == 44);
+  REQUIRE(sim.GetDt(1) == 1.0 / 3);
+  REQUIRE(sim.GetTauS(1) == 3 * (0.7 - 0.5) + 0.5);
+  sim.Iterate(2);
+  REQUIRE(sim.GetTimeStep(0) == 2);
+  REQUIRE(sim.GetTimeStep(1) == 6);
+  REQUIRE(sim.GetTime(1) == Approx(2.0));
+  sim.CalcHydroVars(1);
+  const double rho = sim.GetDensity(30, 30, 15, 1);
+  REQUIRE(std::isfinite(rho));
+  REQUIRE(rho > 0.5);
+  REQUIRE(rho < 1.5);
+}
+
 int main() {
   lambrexInit();
   const int rc = check::run_all();

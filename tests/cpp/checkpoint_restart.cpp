@@ -34,13 +34,15 @@ struct Setup {
   int max_level = 0;
   int precision = LBX_FP64;
   bool fine_mask = false;
+  int ratio = 2;
   std::array<int, NDIMS> periodic{1, 1, 1};
 };
 
 std::unique_ptr<AmrSim> make(const Setup& s) {
   std::unique_ptr<AmrSim> sim(new AmrSim(NX, NY, NZ, s.max_level, s.periodic, 0.7, 0.7));
-  sim->SetMaxGridSize(8);
+  sim->SetMaxGridSize(s.ratio == 2 ? 8 : 9);
   if (s.precision != LBX_FP64) sim->SetPrecision(s.precision, 1.0);
+  if (s.ratio != 2) sim->SetRefRatio(s.ratio);
   std::mt19937_64 rng(12345);
   std::normal_distribution<double> n(0.0, 1e-3);
   std::uniform_real_distribution<double> uu(-0.01, 0.01);
@@ -130,6 +132,14 @@ TEST_CASE("Checkpoint three levels") {
   s.max_level = 2;
   s.fine_mask = true;
   restart_matches(s, "three", 2, 2);
+}
+
+TEST_CASE("Checkpoint two levels at refinement ratio 3") {
+  Setup s;
+  s.max_level = 1;
+  s.fine_mask = true;
+  s.ratio = 3;
+  restart_matches(s, "ratio3", 2, 2);
 }
 
 TEST_CASE("Checkpoint FP32-shifted two levels") {

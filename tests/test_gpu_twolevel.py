@@ -397,3 +397,91 @@ def test_redefined_fine_level_starts_with_unmasked_parent(ctx):
         sim.close()
     assert_bitwise(fo0, fg0, "coarse f")
     assert_bitwise(fo1, fg1, "fine f")
+
+
+def run_ratio(ctx, r, mask_kind, steps, precision=lbx.LBX_FP64):
+    """Two levels at refinement ratio r: coarse 12x9x6 in boxes of 6^3 (or
+    less), fine boxes (aligned to r, fine index space r * dims) over coarse
+    x in [3, 9) and y in [0, 9) (spanning all y: periodic images), a box
+    wrapping onto itself along z, and an isolated box; random state."""
+    dims = (12, 9, 6)
+    cboxes = workloads.chop(dims, 6)
+
+    def fbox(lo, hi):  # coarse cells lo..hi refined by r
+        return (tuple(r * a for a in lo), tuple(r * b + r - 1 for b in hi))
+    fboxes = [fbox((3, 0, 0), (5, 4, 5)), fbox((3, 5, 0), (5, 8, 5)), fbox((6, 0, 0), (8, 8, 5)),
+              fbox((10, 3, 2), (10, 5, 3))]
+    # "reference": the coarse BoxArray itself, coarsened by r (MakeFineMask,
+    # src/AmrSim.cpp:419-428); "fine": the fine BoxArray
+    mboxes = list(cboxes) if mask_kind == "reference" else list(fboxes)
+    rho, u = workloads.random_state(dims, seed=100 + r, rho_sigma=1e-2, u_amp=0.02)
+    tau = 0.8
+    tau_f = r * (tau - 0.5) + 0.5  # ComputeDt at ratio r, src/AmrSim.cpp:313-314
+    sim = lbx.Sim(ctx, dims, precision=precision)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, tau)
+        sim.init_from_fields(rho, u)
+        sim.set_ref_ratio(r)
+        sim.define_level(1, fboxes)
+        sim.set_tau(1, tau_f)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(mboxes)
+        sim.iterate(steps)
+        g = (sim.get_f(0), sim.get_f_bbox(1)[0], sim.get_hydro(0)[0], sim.get_hydro_bbox(1)[0])
+    finally:
+        sim.close()
+    return dims, cboxes, fboxes, mboxes, rho, u, tau, tau_f, g
+
+
+@pytest.mark.parametrize("r", [3, 4])
+@pytest.mark.parametrize("mask_kind", ["fine", "reference"])
+def test_refinement_ratio_two_level_bitwise(ctx, r, mask_kind):
+    """Ratio 3 and 4 (r substeps per coarse step, r^3-block average-down over
+    the valid boxes, parents floor(pos / r)): both levels after 3 coarse
+    steps equal the oracle's at the same ratio, bit for bit."""
+    steps = 3
+    dims, cboxes, fboxes, mboxes, rho, u, tau, tau_f, (fg0, fg1, _, _) = run_ratio(ctx, r, mask_kind, steps)
+    o = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in cboxes])
+    o.init(*c_order(rho, u))
+    o.set_ratio(r)
+    o.add_fine([(list(a), list(b)) for a, b in fboxes], [(list(a), list(b)) for a, b in mboxes])
+    assert o.tau(1) == tau_f
+    o.iterate(steps)
+    assert o.step(1) == r * steps
+    fo0, fo1 = o.get_f(0), o.get_f(1)
+    o.close()
+    assert_bitwise(fo0, fg0, f"coarse f, ratio {r}")
+    assert_bitwise(fo1, fg1, f"fine f, ratio {r}")
+
+
+def test_refinement_ratio_3_fp32_tracks_fp64(ctx):
+    """FP32-shifted storage at ratio 3 follows the FP64 path from the same
+    state: densities of both levels within 1e-6 relative after 3 steps."""
+    *_, g64 = run_ratio(ctx, 3, "fine", 3)
+    *_, g32 = run_ratio(ctx, 3, "fine", 3, lbx.LBX_FP32_SHIFTED)
+    for a, b in ((g64[2], g32[2]), (g64[3], g32[3])):
+        ok = a > 0  # the fine bounding box has cells outside every box (zero)
+        assert float(np.max(np.abs(b[ok] - a[ok]) / a[ok])) <= 1e-6
+
+
+def test_refinement_ratio_is_checked(ctx):
+    """Ratio outside 2..4, a ratio change once level 1 exists, boxes not
+    aligned to the ratio and a level 2 at ratio 3 are refused."""
+    dims = (12, 9, 6)
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, workloads.chop(dims, 6))
+        with pytest.raises(lbx.LbxError):
+            sim.set_ref_ratio(5)
+        sim.set_ref_ratio(3)
+        with pytest.raises(lbx.LbxError):
+            sim.define_level(1, [((2, 0, 0), (7, 2, 2))])  # lo not a multiple of 3
+        sim.define_level(1, [((9, 0, 0), (17, 8, 8))])
+        with pytest.raises(lbx.LbxError):
+            sim.set_ref_ratio(2)
+        with pytest.raises(lbx.LbxError) as e:
+            sim.define_level(2, [((20, 2, 2), (23, 5, 5))])
+        assert e.value.code == lbx.LBX_E_UNSUPPORTED
+    finally:
+        sim.close()

@@ -33,8 +33,12 @@ oracle's code:
   equal the single-level coarse step (unmasked) or +0 (masked) plus the 1/8
   sum of its 8 fine children after the step, bit for bit.
 
-test_mutants_fail_a_pin builds the oracle with each planted mistake
-(ORC_MUTANT, lbx_oracle.c) and shows that some check above fails on it.
+Every check runs at refinement ratio 2 (the reference's, src/AmrSim.cpp:766)
+and at ratios 3 and 4 (the generalisation: parents floor(pos / r), r^3
+blocks times 1/r^3 over the box grown by 2 / r coarse cells, r fine
+substeps).  test_mutants_fail_a_pin builds the oracle with each planted
+mistake (ORC_MUTANT, lbx_oracle.c) and shows that some check above fails on
+it.
 """
 import itertools
 import subprocess
@@ -74,21 +78,32 @@ def inside(x, y, z, lo, hi):
 
 
 # --- geometry of the piece checks -------------------------------------------
-CDIMS = (8, 6, 6)  # coarse domain, fine domain 16 x 12 x 12
-# fine boxes (fine index space): two boxes side by side in y spanning all y
-# and z (ghosts from each other and from periodic images), an isolated box
-# (ghosts from the coarse level) and a 2^3 box
-FBOXES = [((0, 0, 0), (5, 5, 11)), ((0, 6, 0), (5, 11, 11)), ((10, 2, 4), (11, 5, 7)),
-          ((10, 8, 0), (11, 9, 1))]
+CDIMS = (8, 6, 6)  # coarse domain, fine domain r * CDIMS
+RATIOS = (2, 3, 4)
 
 
-def piece_sim(library):
+def fine_boxes(r):
+    """Fine boxes (fine index space, aligned to r) for ratio r: two boxes
+    side by side in y spanning all y and z (ghosts from each other and from
+    periodic images), an isolated box (ghosts from the coarse level) and a
+    one-coarse-cell box."""
+    def box(lo, hi):  # coarse cells lo..hi refined by r
+        return (tuple(r * a for a in lo), tuple(r * b + r - 1 for b in hi))
+    return [box((0, 0, 0), (2, 2, 5)), box((0, 3, 0), (2, 5, 5)), box((5, 1, 2), (5, 2, 3)),
+            box((5, 4, 0), (5, 4, 0))]
+
+
+FBOXES = fine_boxes(2)
+
+
+def piece_sim(library, r=2):
     cboxes = orc.max_size_chop((0, 0, 0), tuple(n - 1 for n in CDIMS), 4)
     s = orc.Sim(*CDIMS, 0.8, boxes=cboxes, library=library)
     rho = np.ones(CDIMS[::-1])
     u = np.zeros((3,) + CDIMS[::-1])
     s.init(*c_order(rho, u))
-    s.add_fine(lists(FBOXES), [])
+    s.set_ratio(r)
+    s.add_fine(lists(fine_boxes(r)), [])
     return s
 
 
@@ -96,8 +111,8 @@ def code_coarse(q, x, y, z):
     return -(1.0 + q * 10000.0 + (z * CDIMS[1] + y) * CDIMS[0] + x)
 
 
-def code_fine(q, x, y, z):
-    fd = [2 * n for n in CDIMS]
+def code_fine(q, x, y, z, r=2):
+    fd = [r * n for n in CDIMS]
     return 1.0 + q * 100000.0 + (z * fd[1] + y) * fd[0] + x
 
 
@@ -113,27 +128,28 @@ def set_coded(s, level, code):
         s.set_grown(level, b, data, which=0)
 
 
-def check_fill(library):
+def check_fill(library, r=2):
     """InitPostCollision(1) on coded fields; returns a list of failures."""
     fails = []
-    s = piece_sim(library)
+    s = piece_sim(library, r)
+    fboxes = fine_boxes(r)
     set_coded(s, 0, code_coarse)
-    set_coded(s, 1, code_fine)
+    set_coded(s, 1, lambda q, x, y, z: code_fine(q, x, y, z, r))
     s.stage(1, orc.STAGE_INIT_POST_COLLISION)
-    fd = np.array([2 * n for n in CDIMS])
-    for b, (lo, hi) in enumerate(FBOXES):
+    fd = np.array([r * n for n in CDIMS])
+    for b, (lo, hi) in enumerate(fboxes):
         got = s.get_grown(1, b, which=1)
         x, y, z = grown_positions(lo, hi)
         wx, wy, wz = x % fd[0], y % fd[1], z % fd[2]
         fine_image = np.zeros(x.shape, dtype=bool)
-        for blo, bhi in FBOXES:
+        for blo, bhi in fboxes:
             fine_image |= inside(wx, wy, wz, blo, bhi)
         # piecewise-constant parent: floor division, then the coarse periodic image
-        px, py, pz = np.floor_divide(x, 2) % CDIMS[0], np.floor_divide(y, 2) % CDIMS[1], np.floor_divide(z, 2) % CDIMS[2]
+        px, py, pz = np.floor_divide(x, r) % CDIMS[0], np.floor_divide(y, r) % CDIMS[1], np.floor_divide(z, r) % CDIMS[2]
         outer = (x == lo[0] - 2) | (x == hi[0] + 2) | (y == lo[1] - 2) | (y == hi[1] + 2) | \
                 (z == lo[2] - 2) | (z == hi[2] + 2)
         for q in range(15):
-            exp = np.where(fine_image, code_fine(q, wx, wy, wz), code_coarse(q, px, py, pz))
+            exp = np.where(fine_image, code_fine(q, wx, wy, wz, r), code_coarse(q, px, py, pz))
             if q == 0:
                 exp = np.where(outer, 0.0, exp)
             if np.isnan(got[q]).any():
@@ -144,17 +160,18 @@ def check_fill(library):
     return fails
 
 
-def check_zero_invalid(library):
+def check_zero_invalid(library, r=2):
     """ZeroInvalidComponents(1) on all-ones FABs against the enumeration of
     vbox.contains(pos - 2 c_m) over every ghost cell and population."""
     fails = []
-    s = piece_sim(library)
-    for b, (lo, hi) in enumerate(FBOXES):
+    s = piece_sim(library, r)
+    fboxes = fine_boxes(r)
+    for b, (lo, hi) in enumerate(fboxes):
         lo, hi = np.array(lo), np.array(hi)
         shape = s.get_grown(1, b).shape
         s.set_grown(1, b, np.ones(shape), which=1)
     s.stage(1, orc.STAGE_ZERO_INVALID)
-    for b, (lo, hi) in enumerate(FBOXES):
+    for b, (lo, hi) in enumerate(fboxes):
         got = s.get_grown(1, b, which=1)
         lo, hi = np.array(lo), np.array(hi)
         for (k, j, i) in itertools.product(range(got.shape[1]), range(got.shape[2]), range(got.shape[3])):
@@ -172,14 +189,17 @@ def check_zero_invalid(library):
     return fails
 
 
-def check_sum(library):
-    """SumFromFine(0): random fine FABs (valid + ghosts) ADDed as 1/8 block
-    sums into random coarse `next` values."""
+def check_sum(library, r=2):
+    """SumFromFine(0): random fine FABs (valid + ghosts) ADDed as 1/r^3 block
+    sums into random coarse `next` values, over each box coarsened and grown
+    by 2 // r coarse cells."""
     fails = []
     rng = np.random.default_rng(3)
-    s = piece_sim(library)
+    s = piece_sim(library, r)
+    fboxes = fine_boxes(r)
+    ng = 2 // r
     fine = []
-    for b in range(len(FBOXES)):
+    for b in range(len(fboxes)):
         d = rng.standard_normal(s.get_grown(1, b).shape)
         s.set_grown(1, b, d, which=0)
         fine.append(d)
@@ -193,23 +213,23 @@ def check_sum(library):
     after = [s.get_grown(0, b, which=1) for b in range(cnb)]
     # contributions per coarse cell (domain coordinates), in box order
     contrib = {}
-    for b, (lo, hi) in enumerate(FBOXES):
+    for b, (lo, hi) in enumerate(fboxes):
         lo, hi = np.array(lo), np.array(hi)
-        clo, chi = np.floor_divide(lo, 2) - 1, np.floor_divide(hi, 2) + 1
+        clo, chi = np.floor_divide(lo, r) - ng, np.floor_divide(hi, r) + ng
         for cz in range(clo[2], chi[2] + 1):
             for cy in range(clo[1], chi[1] + 1):
                 for cx in range(clo[0], chi[0] + 1):
                     vals = []
-                    for dk in range(2):  # amrex_avgdown: k outer, i inner
-                        for dj in range(2):
-                            for di in range(2):
-                                p = np.array([2 * cx + di, 2 * cy + dj, 2 * cz + dk]) - lo + G
+                    for dk in range(r):  # amrex_avgdown: k outer, i inner
+                        for dj in range(r):
+                            for di in range(r):
+                                p = np.array([r * cx + di, r * cy + dj, r * cz + dk]) - lo + G
                                 vals.append(fine[b][:, p[2], p[1], p[0]])
                     acc = np.zeros(15)
                     for v in vals:
                         acc = acc + v
                     key = (cx % CDIMS[0], cy % CDIMS[1], cz % CDIMS[2])
-                    contrib.setdefault(key, []).append(acc * 0.125)
+                    contrib.setdefault(key, []).append(acc * (1.0 / r ** 3))
     for b in range(cnb):
         lo, hi = s.box(0, b)
         x, y, z = grown_positions(lo, hi)
@@ -233,35 +253,41 @@ def check_sum(library):
     return fails
 
 
-def check_whole_cycle(library, masked):
+def check_whole_cycle(library, masked, r=2):
     """One Rohde cycle with level 1 covering the whole domain (8^3 coarse in
-    one box, 16^3 fine in one box), random state, unmasked or masked coarse."""
+    one box, (8r)^3 fine in one box), random state, unmasked or masked
+    coarse."""
     fails = []
     dims = (8, 8, 8)
-    fdims = (16, 16, 16)
+    nf = 8 * r
+    fdims = (nf, nf, nf)
     rng = np.random.default_rng(11)
     rho = 1.0 + 1e-2 * rng.standard_normal(dims[::-1])
     u = 0.02 * (rng.random((3,) + dims[::-1]) - 0.5)
     tau = 0.8
-    tau_f = 2 * (tau - 0.5) + 0.5
+    tau_f = r * (tau - 0.5) + 0.5
     one = [((0, 0, 0), (7, 7, 7))]
-    fbox = [((0, 0, 0), (15, 15, 15))]
-    mask = [((0, 0, 0), (15, 15, 15))] if masked else []
+    fbox = [((0, 0, 0), (nf - 1, nf - 1, nf - 1))]
+    mask = list(fbox) if masked else []
     s = orc.Sim(*dims, tau, boxes=lists(one), library=library)
     s.init(*c_order(rho, u))
+    s.set_ratio(r)
     s.add_fine(lists(fbox), lists(mask))
     f_fine0 = s.get_f(1)
     s.iterate(1)
     cg, fg = s.get_f(0), s.get_f(1)
+    if s.tau(1) != tau_f or s.step(1) != r:
+        fails.append(f"cycle: fine tau {s.tau(1)} / steps {s.step(1)} are not ComputeDt's / r")
     s.close()
-    # the fine level alone: a periodic single level on the fine domain, 2 steps
+    # the fine level alone: a periodic single level on the fine domain, r
+    # steps; the ghosts, filled once and never refreshed, reach r layers in
     ref_f = orc.Sim(*fdims, tau_f, boxes=lists(fbox), library=library)
     ref_f.init(*c_order(np.ones(fdims[::-1]), np.zeros((3,) + fdims[::-1])))
     ref_f.set_f(f_fine0)
-    ref_f.iterate(2)
-    deep = (slice(None), slice(2, 14), slice(2, 14), slice(2, 14))
+    ref_f.iterate(r)
+    deep = (slice(None), slice(r, nf - r), slice(r, nf - r), slice(r, nf - r))
     if not np.array_equal(ref_f.get_f(0)[deep], fg[deep]):
-        fails.append("cycle: fine cells two layers inside the box differ from a single-level run")
+        fails.append(f"cycle: fine cells {r} layers inside the box differ from a single-level run")
     ref_f.close()
     # the coarse level alone, one single-level step (same arithmetic per cell)
     ref_c = orc.Sim(*dims, tau, boxes=lists(one), library=library)
@@ -271,37 +297,44 @@ def check_whole_cycle(library, masked):
     ref_c.close()
     inner = (slice(None), slice(1, 7), slice(1, 7), slice(1, 7))
     acc = np.zeros((15, 8, 8, 8))
-    for dk in range(2):
-        for dj in range(2):
-            for di in range(2):
-                acc = acc + fg[:, dk::2, dj::2, di::2]
-    avg = acc * 0.125
+    for dk in range(r):
+        for dj in range(r):
+            for di in range(r):
+                acc = acc + fg[:, dk::r, dj::r, di::r]
+    avg = acc * (1.0 / r ** 3)
     exp = (0.0 + avg) if masked else (a + avg)
     if not np.array_equal(cg[inner], exp[inner]):
         fails.append(f"cycle: interior coarse cells differ from the closed form ({'masked' if masked else 'unmasked'})")
     return fails
 
 
-def all_checks(library):
-    return (check_fill(library) + check_zero_invalid(library) + check_sum(library) +
-            check_whole_cycle(library, False) + check_whole_cycle(library, True))
+def all_checks(library, ratios=(2, 3)):
+    fails = []
+    for r in ratios:
+        fails += (check_fill(library, r) + check_zero_invalid(library, r) + check_sum(library, r) +
+                  check_whole_cycle(library, False, r) + check_whole_cycle(library, True, r))
+    return fails
 
 
-def test_fill_two_levels_pinned():
-    assert check_fill(orc.lib()) == []
+@pytest.mark.parametrize("r", RATIOS)
+def test_fill_two_levels_pinned(r):
+    assert check_fill(orc.lib(), r) == []
 
 
-def test_zero_invalid_components_pinned():
-    assert check_zero_invalid(orc.lib()) == []
+@pytest.mark.parametrize("r", RATIOS)
+def test_zero_invalid_components_pinned(r):
+    assert check_zero_invalid(orc.lib(), r) == []
 
 
-def test_sum_fine_to_coarse_pinned():
-    assert check_sum(orc.lib()) == []
+@pytest.mark.parametrize("r", RATIOS)
+def test_sum_fine_to_coarse_pinned(r):
+    assert check_sum(orc.lib(), r) == []
 
 
+@pytest.mark.parametrize("r", RATIOS)
 @pytest.mark.parametrize("masked", [False, True])
-def test_whole_cycle_on_fully_refined_domain(masked):
-    assert check_whole_cycle(orc.lib(), masked) == []
+def test_whole_cycle_on_fully_refined_domain(masked, r):
+    assert check_whole_cycle(orc.lib(), masked, r) == []
 
 
 MUTANTS = {
@@ -315,6 +348,9 @@ MUTANTS = {
     8: "every component zeroed on the outer layer",
     9: "ZeroInvalidComponents keeps pos + 2c",
     10: "ZeroInvalidComponents keeps pos - c",
+    11: "average-down over the valid boxes only (nGrow 0) at every ratio",
+    12: "average-down volume fraction 1/8 at every ratio",
+    13: "the finest level takes 2 substeps at every ratio",
 }
 
 
