@@ -129,6 +129,10 @@ struct lbx_ctx {
   // join it back to `stream` for independent patch groups
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // third stream (default priority): a hierarchy step's level-0 push, which
+  // only reads level 0's current state, runs beside the finer levels' work
+  hipStream_t push = nullptr;
+  hipEvent_t push_fork_ev = nullptr, push_join_ev = nullptr;
   std::string err;
   double* sums = nullptr;  // conservation sums: kPartials*4 partials + 4 results
   // staging buffer of host transfers (grows on demand, kept until destroy;
@@ -602,9 +606,22 @@ hipError_t enqueue_hierarchy_step(lbx_sim* sim, hipStream_t st) {
   const void* cnow = C.buf[C.cur];
   void* cnext = C.buf[1 - C.cur];
   int fpar = sim->lev[finest].cur;
-  hipError_t e = enqueue_children(sim, 0, finest, fpar, st);
-  if (e == hipSuccess)
-    e = lbx::launch_coarse_push(sim->precision, cnow, cnext, C.lat, C.d_mask, rc, C.push_shape, st);
+  // CoarseCollide + Stream(0) read level 0's `now` and write its `next`;
+  // the finer levels only read `now` (their parent fills), so the push runs
+  // on its own stream beside them, and SumFromFine(0) into `next` waits for
+  // both.
+  lbx_ctx* ctx = sim->ctx;
+  hipError_t e = hipEventRecord(ctx->push_fork_ev, st);
+  if (e == hipSuccess) e = hipStreamWaitEvent(ctx->push, ctx->push_fork_ev, 0);
+  if (e != hipSuccess) return e;
+  const hipError_t ep = lbx::launch_coarse_push(sim->precision, cnow, cnext, C.lat, C.d_mask, rc, C.push_shape,
+                                                ctx->push);
+  e = enqueue_children(sim, 0, finest, fpar, st);
+  // join even after a failed launch, so a capture can end
+  const hipError_t ej = hipEventRecord(ctx->push_join_ev, ctx->push);
+  const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(st, ctx->push_join_ev, 0) : ej;
+  if (e == hipSuccess) e = ep;
+  if (e == hipSuccess) e = ew;
   if (e == hipSuccess)
     e = lbx::launch_sum_apply(sim->precision, F.d_avg, F.avg_stride, cnext, C.lat, F.d_sum_cell,
                               F.d_sum_start, F.d_sum_contrib, F.sum_ncell, sim->rho0, st);
@@ -1221,10 +1238,17 @@ int lbx_ctx_create(int device, lbx_ctx** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->push, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->push_fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->push_join_ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&ctx->sums, (kPartials * 4 + 4) * sizeof(double));
   if (e == hipSuccess) e = scratch_setup(device);
   if (e != hipSuccess) {
     hip_fail(nullptr, e, "context set-up");
+    if (ctx->sums) (void)hipFree(ctx->sums);
+    if (ctx->push_join_ev) (void)hipEventDestroy(ctx->push_join_ev);
+    if (ctx->push_fork_ev) (void)hipEventDestroy(ctx->push_fork_ev);
+    if (ctx->push) (void)hipStreamDestroy(ctx->push);
     if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1242,11 +1266,15 @@ int lbx_ctx_destroy(lbx_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(ctx->side);
+  (void)hipStreamSynchronize(ctx->push);
   if (ctx->sums) (void)hipFree(ctx->sums);
   if (ctx->stage) (void)hipFree(ctx->stage);
   (void)hipEventDestroy(ctx->join_ev);
   (void)hipEventDestroy(ctx->fork_ev);
   (void)hipStreamDestroy(ctx->side);
+  (void)hipEventDestroy(ctx->push_join_ev);
+  (void)hipEventDestroy(ctx->push_fork_ev);
+  (void)hipStreamDestroy(ctx->push);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LBX_OK;
