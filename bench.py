@@ -222,6 +222,35 @@ def serve(ctx, trace, max_active, policy=lbx.LBX_SERVE_FIFO, max_wait_s=0.0):
             "steps_launched": st["steps_launched"]}
 
 
+def refined_mix(ctx, max_active):
+    """A burst of 24 FP32-shifted requests at once: 16 single-level runs
+    (edges 32-48, 2000-4000 steps) and 8 two-level runs (coarse 64^3 in
+    boxes of 32^3, a fine slab over coarse x in [24, 40) and all y, z in
+    boxes of 32^3, 300-600 coarse steps, plane pulse at coarse x = 32), served
+    by one server; aggregate MLUPS over the makespan counts both levels'
+    updates."""
+    rng = np.random.default_rng(11)
+    fine = [((48 + a[0], a[1], a[2]), (48 + b[0], b[1], b[2])) for a, b in workloads.chop((32, 128, 128), 32)]
+    srv = lbx.Server(ctx, precision=lbx.LBX_FP32_SHIFTED, arena_bytes=8 << 30, max_active=max_active)
+    t0 = time.perf_counter()
+    for k in range(24):
+        if k % 3 == 2:
+            srv.submit((64, 64, 64), int(rng.integers(300, 601)), float(rng.uniform(0.6, 1.2)), amplitude=1e-3,
+                       boxes=workloads.chop((64, 64, 64), 32), fine_boxes=fine, plane=32)
+        else:
+            e = int(rng.choice([32, 48]))
+            srv.submit((e, e, e), int(rng.integers(2000, 4001)), float(rng.uniform(0.6, 1.2)), amplitude=1e-3)
+    srv.drain()
+    res = srv.poll()
+    makespan = time.perf_counter() - t0
+    lu = srv.stats()["lattice_updates"]
+    srv.close()
+    assert len(res) == 24 and all(r["status"] == lbx.LBX_OK for r in res)
+    lat = [r["t_done"] - r["t_submit"] for r in res]
+    return {"mlups": lu / makespan / 1e6, "makespan_s": makespan, "lattice_updates": lu,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3)}
+
+
 def two_level_geometry():
     """Two-level configuration: coarse 128^3 periodic (boxes of 32^3), fine
     level over coarse x in [48, 80) and all y, z (fine 64x256x256 in boxes of
@@ -536,12 +565,22 @@ def main():
             seq = serve(ctx, trace, max_active=1)
             log(f"[bench] sequential: {seq['mlups']:.0f} MLUPS, p50 {seq['p50_ms']:.0f} ms, "
                 f"p99 {seq['p99_ms']:.0f} ms, makespan {seq['makespan_s']:.1f} s")
+            log("[bench] serving: 24-request burst with 8 two-level runs, batched and one at a time")
+            mix_b = refined_mix(ctx, 44)
+            mix_s = refined_mix(ctx, 1)
+            log(f"[bench] refined mix: batched {mix_b['mlups']:.0f} MLUPS (makespan {mix_b['makespan_s']:.2f} s), "
+                f"one at a time {mix_s['mlups']:.0f} MLUPS ({mix_s['makespan_s']:.2f} s)")
             e_rho, e_u = serving_precision_sample(ctx, trace, 16)
             log(f"[bench] serving FP32 vs FP64 on the first 16 requests: max rel rho err {e_rho:.2e}, "
                 f"max |du|/amplitude {e_u:.2e}")
             serving = {"requests": len(trace), "arrival": "Poisson 200/s (seed 7)", "precision": "FP32-shifted",
                        "batched": batched, "batched_shortest_first": sjf, "sequential": seq,
                        "speedup_aggregate_mlups": round(batched["mlups"] / seq["mlups"], 3),
+                       "refined_mix": {"workload": "24 FP32-shifted requests at once: 16 single-level (32^3-48^3, "
+                                                   "2000-4000 steps) + 8 two-level (64^3 + fine slab 32x128x128, "
+                                                   "300-600 coarse steps)",
+                                       "batched": mix_b, "one_at_a_time": mix_s,
+                                       "speedup_aggregate_mlups": round(mix_b["mlups"] / mix_s["mlups"], 3)},
                        "fp32_vs_fp64_first_16": {"max_rel_rho_err": e_rho, "max_du_over_amplitude": e_u}}
         r32 = None
         if not args.no_fp32:
