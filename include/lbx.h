@@ -39,7 +39,8 @@ extern "C" {
 #define LBX_E_NOMEM (-4)    /* device or host allocation failed */
 #define LBX_E_UNSUPPORTED (-5)
 
-/* Levels 0 .. LBX_MAX_LEVELS-1, refinement ratio 2 between neighbours. */
+/* Levels 0 .. LBX_MAX_LEVELS-1, refinement ratio r between neighbours (2
+ * unless lbx_set_ref_ratio sets 3 or 4). */
 #define LBX_MAX_LEVELS 6
 
 #define LBX_FP64 0          /* populations stored and collided in FP64 */
@@ -83,12 +84,12 @@ int lbx_sim_destroy(lbx_sim* sim);
  * (src/AmrSim.cpp:665-713, include/multilevel.h:62-67). */
 int lbx_level_define(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi);
 
-/* Level l >= 1 (refinement ratio 2, index space [0, 2^l * dims)): level
+/* Level l >= 1 (refinement ratio r, index space [0, r^l * dims)): level
  * l-1 must be defined; boxes must lie in the level's domain, be aligned to
- * the ratio (lo even, hi odd), not overlap, and be properly nested: the
- * parent of every cell of each box grown by 2 ghost cells that no box of
- * level l holds (periodic images included), and of every valid cell, must
- * be a valid cell of level l-1 (LBX_E_ARG otherwise; AMReX builds its grids
+ * the ratio (lo and hi + 1 multiples of r), not overlap, and be properly
+ * nested: the parent of every cell of each box grown by 2 ghost cells that
+ * no box of level l holds (periodic images included), and of every valid
+ * cell, must be a valid cell of level l-1 (LBX_E_ARG otherwise; AMReX builds its grids
  * that way).  Each box is stored with 2 ghost layers like the reference's
  * FABs.  Defining a level releases every finer level and leaves level l-1
  * unmasked (lbx_set_level_mask sets its mask). */
@@ -137,17 +138,19 @@ int lbx_cluster_tags_allowed(const int32_t dims[3], const uint8_t* tags, const u
  * unmasked. */
 int lbx_level_clear(lbx_sim* sim, int level);
 
-/* Refinement ratio of level 1 to level 0: 2 (the default, the reference's
- * constructor fixes it, src/AmrSim.cpp:766), 3 or 4; set before level 1 is
- * defined.  Level 1's boxes are then given in the index space ratio * dims,
- * aligned to the ratio; their parents are floor(pos / ratio), the mask
- * boxes are coarsened by the ratio, the average-down runs over ratio^3
- * blocks times 1/ratio^3 over each box coarsened and grown by 2 / ratio
- * coarse cells (AMReX's sum_fine_to_coarse), and level 1 takes `ratio`
- * substeps per level-0 step (the reference writes ratio 2's two substeps
- * out, src/AmrSim.cpp:446-452, 621-624).  The caller sets level 1's tau as
- * ComputeDt does at that ratio (src/AmrSim.cpp:308-315).  Levels below
- * level 1 need ratio 2 (LBX_E_UNSUPPORTED otherwise). */
+/* Refinement ratio between consecutive levels, the same on every level: 2
+ * (the default, the reference's constructor fixes it, src/AmrSim.cpp:766),
+ * 3 or 4; set before level 1 is defined.  Level l's boxes are then given in
+ * the index space ratio^l * dims, aligned to the ratio; their parents are
+ * floor(pos / ratio), the mask boxes are coarsened by the ratio, the
+ * average-down runs over ratio^3 blocks times 1/ratio^3 over each box
+ * coarsened and grown by 2 / ratio coarse cells (AMReX's
+ * sum_fine_to_coarse), a level between 0 and the finest takes `ratio` steps
+ * per step of its parent (the REF_RATIO loop, src/AmrSim.cpp:453-455), and
+ * the finest level `ratio` substeps per step of its parent (the reference
+ * writes ratio 2's two substeps out, src/AmrSim.cpp:446-452, 621-624).  The
+ * caller sets each level's tau as ComputeDt does at that ratio
+ * (src/AmrSim.cpp:308-315). */
 int lbx_set_ref_ratio(lbx_sim* sim, int ratio);
 
 /* Relaxation rates of a level, omega = 1/(tau + 1/2) computed by the caller
@@ -156,8 +159,8 @@ int lbx_set_relax(lbx_sim* sim, int level, double omega_s, double omega_b);
 
 /* Mask of level `level`'s collide when it has a finer level (MakeFineMask,
  * src/AmrSim.cpp:419-428, used by CoarseCollide :487-500): its valid cells
- * inside coarsen(box, 2) of any of the given boxes (index space of level
- * level+1) have their populations zeroed instead of collided in the Rohde
+ * inside coarsen(box, r) of any of the given boxes (index space of level
+ * level+1, r the refinement ratio) have their populations zeroed instead of collided in the Rohde
  * cycle.  The reference passes the level's own BoxArray as these boxes;
  * passing the finer level's BoxArray masks the refined region.  A level
  * starts unmasked. */
@@ -184,9 +187,9 @@ int lbx_download_f(lbx_sim* sim, int level, int box, double* f);
  * single-level collide-stream step (src/AmrSim.cpp:124-135, 109-122,
  * 324-333) on every box, or, when finer levels are defined, one Rohde cycle
  * per step (src/AmrSim.cpp:430-631: masked collide and stream of each
- * level, two steps of each level per step of its parent, two substeps of
- * the finest level per step of its parent, average-down and add).  A level
- * l advances 2^l steps per call step.  Stream-ordered: may return before
+ * level, r steps of each level per step of its parent, r substeps of the
+ * finest level per step of its parent, average-down and add).  A level l
+ * advances r^l steps per call step.  Stream-ordered: may return before
  * the GPU finishes; lbx_sync() waits. */
 int lbx_iterate(lbx_sim* sim, int nsteps);
 

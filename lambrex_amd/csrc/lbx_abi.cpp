@@ -157,8 +157,8 @@ struct lbx_sim {
   unsigned walls = 0;  // bit d: bounce-back walls normal to axis d
   int precision = LBX_FP64;
   double rho0 = 1.0;
-  // refinement ratio of level 1 to level 0 (2, 3 or 4; deeper levels only
-  // at 2)
+  // refinement ratio between consecutive levels (2, 3 or 4; the same on
+  // every level)
   int ratio = 2;
   Level lev[LBX_MAX_LEVELS];
   StepGraphs graphs;
@@ -225,7 +225,6 @@ int wrap(int v, int n) {
   return r < 0 ? r + n : r;
 }
 int floor_div(int v, int r) { return v >= 0 ? v / r : -((-v + r - 1) / r); }
-int floor_div2(int v) { return floor_div(v, 2); }
 
 template <typename T>
 hipError_t upload(T** dptr, const std::vector<T>& h, hipStream_t st) {
@@ -565,7 +564,10 @@ hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStrea
 // the masked collide into the other buffer; the finer levels' steps, which
 // read m's state; then Stream + ZeroInvalidComponents back into the state
 // buffer and SumFromFine into its valid cells (one fused kernel).  The state
-// stays in buf[cur].
+// stays in buf[cur].  In the last of its parent's r steps (`last`) the
+// block averages of the result for the parent are formed too: in the same
+// pass at ratio 2, where they include 2 fine ghost layers, or by a pass over
+// the valid cells above it (nGrow = 2 / r = 0).
 hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipStream_t st, bool last) {
   Level& L = sim->lev[m];
   const Level& P = sim->lev[m - 1];
@@ -577,7 +579,13 @@ hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipSt
                                                P.patched ? P.d_desc : nullptr, L.d_desc, L.d_desc, 0, np,
                                                L.max_cells, L.d_fill, L.d_mask, r, st);
   if (e == hipSuccess) e = enqueue_children(sim, m, finest, fpar, st);
-  if (e == hipSuccess && last)  // the state the parent averages down: its block sums in the same pass
+  if (e == hipSuccess && last && L.ratio != 2) {
+    e = lbx::launch_fine_stream_zero(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_desc, 0, np, L.max_cells, sim->rho0,
+                                     F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start, F.d_sum_contrib, st);
+    if (e == hipSuccess)
+      e = lbx::launch_fine_avg_valid(prec, L.buf[L.cur], L.d_avg, L.avg_stride, L.d_desc, 0, np, L.max_blocks,
+                                     L.ratio, sim->rho0, st);
+  } else if (e == hipSuccess && last)  // the state the parent averages down: its block sums in the same pass
     e = lbx::launch_middle_stream_avg(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_avg, L.avg_stride, L.d_desc, np,
                                       L.max_blocks, sim->rho0, F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start,
                                       F.d_sum_contrib, st);
@@ -587,14 +595,15 @@ hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipSt
   return e;
 }
 
-// What level l's step needs of level l + 1: the finest level's two
-// substeps, or refRatio = 2 middle steps followed by the block averages of
-// the result (src/AmrSim.cpp:445-455).
+// What level l's step needs of level l + 1: the finest level's substeps,
+// or refRatio middle steps, the last followed by the block averages of the
+// result (src/AmrSim.cpp:445-455).
 hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStream_t st) {
   const int c = l + 1;
   if (c == finest) return enqueue_finest_pair(sim, c, fpar, st);
-  hipError_t e = enqueue_middle_step(sim, c, finest, fpar, st, false);
-  if (e == hipSuccess) e = enqueue_middle_step(sim, c, finest, fpar, st, true);
+  const int r = sim->lev[c].ratio;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < r && e == hipSuccess; ++k) e = enqueue_middle_step(sim, c, finest, fpar, st, k == r - 1);
   return e;
 }
 
@@ -661,9 +670,12 @@ int launch_hierarchy_step(lbx_sim* sim) {
     LBX_HIP(ctx, enqueue_hierarchy_step(sim, ctx->stream));
     if (int rc = ps.close()) return rc;
   }
-  // the finest level takes 2^(finest-1) substep pairs per level-0 step; its
-  // buffer parity only changes when that count is odd
-  if (finest == 1) F.cur = 1 - F.cur;
+  // the finest level's substeps run once per step of its parent, which takes
+  // r^(finest-1) steps per level-0 step (the ratios of the levels between);
+  // each run flips its buffer, so its parity changes when that count is odd
+  long long runs = 1;
+  for (int l = 1; l < finest; ++l) runs *= sim->lev[l].ratio;
+  if (runs % 2) F.cur = 1 - F.cur;
   C.cur = 1 - C.cur;
   return LBX_OK;
 }
@@ -779,8 +791,8 @@ bool cell_ref(const Level& C, int x, int y, int z, long long* ref, int& hint) {
 }
 
 // Mask bytes of a patched level, one per grown cell: 1 where a valid cell
-// lies in coarsen(mask box, 2) for some mask box (finer index space).
-std::vector<unsigned char> patch_mask(const Level& C, const std::vector<Box>& mask) {
+// lies in coarsen(mask box, r) for some mask box (finer index space).
+std::vector<unsigned char> patch_mask(const Level& C, const std::vector<Box>& mask, int r) {
   std::vector<unsigned char> m((size_t)C.grown_cells, 0);
   for (size_t b = 0; b < C.boxes.size(); ++b) {
     const PatchDesc& P = C.desc[b];
@@ -788,8 +800,8 @@ std::vector<unsigned char> patch_mask(const Level& C, const std::vector<Box>& ma
       int lo[3], hi[3];
       bool empty = false;
       for (int d = 0; d < 3; ++d) {
-        lo[d] = std::max(P.lo[d], floor_div2(mb.lo[d]));
-        hi[d] = std::min(P.hi[d], floor_div2(mb.hi[d]));
+        lo[d] = std::max(P.lo[d], floor_div(mb.lo[d], r));
+        hi[d] = std::min(P.hi[d], floor_div(mb.hi[d], r));
         empty |= lo[d] > hi[d];
       }
       if (empty) continue;
@@ -809,11 +821,9 @@ int build_level(lbx_sim* sim, int level, const std::vector<Box>& boxes, Level& F
   Level& C = sim->lev[level - 1];
   if (!C.defined) return fail(ctx, LBX_E_STATE, "define level %d before level %d", level - 1, level);
   if (sim->walls) return fail(ctx, LBX_E_UNSUPPORTED, "refined levels need a fully periodic domain");
-  if (level >= 2 && sim->ratio != 2)
-    return fail(ctx, LBX_E_UNSUPPORTED, "levels below level 1 need refinement ratio 2 (this simulation has %d)",
-                sim->ratio);
-  const int r = level == 1 ? sim->ratio : 2;
-  const long long scale = level == 1 ? r : (1ll << level);
+  const int r = sim->ratio;
+  long long scale = 1;
+  for (int l = 0; l < level; ++l) scale *= r;
   int fdom[3];
   for (int d = 0; d < 3; ++d) {
     if ((long long)sim->dims[d] * scale > (1ll << 30))
@@ -1039,7 +1049,7 @@ int unmask_level(lbx_sim* sim, int level) {
   lbx_ctx* ctx = sim->ctx;
   Level& C = sim->lev[level];
   if (!C.defined) return LBX_OK;
-  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, {}) : coarse_flags(C, {}, sim->ratio);
+  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, {}, sim->ratio) : coarse_flags(C, {}, sim->ratio);
   if (!C.d_mask) LBX_HIP(ctx, hipMalloc(&C.d_mask, bytes.size()));
   LBX_HIP(ctx, hipMemcpyAsync(C.d_mask, bytes.data(), bytes.size(), hipMemcpyHostToDevice, ctx->stream));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1419,7 +1429,7 @@ int lbx_set_level_mask(lbx_sim* sim, int level, int n_boxes, const int32_t* lohi
   if (!C.defined) return fail(ctx, LBX_E_STATE, "level %d is not defined", level);
   if (n_boxes < 0 || (n_boxes > 0 && !lohi_fine)) return fail(ctx, LBX_E_ARG, "bad mask boxes");
   const std::vector<Box> mask = read_boxes(n_boxes, lohi_fine);
-  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, mask) : coarse_flags(C, mask, sim->ratio);
+  const std::vector<unsigned char> bytes = C.patched ? patch_mask(C, mask, sim->ratio) : coarse_flags(C, mask, sim->ratio);
   LBX_HIP(ctx, hipSetDevice(ctx->device));
   LBX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (!C.d_mask) {

@@ -15,21 +15,22 @@
 #include <string>
 
 namespace {
-// ratio 2 on every level (src/AmrSim.cpp:766) unless SetRefRatio changes
-// level 1's
+// ratio 2 on every level (src/AmrSim.cpp:766) unless SetRefRatio sets
+// another one for every level
 constexpr int kRefRatio = 2;
 }
 
 void AmrSim::SetRefRatio(int ratio) {
   if (sim_) throw lambrex::Error("SetRefRatio must be called before InitFromScratch");
   if (ratio < 2 || ratio > 4) throw lambrex::Error("SetRefRatio: the ratio must be 2, 3 or 4");
-  if (ratio != kRefRatio && max_level_ > 1)
-    throw lambrex::Error("SetRefRatio: a ratio other than 2 needs max_level <= 1");
   ref_ratio_ = ratio;
 }
 
 // Ratio between level `level` and level `level` + 1.
-int AmrSim::Ratio(int level) const { return level == 0 ? ref_ratio_ : kRefRatio; }
+int AmrSim::Ratio(int level) const {
+  (void)level;
+  return ref_ratio_;
+}
 
 AmrSim::AmrSim(int nx, int ny, int nz, int max_ref_level,
                const std::array<int, NDIMS>& periodicity, double tau_s, double tau_b)
@@ -260,20 +261,20 @@ void AmrSim::Iterate(int nsteps) {
     if (nsteps % 2) C.cur = 1 - C.cur;  // UpdateNow swaps Density too
   } else {
     // UpdateDistribution (:619-631): a level between 0 and the finest takes
-    // one step per RohdeCycle, 2^l cycles per level-0 step (levels below 1
-    // exist at ratio 2 only); the finest level takes ratio steps per cycle
-    // of its parent (two at ratio 2, as the reference writes out).
+    // one step per RohdeCycle, r^l cycles per level-0 step (the product of
+    // the ratios above it); the finest level takes r steps per cycle of its
+    // parent (two at ratio 2, as the reference writes out).
     for (int t = 0; t < nsteps; ++t)
-      for (int l = 0; l <= finest_level_; ++l) {
+      for (int l = 0, cycles = 1; l <= finest_level_; cycles *= Ratio(l), ++l) {
         LevelHost& L = levels_.at(l);
         if (l > 0 && l == finest_level_) {
           const int r = Ratio(l - 1);  // substeps per cycle of its parent
-          for (int c = 0; c < (1 << (l - 1)); ++c) {
+          for (int c = 0; c < cycles / r; ++c) {
             L.time += r * L.dt;
             L.step += r;
           }
         } else {
-          for (int c = 0; c < (1 << l); ++c) {
+          for (int c = 0; c < cycles; ++c) {
             L.time += L.dt;
             ++L.step;
           }
@@ -809,8 +810,7 @@ void AmrSim::ReadCheckpoint(const std::string& dir) {
   const bool mask_fine = r.integer() != 0;
   r.expect("ref_ratio");
   const long long ratio = r.integer();
-  if (ratio < 2 || ratio > 4 || (ratio != kRefRatio && max_level_ > 1))
-    throw lambrex::Error("ReadCheckpoint: bad refinement ratio");
+  if (ratio < 2 || ratio > 4) throw lambrex::Error("ReadCheckpoint: bad refinement ratio");
   std::vector<lambrex::IBox> tags(max_level_ + 1);
   for (int l = 0; l <= max_level_; ++l) {
     r.expect("static_tag");

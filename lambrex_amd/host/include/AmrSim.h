@@ -7,8 +7,8 @@
 //     direction (src/AmrSim.cpp:788-797), this build puts halfway
 //     bounce-back walls on its faces (single-level simulations only);
 //   - up to LBX_MAX_LEVELS levels (max_ref_level < LBX_MAX_LEVELS), ratio 2
-//     (or 3 or 4 between levels 0 and 1 of a two-level simulation,
-//     SetRefRatio), advanced by the reference's recursive Rohde cycle;
+//     (or 3 or 4 on every level, SetRefRatio), advanced by the reference's
+//     recursive Rohde cycle;
 //   - initial density / velocity vectors in C order, i slowest, velocity
 //     component fastest (include/AmrSim.h:79-83);
 //   - omega = 1/(tau + 1/2) (src/AmrSim.cpp:125-126);
@@ -96,11 +96,12 @@ class AmrSim {
   void SetMaxGridSize(int max_grid_size);
   // LBX_FP64 (default) or LBX_FP32_SHIFTED with reference density rho0.
   void SetPrecision(int precision, double rho0);
-  // Refinement ratio of level 1 to level 0: 2 (the reference's, fixed in its
-  // constructor, src/AmrSim.cpp:766), 3 or 4; a ratio other than 2 needs
-  // max_level <= 1.  Before InitFromScratch.  Level 1 then takes `ratio`
-  // substeps per level-0 step (dt / ratio, tau rescaled by ComputeDt at that
-  // ratio) and Regrid is unavailable (its clustering refines by 2).
+  // Refinement ratio between consecutive levels, the same on every level: 2
+  // (the reference's, fixed in its constructor, src/AmrSim.cpp:766), 3 or 4.
+  // Before InitFromScratch.  A level then takes `ratio` steps per step of
+  // its parent (dt / ratio, tau rescaled by ComputeDt at that ratio) and
+  // Regrid is unavailable (its clustering refines by 2; static refinement
+  // works at any ratio).
   void SetRefRatio(int ratio);
   // Mask the coarse collide with the fine BoxArray instead of the coarse one
   // the reference passes to makeFineMask (src/AmrSim.cpp:419-428).

@@ -278,20 +278,21 @@ class Sim:
         return tags
 
     def set_ref_ratio(self, ratio):
-        """Refinement ratio (2, 3 or 4) of level 1 to level 0, before level 1 is defined."""
+        """Refinement ratio (2, 3 or 4) between consecutive levels, the same on
+        every level; before level 1 is defined."""
         self.ctx.check(self._lib.lbx_set_ref_ratio(self.h, int(ratio)))
 
     def set_relax(self, level, omega_s, omega_b):
         self.ctx.check(self._lib.lbx_set_relax(self.h, level, float(omega_s), float(omega_b)))
 
     def set_fine_mask(self, boxes):
-        """Coarse cells under coarsen(box, 2) of these (fine index space)
+        """Coarse cells under coarsen(box, r) of these (fine index space)
         boxes are masked in the two-level coarse collide."""
         lohi = _lohi(boxes) if boxes else np.zeros(6, dtype=np.int32)
         self.ctx.check(self._lib.lbx_set_fine_mask(self.h, len(boxes), _ptr(lohi, ctypes.c_int32)))
 
     def set_level_mask(self, level, boxes):
-        """Cells of `level` under coarsen(box, 2) of these boxes (index space
+        """Cells of `level` under coarsen(box, r) of these boxes (index space
         of level + 1) are masked in that level's collide."""
         lohi = _lohi(boxes) if boxes else np.zeros(6, dtype=np.int32)
         self.ctx.check(self._lib.lbx_set_level_mask(self.h, level, len(boxes), _ptr(lohi, ctypes.c_int32)))

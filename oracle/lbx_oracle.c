@@ -419,7 +419,9 @@ static void swap_now_next(orc_level* L) {
  *   9 ZeroInvalid keeps pos + 2c in valid    10 ... keeps pos - c in valid
  *  11 average-down over the valid boxes only (nGrow 0) at every ratio
  *  12 average-down volume fraction 1/8 at every ratio
- *  13 the finest level takes 2 substeps at every ratio */
+ *  13 the finest level takes 2 substeps at every ratio
+ *  14 a level between 0 and the finest takes 2 steps per parent step at
+ *     every ratio */
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
 #endif
@@ -658,7 +660,7 @@ static void rohde_cycle(orc_sim* s, int l) {
   if (l + 1 == s->nlev - 1) {
     finest_pair(L, F);
   } else {
-    for (int iter = 0; iter < F->ratio; ++iter) rohde_cycle(s, l + 1);  /* REF_RATIO, :453-455 */
+    for (int iter = 0; iter < (ORC_MUTANT == 14 ? 2 : F->ratio); ++iter) rohde_cycle(s, l + 1);  /* REF_RATIO, :453-455 */
   }
   stream_level(L, &L->next);
   sum_fine_to_coarse(F, L, L->next);

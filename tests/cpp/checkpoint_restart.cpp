@@ -4,8 +4,9 @@
 // level as one AmrSim that runs Iterate(a + b), with the same time, step
 // count and (after CalcHydroVars) density.  Single level, two levels with the
 // reference's mask (src/AmrSim.cpp:419-428) and with the fine-grid mask,
-// three levels, and FP32-shifted storage.  The reference has no I/O; the
-// stepping it continues is its Iterate (src/AmrSim.cpp:981-993).
+// three levels (at ratios 2 and 3), and FP32-shifted storage.  The
+// reference has no I/O; the stepping it continues is its Iterate
+// (src/AmrSim.cpp:981-993).
 #include <unistd.h>
 
 #include <cmath>
@@ -140,6 +141,14 @@ TEST_CASE("Checkpoint two levels at refinement ratio 3") {
   s.fine_mask = true;
   s.ratio = 3;
   restart_matches(s, "ratio3", 2, 2);
+}
+
+TEST_CASE("Checkpoint three levels at refinement ratio 3") {
+  Setup s;
+  s.max_level = 2;
+  s.fine_mask = true;
+  s.ratio = 3;
+  restart_matches(s, "ratio3_three", 1, 2);
 }
 
 TEST_CASE("Checkpoint FP32-shifted two levels") {

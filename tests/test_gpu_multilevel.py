@@ -121,6 +121,58 @@ def test_four_levels_bitwise(ctx, mask):
         assert_bitwise(want[l], got[l], f"level {l} f")
 
 
+def three_levels_ratio(r):
+    """Three levels at ratio r: coarse 12x9x6 in boxes of 6^3; level 1 over
+    coarse x in [3, 9), all y and z (boxes of 3r); level 2 over level-1 x in
+    [4r, 6r), y in [r, 4r), all z (wrapping onto itself), plus one box over
+    level-1 x in [7r, 8r), y in [6r, 7r), z in [r, 2r)."""
+    dims = (12, 9, 6)
+    l0 = workloads.chop(dims, 6)
+    l1 = fine_boxes_in((3 * r, 0, 0), (9 * r - 1, 9 * r - 1, 6 * r - 1), 3 * r)
+    l2 = fine_boxes_in((4 * r * r, r * r, 0), (6 * r * r - 1, 4 * r * r - 1, 6 * r * r - 1), 3 * r) + \
+        [((7 * r * r, 6 * r * r, r * r), (8 * r * r - 1, 7 * r * r - 1, 2 * r * r - 1))]
+    return dims, [l0, l1, l2]
+
+
+@pytest.mark.parametrize("mask", ["reference", "fine"])
+def test_three_levels_ratio_3_bitwise(ctx, mask):
+    """Ratio 3 on every level (3 middle steps per level-0 step, 3 finest
+    substeps per middle step, 27-cell block averages into a patched parent
+    and into level 0): all three levels after 2 level-0 steps equal the
+    oracle's at the same ratio, bit for bit."""
+    r = 3
+    dims, levels = three_levels_ratio(r)
+    masks = masks_for(mask, levels)
+    rho, u = workloads.random_state(dims, seed=79, rho_sigma=1e-2, u_amp=0.02)
+    o = orc.Sim(*dims, 0.8, boxes=lists(levels[0]))
+    o.set_ratio(r)
+    o.init(*c_order(rho, u))
+    for l in (1, 2):
+        o.add_fine(lists(levels[l]), lists(masks[l - 1]))
+    o.iterate(2)
+    want = [o.get_f(l) for l in range(3)]
+    t = [o.tau(l) for l in range(3)]
+    assert o.step(2) == 2 * r * r
+    o.close()
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, levels[0])
+        sim.set_tau(0, t[0])
+        sim.init_from_fields(rho, u)
+        sim.set_ref_ratio(r)
+        for l in (1, 2):
+            sim.define_level(l, levels[l])
+            sim.set_tau(l, t[l])
+            sim.fill_from_coarse(l)
+            sim.set_level_mask(l - 1, masks[l - 1])
+        sim.iterate(2)
+        got = gpu_levels(sim, 3)
+    finally:
+        sim.close()
+    for l in range(3):
+        assert_bitwise(want[l], got[l], f"ratio {r}, level {l} f")
+
+
 def test_fill_from_patched_parent_is_piecewise_constant(ctx):
     """DistFnFillFromCoarse of level 2 copies each cell's level-1 parent."""
     dims, levels = three_levels()

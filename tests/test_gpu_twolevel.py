@@ -466,8 +466,9 @@ def test_refinement_ratio_3_fp32_tracks_fp64(ctx):
 
 
 def test_refinement_ratio_is_checked(ctx):
-    """Ratio outside 2..4, a ratio change once level 1 exists, boxes not
-    aligned to the ratio and a level 2 at ratio 3 are refused."""
+    """Ratio outside 2..4, a ratio change once level 1 exists and boxes not
+    aligned to the ratio (on level 1 or level 2) are refused; an aligned,
+    nested level 2 at the same ratio is accepted."""
     dims = (12, 9, 6)
     sim = lbx.Sim(ctx, dims)
     try:
@@ -481,7 +482,9 @@ def test_refinement_ratio_is_checked(ctx):
         with pytest.raises(lbx.LbxError):
             sim.set_ref_ratio(2)
         with pytest.raises(lbx.LbxError) as e:
-            sim.define_level(2, [((20, 2, 2), (23, 5, 5))])
-        assert e.value.code == lbx.LBX_E_UNSUPPORTED
+            sim.define_level(2, [((31, 3, 3), (38, 11, 11))])  # lo not a multiple of 3
+        assert e.value.code == lbx.LBX_E_ARG
+        # level-2 cells 28..40 (grown by 2) have level-1 parents 9..13, inside 9..17
+        sim.define_level(2, [((30, 3, 3), (38, 11, 11))])
     finally:
         sim.close()

@@ -5,14 +5,17 @@ No reference test runs more than two levels.  The recursion is pinned by:
   * an exact reduction: when level 1 covers the whole domain, level 0 never
     feeds it (every level-1 ghost has a level-1 periodic image), so levels
     1..n must evolve bit for bit like the hierarchy one level shallower
-    whose level 0 is that level 1, stepped twice as often;
+    whose level 0 is that level 1, stepped r times as often (ratios 2, 3
+    and 4; a planted two-step middle level fails it at ratio 3);
   * a second restatement of the recursion written here with numpy arrays per
     FAB (the two-level N-version of test_oracle_twolevel.py, made
     recursive), on a hierarchy whose level 1 does not cover the domain;
   * the time, step and relaxation bookkeeping of ComputeDt and
-    UpdateDistribution (src/AmrSim.cpp:297-322, 619-631);
+    UpdateDistribution (src/AmrSim.cpp:297-322, 619-631), at ratios 2 and 3;
   * refusal of a level that is not properly nested in its parent.
 """
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -173,6 +176,89 @@ def test_full_level1_reduces_to_shallower_hierarchy(nlev):
                 f"level {l}, step {step + 1}"
     deep.close()
     shallow.close()
+
+
+def check_ratio_reduction(library, r):
+    """The same reduction with every ratio r: three levels at ratio r whose
+    level 1 covers the domain against two levels at ratio r started from that
+    level 1 and stepped r times as often.  It fixes the REF_RATIO loop of a
+    middle level (r steps per parent step) and the r^3 average-down into a
+    patched parent.  Returns the failed comparisons."""
+    fails = []
+    dims = (4, 4, 2)
+    tau = 0.8
+    b0 = [(tuple(a), tuple(b)) for a, b in workloads.chop(dims, 4)]
+    d1 = tuple(r * d for d in dims)
+    b1 = [(tuple(a), tuple(b)) for a, b in workloads.chop(d1, 2 * r)]
+    # level 2 (index space r^2 * dims) inside level 1 (whole domain): one
+    # box over level-1 cells x in [r, 3r), y in [0, 2r), all z (wrapping onto
+    # itself), one over x in [3r, 4r), y in [2r, 4r), z in [0, r)
+    deeper = [((r * r, 0, 0), (3 * r * r - 1, 2 * r * r - 1, r * d1[2] - 1)),
+              ((3 * r * r, 2 * r * r, 0), (4 * r * r - 1, 4 * r * r - 1, r * r - 1))]
+    rho, u = workloads.random_state(dims, seed=5 + r, rho_sigma=1e-2, u_amp=0.03)
+    deep = orc.Sim(*dims, tau, boxes=as_lists(b0), library=library)
+    deep.set_ratio(r)
+    deep.init(*c_order(rho, u))
+    deep.add_fine(as_lists(b1), as_lists(b0))
+    rng = np.random.default_rng(19)
+    f1 = deep.get_f(1)
+    deep.set_f(f1 * (1.0 + 1e-3 * rng.standard_normal(f1.shape)), 1)
+    deep.add_fine(as_lists(deeper), as_lists(b1))
+
+    tau1 = r * (tau - 0.5) + 0.5
+    assert deep.tau(1) == tau1
+    shallow = orc.Sim(*d1, tau1, boxes=as_lists(b1), library=library)
+    shallow.set_ratio(r)
+    n1 = d1[0] * d1[1] * d1[2]
+    shallow.init(np.ones(n1), np.zeros(3 * n1))
+    shallow.set_f(deep.get_f(1), 0)
+    shallow.add_fine(as_lists(deeper), as_lists(b1))
+    assert (shallow.get_f(1).view(np.uint64) == deep.get_f(2).view(np.uint64)).all()
+
+    for step in range(2):
+        deep.iterate(1)
+        shallow.iterate(r)
+        for l in (1, 2):
+            if not (deep.get_f(l).view(np.uint64) == shallow.get_f(l - 1).view(np.uint64)).all():
+                fails.append(f"ratio {r}, level {l}, step {step + 1}")
+    deep.close()
+    shallow.close()
+    return fails
+
+
+@pytest.mark.parametrize("r", [3, 4])
+def test_full_level1_reduces_at_ratio(r):
+    assert check_ratio_reduction(orc.lib(), r) == []
+
+
+def test_ratio_reduction_fails_on_a_two_step_middle_level(tmp_path):
+    """Planted mistake (ORC_MUTANT 14, lbx_oracle.c): a middle level stepped
+    twice per parent step at every ratio fails the reduction."""
+    out = tmp_path / "liblbx_oracle_m14.so"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-std=c11", "-DORC_MUTANT=14",
+                    "-shared", "-o", str(out), str(orc._HERE / "lbx_oracle.c")], check=True)
+    assert check_ratio_reduction(orc.load(out), 3) != []
+
+
+def test_multilevel_bookkeeping_at_ratio_3():
+    """ComputeDt and UpdateDistribution at ratio 3: level l takes 3^l steps
+    per level-0 step with dt 3^-l and tau_l = 3 (tau_{l-1} - 1/2) + 1/2."""
+    dims = (4, 4, 4)
+    tau = 0.6
+    o = orc.Sim(*dims, tau, boxes=[([0, 0, 0], [3, 3, 3])])
+    o.set_ratio(3)
+    rho, u = workloads.random_state(dims, seed=2, rho_sigma=1e-3, u_amp=0.01)
+    o.init(*c_order(rho, u))
+    o.add_fine([([3, 3, 3], [8, 8, 8])], [([0, 0, 0], [3, 3, 3])])
+    o.add_fine([([12, 12, 12], [20, 20, 20])], [([3, 3, 3], [8, 8, 8])])
+    o.iterate(2)
+    t = tau
+    for l in range(3):
+        assert o.step(l) == 2 * 3 ** l
+        assert o.time(l) == pytest.approx(2.0, rel=1e-14)
+        assert o.tau(l, 0) == t
+        t = 3 * (t - 0.5) + 0.5
+    o.close()
 
 
 def test_multilevel_bookkeeping():
