@@ -301,7 +301,10 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
 // added in (k, j, i) order in FP64, to avg[q * avg_stride + avg_base + block].
 // A workgroup takes 32 consecutive blocks; thread t computes child t / 32 of
 // block t % 32, so each wavefront reads whole x-pairs of its blocks.  The
-// children meet in LDS for the ordered sums.  With SUM (a middle level's
+// children meet in LDS for the ordered sums, as stored values (S): an FP32
+// workgroup then needs half the LDS of an FP64 one, and each child is
+// converted to FP64 f as it is added (the same bits as converting first).
+// With SUM (a middle level's
 // last step of its parent's step) the stream is k_fine_stream_zero's, the
 // child's block averages (slot, cavg, start, contrib) added into valid
 // cells, and the block sums are those of the level's resulting state.
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
                                                          long long cavg_stride = 0,
                                                          const int* __restrict__ start = nullptr,
                                                          const int* __restrict__ contrib = nullptr) {
-  __shared__ double part[NV][8][kAvgBlocks];
+  __shared__ S part[NV][8][kAvgBlocks];
   const PatchDesc P = patches[first + blockIdx.y];
   const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
   const int nblk = c0 * c1 * c2;
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
       if (SUM || valid) out[P.base + q * P.qs + i] = f[q];
-      part[q][child][lb] = to_f64(f[q], q, rho0);
+      part[q][child][lb] = f[q];
     }
   }
   __syncthreads();
@@ -357,7 +360,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     if (b0 + b >= nblk) continue;
     double sum = 0.0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) sum += part[q][c][b];
+    for (int c = 0; c < 8; ++c) sum += to_f64(part[q][c][b], q, rho0);
     avg[q * avg_stride + P.avg_base + b0 + b] = sum * 0.125;
   }
 }
