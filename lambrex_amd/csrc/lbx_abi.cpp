@@ -372,7 +372,8 @@ int tune_shape(lbx_ctx* ctx, const Lattice& L, int precision, LaunchShape* out) 
     return LBX_OK;
   }
   const Lattice P =
-      lbx::make_lattice(L.nx, L.ny, std::min(L.nz, std::max(4, (1 << 24) / std::max(1, L.nx * L.ny))));
+      lbx::make_lattice(L.nx, L.ny, std::min(L.nz, std::max(4, (1 << 24) / std::max(1, L.nx * L.ny))),
+                        (int)elem_size(precision));
   const size_t bytes = (size_t)P.elems() * elem_size(precision);
   DeviceTemp a, b;
   LBX_HIP(ctx, hipMalloc(&a.p, bytes));
@@ -703,7 +704,7 @@ int define_level0(lbx_sim* sim, const std::vector<Box>& boxes) {
   Level& L = sim->lev[0];
   free_levels_from(sim, 0);
   L.boxes = boxes;
-  L.lat = lbx::make_lattice(sim->dims[0], sim->dims[1], sim->dims[2]);
+  L.lat = lbx::make_lattice(sim->dims[0], sim->dims[1], sim->dims[2], (int)elem_size(sim->precision));
   L.lat.walls = sim->walls;
   const size_t bytes = (size_t)L.lat.elems() * elem_size(sim->precision);
   for (int i = 0; i < 2; ++i) {
@@ -1677,7 +1678,7 @@ int lbx_tuned_shape(lbx_ctx* ctx, int nx, int ny, int precision, int32_t* bx, in
   auto it = ctx->shapes.find(std::make_tuple(nx, ny, precision));
   if (it == ctx->shapes.end()) {  // never tuned: launches use the default shape
     LBX_HIP(ctx, hipSetDevice(ctx->device));
-    const LaunchShape d = lbx::default_shape(lbx::make_lattice(nx, ny, 1), precision);
+    const LaunchShape d = lbx::default_shape(lbx::make_lattice(nx, ny, 1, (int)elem_size(precision)), precision);
     *bx = d.bx;
     *by = d.by;
     if (wg_per_cu) *wg_per_cu = d.cap;

@@ -191,7 +191,7 @@ long long cells_of(const lbx_request& q) { return (long long)q.dims[0] * q.dims[
 // block averages, and the parent's touched-cell slots; 1/8 more for the
 // smaller tables.
 size_t refined_bytes_estimate(const Request& q, int precision) {
-  const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2]);
+  const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2], (int)esize(precision));
   const long long coarse_cells = (long long)L.nx * L.ny * L.nz;
   size_t bytes = 2 * (size_t)L.elems() * esize(precision) + (size_t)coarse_cells * 5;
   for (size_t b = 0; b + 5 < q.fine.size(); b += 6) {
@@ -332,7 +332,7 @@ int admit(lbx_server* s) {
       s->active.push_back(std::move(r));
       continue;
     }
-    const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2]);
+    const lbx::Lattice L = lbx::make_lattice(q.req.dims[0], q.req.dims[1], q.req.dims[2], (int)esize(s->precision));
     const size_t bytes = (size_t)2 * L.elems() * esize(s->precision);
     size_t off = 0;
     if (!s->arena.alloc(bytes, &off)) {
@@ -689,7 +689,7 @@ int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
   if (req->steps < 1) return sfail(s, LBX_E_ARG, "steps must be >= 1");
   if (!(req->tau_s > -0.5) || !(req->tau_b > -0.5) || !std::isfinite(req->tau_s) || !std::isfinite(req->tau_b))
     return sfail(s, LBX_E_ARG, "tau must be finite and > -1/2");
-  const lbx::Lattice L = lbx::make_lattice(req->dims[0], req->dims[1], req->dims[2]);
+  const lbx::Lattice L = lbx::make_lattice(req->dims[0], req->dims[1], req->dims[2], (int)esize(s->precision));
   if (!ref && (size_t)2 * L.elems() * esize(s->precision) > s->arena.size)
     return sfail(s, LBX_E_NOMEM, "run does not fit the serving arena");
   if (req->init_kind != LBX_INIT_PLANE_PULSE && req->init_kind != LBX_INIT_FIELDS)
