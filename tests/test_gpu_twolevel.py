@@ -97,6 +97,21 @@ def test_pulse_two_level_bitwise(ctx, mask, steps):
     assert_bitwise(fo1, fg1, "fine f")
 
 
+def test_padded_coarse_lattice_bitwise(ctx):
+    """A coarse level whose FP64 population planes are 512 KB (256 x 256),
+    stored with its planes padded by 1 KB: the push, the sums into it, the
+    fills from it and the fine level's first fill all address it through its
+    strides.  Coarse 256x256x4 in boxes of 64 x 64 x 4; fine level over coarse
+    x in [100, 108), y in [120, 136), all z; 2 coarse steps against the
+    oracle, bit for bit."""
+    dims = (256, 256, 4)
+    rho, u = workloads.random_state(dims, seed=29, rho_sigma=1e-2, u_amp=0.02)
+    (fo0, fo1, _), (fg0, fg1, _, _), _ = run_both(ctx, dims, 64, 0.8, rho, u, (200, 240, 0), (215, 271, 7), 8,
+                                                  "fine", 2)
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
+
+
 def test_random_two_level_bitwise_uneven_boxes(ctx):
     """Random state; coarse 12x10x8 cut at 5; fine region in the middle of the
     domain, cut at 6 (boxes of different sizes, ghost fill from neighbours,
