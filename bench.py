@@ -134,6 +134,21 @@ def step_traffic(key, workload):
         return None
 
 
+def hierarchy_roofline(mlups, ms_per_step, precision, traffic):
+    """Where a hierarchy step sits against the 8 TB/s HBM peak: its
+    algorithmic bytes (240 B or 120 B per lattice update) per second, and,
+    for FP64, the measured bytes per step (every kernel's FETCH_SIZE +
+    WRITE_SIZE, which also count lines served by the MALL) per second."""
+    bpl = 240 if precision == lbx.LBX_FP64 else 120
+    alg = mlups * 1e6 * bpl / 1e9
+    out = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic_gbs": round(alg, 1),
+           "algorithmic_frac": round(alg / HBM_PEAK_GBS, 4)}
+    if traffic:
+        fab = traffic["bytes_per_step"] / (ms_per_step * 1e-3) / 1e9
+        out.update({"measured_gbs": round(fab, 1), "measured_frac": round(fab / HBM_PEAK_GBS, 4)})
+    return out
+
+
 def cpu_baseline(dims, mgs, tau, rho, u, steps):
     """The CPU oracle (the reference's algorithm restated in C, serial, same
     FillPatch / collide / FillBoundary / stream structure) on the same
@@ -297,13 +312,14 @@ def two_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
         sim.close()
     lu = lu_per_step * steps
     name = "FP64" if precision == lbx.LBX_FP64 else "FP32-shifted"
+    traffic = step_traffic("two_level_step", "128^3 + 64x256x256") if precision == lbx.LBX_FP64 else None
     return {"workload": "coarse 128^3 (64 boxes of 32^3) + fine 64x256x256 (128 boxes of 32^3), "
                         f"ratio 2, 2 fine substeps per coarse step, {name}",
             "coarse_steps": steps, "warmup_steps": warmup, "mlups": lu / (t1 - t0) / 1e6,
             "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
             "graph": "one coarse step captured as a HIP graph",
-            "traffic_per_step": (step_traffic("two_level_step", "128^3 + 64x256x256")
-                                 if precision == lbx.LBX_FP64 else None)}, fields
+            "traffic_per_step": traffic,
+            "roofline": hierarchy_roofline(lu / (t1 - t0) / 1e6, (t1 - t0) * 1e3 / steps, precision, traffic)}, fields
 
 
 def three_level_geometry():
@@ -353,13 +369,15 @@ def three_level(ctx, steps, warmup, precision=lbx.LBX_FP64):
         sim.close()
     lu = lu_per_step * steps
     name = "FP64" if precision == lbx.LBX_FP64 else "FP32-shifted"
+    traffic = (step_traffic("three_level_step", "128^3 + 64x256x256 + 32x512x512")
+               if precision == lbx.LBX_FP64 else None)
     return {"workload": "coarse 128^3 (64 boxes of 32^3) + level 1 64x256x256 (128 boxes of 32^3) + level 2 "
                         f"32x512x512 (256 boxes of 32^3), ratio 2, {name}",
             "coarse_steps": steps, "warmup_steps": warmup, "mlups": lu / (t1 - t0) / 1e6,
             "ms_per_coarse_step": (t1 - t0) * 1e3 / steps,
             "graph": "one level-0 step captured as a HIP graph",
-            "traffic_per_step": (step_traffic("three_level_step", "128^3 + 64x256x256 + 32x512x512")
-                                 if precision == lbx.LBX_FP64 else None),
+            "traffic_per_step": traffic,
+            "roofline": hierarchy_roofline(lu / (t1 - t0) / 1e6, (t1 - t0) * 1e3 / steps, precision, traffic),
             "final_rho_range_per_level": [[float(np.min(r)), float(np.max(r))] for r in rhos]}, rhos
 
 
@@ -523,7 +541,9 @@ def main():
         if args.two_level_steps:
             log(f"[bench] two-level: {args.two_level_steps} coarse steps")
             twolev, f64_fields = two_level(ctx, args.two_level_steps, 5)
-            log(f"[bench] two-level: {twolev['mlups']:.0f} MLUPS, {twolev['ms_per_coarse_step']:.3f} ms per coarse step")
+            rf = twolev["roofline"]
+            log(f"[bench] two-level: {twolev['mlups']:.0f} MLUPS, {twolev['ms_per_coarse_step']:.3f} ms per coarse step, "
+                f"{rf['algorithmic_gbs']:.0f} GB/s algorithmic, {rf.get('measured_gbs', float('nan')):.0f} GB/s measured")
             if not args.no_fp32:
                 tl32, f32_fields = two_level(ctx, args.two_level_steps, 5, lbx.LBX_FP32_SHIFTED)
                 e_rho, e_u = precision_errors(f64_fields, f32_fields)
@@ -545,6 +565,7 @@ def main():
                 t32, r32s = three_level(ctx, args.three_level_steps, 5, lbx.LBX_FP32_SHIFTED)
                 e_rho = max(float(np.max(np.abs(b - a) / a)) for a, b in zip(r64s, r32s))
                 threelev["fp32_shifted"] = {"mlups": t32["mlups"], "ms_per_coarse_step": t32["ms_per_coarse_step"],
+                                            "roofline": t32["roofline"],
                                             "speedup_vs_fp64": round(t32["mlups"] / threelev["mlups"], 3),
                                             "max_rel_rho_err_vs_fp64": e_rho}
                 log(f"[bench] three-level FP32-shifted: {t32['mlups']:.0f} MLUPS, max rel rho err {e_rho:.2e}")

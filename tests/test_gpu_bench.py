@@ -36,6 +36,9 @@ def test_bench_prints_one_strict_json_line(steps):
     assert rf["achieved"] > 0 and 0 < rf["frac"] < 1
     assert out["clock_warmup_ms"] > 0
     assert out["two_level"]["mlups"] > 0
+    tr = out["two_level"]["roofline"]
+    assert tr["bound"] == "hbm" and 0 < tr["algorithmic_frac"] < 1
+    assert tr["measured_gbs"] > tr["algorithmic_gbs"]  # the step moves more than 240 B per update
     w = out["walls"]
     assert w["periodic"] == [1, 1, 0] and w["mlups"] > 0 and 0 < w["frac_of_hbm_peak"] < 1
     assert w["mass_drift_rel"] < 1e-12
