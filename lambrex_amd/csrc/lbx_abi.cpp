@@ -101,15 +101,16 @@ struct Level {
 
 // One captured level-0 step of a hierarchy per (level-0 parity, finest-level
 // parity).
+// Captured level-0 steps, one per combination of the levels' buffer
+// parities at the start of the step (bit l = level l's `cur`).
 struct StepGraphs {
-  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t exec[1 << LBX_MAX_LEVELS] = {};
   void clear() {
-    for (auto& row : exec)
-      for (auto& e : row)
-        if (e) {
-          (void)hipGraphExecDestroy(e);
-          e = nullptr;
-        }
+    for (auto& e : exec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
   }
 };
 
@@ -478,16 +479,17 @@ void free_levels_from(lbx_sim* sim, int level) {
 constexpr long long kFineGroupBytes = 48ll << 20;
 
 // The finest level's substeps (two at ratio 2, r at ratio r), from its
-// parent's current state
-// (level 0's `now`, or a middle level's state, which stays put until its own
-// stream).  Its block averages land in F.d_avg; fpar is the buffer holding
-// its state and flips.
-hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
+// parent's current state (level 0's `now`, or a middle level's state, which
+// its own step rewrites only after this).  Its block averages land in
+// F.d_avg.  par[l] is the buffer holding level l's state while the step is
+// enqueued; the finest level's flips.
+hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int* par, hipStream_t st) {
   Level& F = sim->lev[f];
   const Level& C = sim->lev[f - 1];
   const int prec = sim->precision;
   const Relax rf{F.omega_s, F.omega_b, sim->rho0};
-  const void* cnow = C.buf[C.cur];
+  int& fpar = par[f];
+  const void* cnow = C.buf[par[f - 1]];
   void* fnow = F.buf[fpar];
   void* fnext = F.buf[1 - fpar];
   void* ftmp = F.buf[2];
@@ -558,53 +560,85 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int& fpar, hipStream_t st) {
   return e;
 }
 
-hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStream_t st);
+hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int* par, hipStream_t st);
 
-// One step of a middle level m (0 < m < finest), RohdeCycle(m): the
+// One step of a middle level m (0 < m < finest), RohdeCycle(m): the finer
+// levels' steps, which read m's state; then, per group of m's patches, the
 // two-level fill from the parent with comp 0 zeroed on the outer layer and
-// the masked collide into the other buffer; the finer levels' steps, which
-// read m's state; then Stream + ZeroInvalidComponents back into the state
-// buffer and SumFromFine into its valid cells (one fused kernel).  The state
-// stays in buf[cur].  In the last of its parent's r steps (`last`) the
-// block averages of the result for the parent are formed too: in the same
-// pass at ratio 2, where they include 2 fine ghost layers, or by a pass over
-// the valid cells above it (nGrow = 2 / r = 0).
-hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int& fpar, hipStream_t st, bool last) {
+// the masked collide, and Stream + ZeroInvalidComponents with SumFromFine
+// added into the valid cells (one fused kernel) into m's other buffer,
+// which becomes its state.  The reference collides before the finer levels'
+// steps; the collide reads only m's and its parent's current states, which
+// those steps do not change, so it can follow them, and m's passes then run
+// in the same MALL-sized groups as the finest level's (the array between
+// the two in a group-sized region of buf[2] per stream, consecutive groups
+// on two streams): the fill of a group reads the neighbours' current state,
+// which no group overwrites.  In the last of its parent's r steps (`last`)
+// the block averages of the result for the parent are formed too: in the
+// same pass at ratio 2, where they include 2 fine ghost layers, or by a pass
+// over the valid cells above it (nGrow = 2 / r = 0).
+hipError_t enqueue_middle_step(lbx_sim* sim, int m, int finest, int* par, hipStream_t st, bool last) {
   Level& L = sim->lev[m];
   const Level& P = sim->lev[m - 1];
   const Level& F = sim->lev[m + 1];
   const int prec = sim->precision;
   const Relax r{L.omega_s, L.omega_b, sim->rho0};
-  const int np = (int)L.boxes.size();
-  hipError_t e = lbx::launch_fine_fill_collide(prec, L.buf[L.cur], L.buf[1 - L.cur], P.buf[P.cur], P.lat,
-                                               P.patched ? P.d_desc : nullptr, L.d_desc, L.d_desc, 0, np,
-                                               L.max_cells, L.d_fill, L.d_mask, r, st);
-  if (e == hipSuccess) e = enqueue_children(sim, m, finest, fpar, st);
-  if (e == hipSuccess && last && L.ratio != 2) {
-    e = lbx::launch_fine_stream_zero(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_desc, 0, np, L.max_cells, sim->rho0,
-                                     F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start, F.d_sum_contrib, st);
-    if (e == hipSuccess)
-      e = lbx::launch_fine_avg_valid(prec, L.buf[L.cur], L.d_avg, L.avg_stride, L.d_desc, 0, np, L.max_blocks,
-                                     L.ratio, sim->rho0, st);
-  } else if (e == hipSuccess && last)  // the state the parent averages down: its block sums in the same pass
-    e = lbx::launch_middle_stream_avg(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_avg, L.avg_stride, L.d_desc, np,
-                                      L.max_blocks, sim->rho0, F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start,
-                                      F.d_sum_contrib, st);
-  else if (e == hipSuccess)
-    e = lbx::launch_fine_stream_zero(prec, L.buf[1 - L.cur], L.buf[L.cur], L.d_desc, 0, np, L.max_cells, sim->rho0,
-                                     F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start, F.d_sum_contrib, st);
+  hipError_t e = enqueue_children(sim, m, finest, par, st);
+  if (e != hipSuccess) return e;
+  void* now = L.buf[par[m]];
+  void* next = L.buf[1 - par[m]];
+  const void* pnow = P.buf[par[m - 1]];
+  lbx_ctx* ctx = sim->ctx;
+  const auto& groups = L.groups;
+  const bool fork = groups.size() > 1;
+  if (fork) {
+    e = hipEventRecord(ctx->fork_ev, st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0);
+  }
+  for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
+    const int first = groups[k].first, n = groups[k].second;
+    hipStream_t gs = (k & 1) ? ctx->side : st;
+    void* mid = L.buf[2];
+    long long shift = 0;
+    if (L.regions) {
+      mid = static_cast<char*>(L.buf[2]) + (k & 1) * L.region_span * elem_size(prec);
+      shift = L.gdesc[first].base;
+    }
+    e = lbx::launch_fine_fill_collide(prec, now, mid, pnow, P.lat, P.patched ? P.d_desc : nullptr, L.d_desc,
+                                      L.d_gdesc, first, n, L.max_cells, L.d_fill, L.d_mask, r, gs, shift);
+    if (e == hipSuccess && last && L.ratio != 2) {
+      e = lbx::launch_fine_stream_zero(prec, mid, next, L.d_gdesc, first, n, L.max_cells, sim->rho0, F.d_sum_slot,
+                                       F.d_avg, F.avg_stride, F.d_sum_start, F.d_sum_contrib, gs, shift);
+      if (e == hipSuccess)
+        e = lbx::launch_fine_avg_valid(prec, next, L.d_avg, L.avg_stride, L.d_gdesc, first, n, L.max_blocks,
+                                       L.ratio, sim->rho0, gs);
+    } else if (e == hipSuccess && last) {  // the state the parent averages down: its block sums in the same pass
+      e = lbx::launch_middle_stream_avg(prec, mid, shift, next, L.d_avg, L.avg_stride, L.d_gdesc, first, n,
+                                        L.max_blocks, sim->rho0, F.d_sum_slot, F.d_avg, F.avg_stride, F.d_sum_start,
+                                        F.d_sum_contrib, gs);
+    } else if (e == hipSuccess) {
+      e = lbx::launch_fine_stream_zero(prec, mid, next, L.d_gdesc, first, n, L.max_cells, sim->rho0, F.d_sum_slot,
+                                       F.d_avg, F.avg_stride, F.d_sum_start, F.d_sum_contrib, gs, shift);
+    }
+  }
+  if (fork) {  // join even after a failed launch, so a capture can end
+    const hipError_t ej = hipEventRecord(ctx->join_ev, ctx->side);
+    const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(st, ctx->join_ev, 0) : ej;
+    if (e == hipSuccess) e = ew;
+  }
+  par[m] = 1 - par[m];
   return e;
 }
 
 // What level l's step needs of level l + 1: the finest level's substeps,
 // or refRatio middle steps, the last followed by the block averages of the
 // result (src/AmrSim.cpp:445-455).
-hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int& fpar, hipStream_t st) {
+hipError_t enqueue_children(lbx_sim* sim, int l, int finest, int* par, hipStream_t st) {
   const int c = l + 1;
-  if (c == finest) return enqueue_finest_pair(sim, c, fpar, st);
+  if (c == finest) return enqueue_finest_pair(sim, c, par, st);
   const int r = sim->lev[c].ratio;
   hipError_t e = hipSuccess;
-  for (int k = 0; k < r && e == hipSuccess; ++k) e = enqueue_middle_step(sim, c, finest, fpar, st, k == r - 1);
+  for (int k = 0; k < r && e == hipSuccess; ++k) e = enqueue_middle_step(sim, c, finest, par, st, k == r - 1);
   return e;
 }
 
@@ -615,7 +649,8 @@ hipError_t enqueue_hierarchy_step(lbx_sim* sim, hipStream_t st) {
   const Relax rc{C.omega_s, C.omega_b, sim->rho0};
   const void* cnow = C.buf[C.cur];
   void* cnext = C.buf[1 - C.cur];
-  int fpar = sim->lev[finest].cur;
+  int par[LBX_MAX_LEVELS];
+  for (int l = 0; l < LBX_MAX_LEVELS; ++l) par[l] = sim->lev[l].cur;
   // CoarseCollide + Stream(0) read level 0's `now` and write its `next`;
   // the finer levels only read `now` (their parent fills), so the push runs
   // on its own stream beside them, and SumFromFine(0) into `next` waits for
@@ -626,7 +661,7 @@ hipError_t enqueue_hierarchy_step(lbx_sim* sim, hipStream_t st) {
   if (e != hipSuccess) return e;
   const hipError_t ep = lbx::launch_coarse_push(sim->precision, cnow, cnext, C.lat, C.d_mask, rc, C.push_shape,
                                                 ctx->push);
-  e = enqueue_children(sim, 0, finest, fpar, st);
+  e = enqueue_children(sim, 0, finest, par, st);
   // join even after a failed launch, so a capture can end
   const hipError_t ej = hipEventRecord(ctx->push_join_ev, ctx->push);
   const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(st, ctx->push_join_ev, 0) : ej;
@@ -646,7 +681,9 @@ int launch_hierarchy_step(lbx_sim* sim) {
   const int finest = finest_level(sim);
   Level& F = sim->lev[finest];
   if (sim->use_graphs && !ctx->prof) {
-    hipGraphExec_t& ex = sim->graphs.exec[C.cur][F.cur];
+    unsigned key = 0;
+    for (int l = 0; l <= finest; ++l) key |= (unsigned)sim->lev[l].cur << l;
+    hipGraphExec_t& ex = sim->graphs.exec[key];
     if (!ex) {
       hipGraph_t g = nullptr;
       LBX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -671,12 +708,15 @@ int launch_hierarchy_step(lbx_sim* sim) {
     LBX_HIP(ctx, enqueue_hierarchy_step(sim, ctx->stream));
     if (int rc = ps.close()) return rc;
   }
-  // the finest level's substeps run once per step of its parent, which takes
-  // r^(finest-1) steps per level-0 step (the ratios of the levels between);
-  // each run flips its buffer, so its parity changes when that count is odd
-  long long runs = 1;
-  for (int l = 1; l < finest; ++l) runs *= sim->lev[l].ratio;
-  if (runs % 2) F.cur = 1 - F.cur;
+  // every step of a middle level flips its buffer, and level l takes the
+  // product of the ratios of levels 1..l steps per level-0 step; the finest
+  // level's substeps run (and flip its buffer) once per step of its parent
+  long long steps = 1;
+  for (int l = 1; l < finest; ++l) {
+    steps *= sim->lev[l].ratio;
+    if (steps % 2) sim->lev[l].cur = 1 - sim->lev[l].cur;
+  }
+  if (steps % 2) F.cur = 1 - F.cur;
   C.cur = 1 - C.cur;
   return LBX_OK;
 }

@@ -155,9 +155,10 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
                                     const Lattice& clat, const PatchDesc* cpatches, const PatchDesc* patches,
                                     const PatchDesc* own, int first, int npatch, int max_cells,
                                     const long long* fill, const unsigned char* mask, const Relax& r,
-                                    hipStream_t st);
+                                    hipStream_t st, long long oshift = 0);
 // out holds each patch at element P.base - oshift (0: a level buffer; the
-// group's first base: a group-sized scratch array).
+// group's first base: a group-sized scratch array); an input the same way
+// at P.base - ishift.
 // inner: `in` is itself such an output (outer layer not stored, read as
 // zero), for the later substeps at a refinement ratio above 2.
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, long long oshift,
@@ -178,7 +179,8 @@ hipError_t launch_fine_avg_valid(int precision, const void* f, double* avg, long
 // launch_sum_apply.
 hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
                                    int npatch, int max_cells, double rho0, const int* slot, const double* avg,
-                                   long long avg_stride, const int* start, const int* contrib, hipStream_t st);
+                                   long long avg_stride, const int* start, const int* contrib, hipStream_t st,
+                                   long long ishift = 0);
 // avg holds 15 FP64 planes of avg_stride block averages; max_blocks = max (g/2)^3;
 // in holds each patch at element P.base - ishift.
 hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
@@ -188,10 +190,10 @@ hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishif
 // launch_fine_stream_zero (the child's block averages cavg added through
 // slot / start / contrib) fused with the level's own block averages of the
 // result into avg.
-hipError_t launch_middle_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
-                                   const PatchDesc* patches, int npatch, int max_blocks, double rho0, const int* slot,
-                                   const double* cavg, long long cavg_stride, const int* start, const int* contrib,
-                                   hipStream_t st);
+hipError_t launch_middle_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
+                                   long long avg_stride, const PatchDesc* patches, int first, int npatch,
+                                   int max_blocks, double rho0, const int* slot, const double* cavg,
+                                   long long cavg_stride, const int* start, const int* contrib, hipStream_t st);
 // flags: per coarse cell (x fastest), bits 0-5 = x+1, x-1, y+1, y-1, z+1,
 // z-1 leave the cell's box; bit 7 = masked by the fine level.
 hipError_t launch_coarse_push(int precision, const void* cnow, void* cnext, const Lattice& L,

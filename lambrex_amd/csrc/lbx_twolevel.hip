@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
     const S* __restrict__ fnow, S* __restrict__ fnext, const S* __restrict__ cnow, Lattice clat,
     const PatchDesc* __restrict__ cpatches, const PatchDesc* __restrict__ patches,
     const PatchDesc* __restrict__ own, int first, const long long* __restrict__ fill,
-    const unsigned char* __restrict__ mask, Relax r) {
+    const unsigned char* __restrict__ mask, Relax r, long long oshift) {
   // P is read in place: with a by-value copy the PC variants merged the
   // loads of base / qs from P, patches[s] and cpatches[] into one load through
   // a selected pointer, which kept the copy in 80 B of scratch per lane.
@@ -161,8 +161,9 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
   } else if (patch_outer(P, gx, gy, gz)) {
     f[0] = zero_pop<S>(0, r.rho0);
   }
+  const long long ob = P.base - oshift;
 #pragma unroll
-  for (int q = 0; q < NV; ++q) fnext[P.base + q * P.qs + i] = f[q];
+  for (int q = 0; q < NV; ++q) fnext[ob + q * P.qs + i] = f[q];
 }
 
 // Pull of grown cell i over valid + 1 ghost layer, from the patch stored at
@@ -278,14 +279,14 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
                                                           double rho0, const int* __restrict__ slot,
                                                           const double* __restrict__ avg, long long avg_stride,
                                                           const int* __restrict__ start,
-                                                          const int* __restrict__ contrib) {
+                                                          const int* __restrict__ contrib, long long ishift) {
   const PatchDesc P = patches[first + blockIdx.y];
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
   S f[NV];
-  if (fine_pull(P, in, P.base, i, gx, gy, gz, rho0, f)) {
+  if (fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f)) {
     const int u = slot[P.fill_base + i];
     if (u >= 0) add_block_sums(f, u, avg, avg_stride, start, contrib, rho0);
   } else if (!patch_outer(P, gx, gy, gz)) {
@@ -589,19 +590,19 @@ hipError_t patch_io(int op, S* f, const PatchDesc* patches, int b, int valid_cel
 template <typename S>
 hipError_t fill_collide(const S* fnow, S* fnext, const S* cnow, const Lattice& clat, const PatchDesc* cpatches,
                         const PatchDesc* patches, const PatchDesc* own, int first, const long long* fill,
-                        const unsigned char* mask, const Relax& r, dim3 grid, hipStream_t st) {
+                        const unsigned char* mask, const Relax& r, long long oshift, dim3 grid, hipStream_t st) {
   if (cpatches && mask)
     k_fine_fill_collide<S, true, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own, first,
-                                                             fill, mask, r);
+                                                             fill, mask, r, oshift);
   else if (cpatches)
     k_fine_fill_collide<S, true, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own,
-                                                              first, fill, mask, r);
+                                                              first, fill, mask, r, oshift);
   else if (mask)
     k_fine_fill_collide<S, false, true><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own,
-                                                              first, fill, mask, r);
+                                                              first, fill, mask, r, oshift);
   else
     k_fine_fill_collide<S, false, false><<<grid, 256, 0, st>>>(fnow, fnext, cnow, clat, cpatches, patches, own,
-                                                               first, fill, mask, r);
+                                                               first, fill, mask, r, oshift);
   return hipGetLastError();
 }
 
@@ -609,41 +610,42 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
                                     const Lattice& clat, const PatchDesc* cpatches, const PatchDesc* patches,
                                     const PatchDesc* own, int first, int npatch, int max_cells,
                                     const long long* fill, const unsigned char* mask, const Relax& r,
-                                    hipStream_t st) {
+                                    hipStream_t st, long long oshift) {
   const dim3 grid((max_cells + 255) / 256, npatch);
   if (precision == 0)
     return fill_collide((const double*)fnow, (double*)fnext, (const double*)cnow, clat, cpatches, patches, own,
-                        first, fill, mask, r, grid, st);
+                        first, fill, mask, r, oshift, grid, st);
   return fill_collide((const float*)fnow, (float*)fnext, (const float*)cnow, clat, cpatches, patches, own, first,
-                      fill, mask, r, grid, st);
+                      fill, mask, r, oshift, grid, st);
 }
 
 hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, const PatchDesc* patches, int first,
                                    int npatch, int max_cells, double rho0, const int* slot, const double* avg,
-                                   long long avg_stride, const int* start, const int* contrib, hipStream_t st) {
+                                   long long avg_stride, const int* start, const int* contrib, hipStream_t st,
+                                   long long ishift) {
   const dim3 grid((max_cells + 255) / 256, npatch);
   if (precision == 0)
     k_fine_stream_zero<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, rho0, slot,
-                                                     avg, avg_stride, start, contrib);
+                                                     avg, avg_stride, start, contrib, ishift);
   else
     k_fine_stream_zero<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, patches, first, rho0, slot, avg,
-                                                    avg_stride, start, contrib);
+                                                    avg_stride, start, contrib, ishift);
   return hipGetLastError();
 }
 
-hipError_t launch_middle_stream_avg(int precision, const void* in, void* out, double* avg, long long avg_stride,
-                                   const PatchDesc* patches, int npatch, int max_blocks, double rho0, const int* slot,
-                                   const double* cavg, long long cavg_stride, const int* start, const int* contrib,
-                                   hipStream_t st) {
+hipError_t launch_middle_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
+                                   long long avg_stride, const PatchDesc* patches, int first, int npatch,
+                                   int max_blocks, double rho0, const int* slot, const double* cavg,
+                                   long long cavg_stride, const int* start, const int* contrib, hipStream_t st) {
   const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
   if (precision == 0)
     k_fine_stream_avg<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride,
-                                                                patches, 0, rho0, 0, slot, cavg, cavg_stride, start,
-                                                                contrib);
+                                                                patches, first, rho0, ishift, slot, cavg, cavg_stride,
+                                                                start, contrib);
   else
     k_fine_stream_avg<float, true><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride,
-                                                               patches, 0, rho0, 0, slot, cavg, cavg_stride, start,
-                                                               contrib);
+                                                               patches, first, rho0, ishift, slot, cavg, cavg_stride,
+                                                               start, contrib);
   return hipGetLastError();
 }
 
