@@ -281,8 +281,9 @@ typedef struct {
   double* out_rho;         /* optional, filled at completion: [nz][ny][nx] */
   double* out_u;           /* optional: [3][nz][ny][nx] */
   double* out_f;           /* optional: FP64 populations [15][nz][ny][nx] */
-  /* Refined runs (n_fine_boxes > 0): level 1 (ratio 2) over fine_lohi, 6
-   * int32 per box (inclusive lo, hi in the fine index space 2*dims), aligned
+  /* Refined runs (n_fine_boxes > 0): level 1 at refinement ratio r
+   * (ref_ratio: 0 or 2 for the reference's 2, or 3 or 4) over fine_lohi, 6
+   * int32 per box (inclusive lo, hi in the fine index space r*dims), aligned
    * to the ratio; tau of level 1 rescaled as ComputeDt does
    * (src/AmrSim.cpp:313-314); level 0 masked under level 1 (the fine-grid
    * mask).  Level 0 is box_lohi's n_boxes boxes (one box over the domain if
@@ -292,6 +293,7 @@ typedef struct {
   const int32_t* box_lohi;
   int32_t n_fine_boxes;
   const int32_t* fine_lohi;
+  int32_t ref_ratio;
 } lbx_request;
 
 typedef struct {

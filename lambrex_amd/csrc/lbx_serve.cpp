@@ -234,8 +234,9 @@ void box_fields(const Request& q, const int32_t* lohi, std::vector<double>& rho,
 
 // The refined run's simulation on the server's context: level 0 from its
 // boxes (one box over the domain if none were given) at equilibrium, level 1
-// filled from it by piecewise-constant interpolation with tau rescaled as
-// ComputeDt does (src/AmrSim.cpp:297-322), and level 0 masked under level 1.
+// at the request's ratio filled from it by piecewise-constant interpolation
+// with tau rescaled as ComputeDt does (src/AmrSim.cpp:297-322), and level 0
+// masked under level 1.
 int start_refined(lbx_server* s, Run& r) {
   const Request& q = r.r;
   const lbx_request& rq = q.req;
@@ -255,7 +256,10 @@ int start_refined(lbx_server* s, Run& r) {
     box_fields(q, &boxes[6 * b], rho, u);
     if (int rc = abi(lbx_init_equilibrium(r.sim, 0, b, rho.data(), u.data()))) return rc;
   }
-  const double tf_s = 2 * (rq.tau_s - 0.5) + 0.5, tf_b = 2 * (rq.tau_b - 0.5) + 0.5;
+  const int ratio = rq.ref_ratio ? rq.ref_ratio : 2;
+  if (ratio != 2)
+    if (int rc = abi(lbx_set_ref_ratio(r.sim, ratio))) return rc;
+  const double tf_s = ratio * (rq.tau_s - 0.5) + 0.5, tf_b = ratio * (rq.tau_b - 0.5) + 0.5;
   if (int rc = abi(lbx_level_define(r.sim, 1, nf, q.fine.data()))) return rc;
   if (int rc = abi(lbx_set_relax(r.sim, 1, 1.0 / (tf_s + 0.5), 1.0 / (tf_b + 0.5)))) return rc;
   if (int rc = abi(lbx_fill_from_coarse(r.sim, 1))) return rc;
@@ -450,7 +454,8 @@ int deliver_refined(lbx_server* s, Run& r) {
   for (size_t b = 0; b + 5 < r.r.fine.size(); b += 6)
     fine_cells += (long long)(r.r.fine[b + 3] - r.r.fine[b] + 1) * (r.r.fine[b + 4] - r.r.fine[b + 1] + 1) *
                   (r.r.fine[b + 5] - r.r.fine[b + 2] + 1);
-  s->lattice_updates += (n + 2 * fine_cells) * (int64_t)rq.steps;
+  const int ratio = rq.ref_ratio ? rq.ref_ratio : 2;  // fine substeps per coarse step
+  s->lattice_updates += (n + ratio * fine_cells) * (int64_t)rq.steps;
   lbx_sim_destroy(r.sim);
   r.sim = nullptr;
   s->refined_bytes -= r.device_bytes;
@@ -687,6 +692,8 @@ int lbx_serve_submit(lbx_server* s, const lbx_request* req, uint64_t* id) {
   if (req->n_boxes > 0 && !ref)
     return sfail(s, LBX_E_ARG, "level-0 boxes are taken for refined runs only (single-level runs are one lattice)");
   if (req->steps < 1) return sfail(s, LBX_E_ARG, "steps must be >= 1");
+  if (req->ref_ratio != 0 && (req->ref_ratio < 2 || req->ref_ratio > 4))
+    return sfail(s, LBX_E_ARG, "refinement ratio %d is not 2, 3 or 4", req->ref_ratio);
   if (!(req->tau_s > -0.5) || !(req->tau_b > -0.5) || !std::isfinite(req->tau_s) || !std::isfinite(req->tau_b))
     return sfail(s, LBX_E_ARG, "tau must be finite and > -1/2");
   const lbx::Lattice L = lbx::make_lattice(req->dims[0], req->dims[1], req->dims[2], (int)esize(s->precision));

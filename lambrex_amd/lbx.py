@@ -73,7 +73,7 @@ class Request(ctypes.Structure):
                 ("amplitude", ctypes.c_double),
                 ("rho", _dp), ("u", _dp), ("out_rho", _dp), ("out_u", _dp), ("out_f", _dp),
                 ("n_boxes", ctypes.c_int32), ("box_lohi", _i32p),
-                ("n_fine_boxes", ctypes.c_int32), ("fine_lohi", _i32p)]
+                ("n_fine_boxes", ctypes.c_int32), ("fine_lohi", _i32p), ("ref_ratio", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -452,11 +452,12 @@ class Server:
             raise LbxError(rc, self._lib.lbx_serve_last_error(self.h).decode())
 
     def submit(self, dims, steps, tau_s, tau_b=None, amplitude=0.0, axis=0, plane=None,
-               rho=None, u=None, want_rho=False, want_f=False, boxes=None, fine_boxes=None):
+               rho=None, u=None, want_rho=False, want_f=False, boxes=None, fine_boxes=None, ref_ratio=0):
         """Queue one run; returns (id, outputs) where outputs holds the numpy
-        arrays the server fills at completion.  fine_boxes (fine index space)
-        makes it a two-level run over level-0 `boxes` (default: one box);
-        outputs are then level 0's."""
+        arrays the server fills at completion.  fine_boxes (fine index space,
+        ref_ratio * dims; ratio 2 unless ref_ratio is 3 or 4) makes it a
+        two-level run over level-0 `boxes` (default: one box); outputs are
+        then level 0's."""
         nx, ny, nz = (int(d) for d in dims)
         r = Request()
         r.dims[:] = (nx, ny, nz)
@@ -492,6 +493,7 @@ class Server:
             keep["fine"] = _lohi(fine_boxes)
             r.n_fine_boxes = len(fine_boxes)
             r.fine_lohi = _ptr(keep["fine"], ctypes.c_int32)
+        r.ref_ratio = int(ref_ratio)
         rid = ctypes.c_uint64()
         self._check(self._lib.lbx_serve_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
         keep.update(outs)

@@ -130,6 +130,45 @@ def test_refined_runs_in_a_mix_equal_each_run_alone(ctx, precision):
     assert stats["lattice_updates"] == lu
 
 
+@pytest.mark.parametrize("precision", [lbx.LBX_FP32_SHIFTED, lbx.LBX_FP64])
+def test_refined_run_at_ratio_3_equals_it_alone(ctx, precision):
+    """A two-level run at refinement ratio 3 (ref_ratio = 3: fine boxes in
+    the index space 3 * dims, 3 fine substeps per coarse step) served beside
+    a single-level run equals the same hierarchy through lbx_iterate alone,
+    bit for bit, and the lattice updates count 3 fine substeps."""
+    dims, steps, tau, amp = (18, 12, 12), 5, 0.8, 1e-2
+    fine = fine_boxes_in((18, 9, 0), (35, 26, 35), 9)  # coarse x in [6, 12), y in [3, 9), all z
+    srv = lbx.Server(ctx, precision=precision, arena_bytes=256 << 20)
+    single, os_ = srv.submit((16, 12, 10), 7, 0.7, amplitude=1e-3, want_f=True)
+    rid, o = srv.submit(dims, steps, tau, amplitude=amp, axis=0, want_f=True, want_rho=True, fine_boxes=fine,
+                        ref_ratio=3)
+    srv.drain()
+    res = {r["id"]: r for r in srv.poll()}
+    stats = srv.stats()
+    srv.close()
+    assert res[rid]["status"] == lbx.LBX_OK and res[single]["status"] == lbx.LBX_OK
+    sim = lbx.Sim(ctx, dims, precision=precision)
+    try:
+        sim.define_level(0, [((0, 0, 0), tuple(d - 1 for d in dims))])
+        sim.set_tau(0, tau)
+        rho, u = workloads.plane_pulse(dims, amp, axis=0, plane=dims[0] // 2)
+        sim.init_from_fields(rho, u)
+        sim.set_ref_ratio(3)
+        sim.define_level(1, fine)
+        sim.set_tau(1, 3 * (tau - 0.5) + 0.5)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(fine)
+        sim.iterate(steps)
+        f0, rho0 = sim.get_f(0), sim.get_hydro(0)[0]
+    finally:
+        sim.close()
+    assert bitwise_equal(o["f"], f0)
+    assert bitwise_equal(o["rho"], rho0)
+    assert bitwise_equal(os_["f"], alone_plain(ctx, (16, 12, 10), 7, 0.7, 1e-3, precision))
+    fine_cells = sum(np.prod([h - l + 1 for l, h in zip(a, b)]) for a, b in fine)
+    assert stats["lattice_updates"] == 16 * 12 * 10 * 7 + (np.prod(dims) + 3 * fine_cells) * steps
+
+
 def test_refined_request_with_bad_boxes_is_delivered_as_an_error(ctx):
     """A fine box that is not aligned to ratio 2 fails when the run is set
     up: the request comes back with an error status and the server goes on
