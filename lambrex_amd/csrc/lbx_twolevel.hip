@@ -272,7 +272,9 @@ __device__ __forceinline__ void add_block_sums(S* f, int u, const double* __rest
 // half of SumFromFine into this level (src/AmrSim.cpp:464-466, 592-602): a
 // valid cell whose slot (per grown cell, -1 if none) names a touched cell u
 // adds the child's block averages [start[u], start[u+1]) to its streamed
-// value in FP64, in the same order as k_sum_apply.
+// value in FP64, in the same order as k_sum_apply.  `in` holds each patch at
+// element P.base - ishift (a group-sized region when ishift is the group's
+// first base); `out` at P.base.
 template <typename S>
 __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ in, S* __restrict__ out,
                                                           const PatchDesc* __restrict__ patches, int first,
