@@ -242,6 +242,10 @@ def test_bad_requests_are_refused(ctx):
         with pytest.raises(lbx.LbxError) as e:
             srv.submit(dims, steps, tau, plane=plane)
         assert e.value.code == lbx.LBX_E_ARG, (dims, tau, plane)
+    for ratio in (1, 5, -2):  # refinement ratio outside 2..4
+        with pytest.raises(lbx.LbxError) as e:
+            srv.submit((8, 8, 8), 5, 0.8, fine_boxes=[((0, 0, 0), (7, 7, 7))], ref_ratio=ratio)
+        assert e.value.code == lbx.LBX_E_ARG, ratio
     srv.close()
 
 
