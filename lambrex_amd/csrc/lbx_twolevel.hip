@@ -82,11 +82,34 @@ __device__ __forceinline__ void collide_s(S* f, const Relax& r) {
     collide_f32_shifted(f, float(r.rho0), float(r.omega_s), float(r.omega_b));
 }
 
+// n / d for 0 <= n < 2^22 and d >= 1: the float estimate from the hardware
+// reciprocal (relative error below 2^-22 with the product's rounding) is
+// off by less than one, and one correction makes it exact.
+__device__ __forceinline__ int div_small(int n, int d) {
+  int q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+  const int r = n - q * d;
+  q += (r >= d) - (r < 0);
+  return q;
+}
+
+// (x, y, z) of index i < n of a box with rows of n0 and planes of n1 rows;
+// n is uniform over the workgroup, so the branch is too.
+__device__ __forceinline__ void split3(int i, int n, int n0, int n1, int& x, int& y, int& z) {
+  if (n <= (1 << 22)) {
+    const int t = div_small(i, n0);
+    x = i - t * n0;
+    z = div_small(t, n1);
+    y = t - z * n1;
+  } else {
+    x = i % n0;
+    const int t = i / n0;
+    y = t % n1;
+    z = t / n1;
+  }
+}
+
 __device__ __forceinline__ void patch_coords(const PatchDesc& P, int i, int& gx, int& gy, int& gz) {
-  gx = i % P.g[0];
-  const int t = i / P.g[0];
-  gy = t % P.g[1];
-  gz = t / P.g[1];
+  split3(i, P.cells, P.g[0], P.g[1], gx, gy, gz);
 }
 
 __device__ __forceinline__ bool patch_valid(const PatchDesc& P, int gx, int gy, int gz) {
@@ -337,7 +360,8 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
   const int u = b0 + lb;
   if (u < nblk) {
     const int di = child & 1, dj = (child >> 1) & 1, dk = child >> 2;
-    const int cx = u % c0, t = u / c0, cy = t % c1, cz = t / c1;
+    int cx, cy, cz;
+    split3(u, nblk, c0, c1, cx, cy, cz);
     const int gx = 2 * cx + di, gy = 2 * cy + dj, gz = 2 * cz + dk;
     const int i = (gz * P.g[1] + gy) * P.g[0] + gx;
     S f[NV];
@@ -466,8 +490,10 @@ __global__ __launch_bounds__(256) void k_fine_avg_valid(const S* __restrict__ f,
   const PatchDesc& P = patches[first + blockIdx.y];
   const int c0 = (P.hi[0] - P.lo[0] + 1) / r, c1 = (P.hi[1] - P.lo[1] + 1) / r, c2 = (P.hi[2] - P.lo[2] + 1) / r;
   const int u = blockIdx.x * 256 + threadIdx.x;
-  if (u >= c0 * c1 * c2) return;
-  const int cx = u % c0, t = u / c0, cy = t % c1, cz = t / c1;
+  const int nblk = c0 * c1 * c2;
+  if (u >= nblk) return;
+  int cx, cy, cz;
+  split3(u, nblk, c0, c1, cx, cy, cz);
   const double volfrac = 1.0 / (double)(r * r * r);
   const int sy = P.g[0], sz = P.g[0] * P.g[1];
   const long long corner = (long long)(2 + r * cz) * sz + (long long)(2 + r * cy) * sy + (2 + r * cx);
