@@ -504,17 +504,22 @@ hipError_t enqueue_finest_pair(lbx_sim* sim, int f, int* par, hipStream_t st) {
   // substeps is dead once its group ends: with F.regions each stream
   // rewrites one region of buf[2] for it (the group's patches are
   // contiguous, patch p at p.base - base(first)) while its lines are still
-  // in the MALL, instead of writing each group's back to HBM.
+  // in the MALL, instead of writing each group's back to HBM.  The groups
+  // run in storage order when fpar is 0 and in reverse order when it is 1,
+  // so each pair starts with the groups whose state the previous pair wrote
+  // last, while it may still be in the MALL.
   lbx_ctx* ctx = sim->ctx;
   const auto& groups = F.groups;
-  const bool fork = groups.size() > 1;
+  const size_t ng = groups.size();
+  const bool fork = ng > 1, rev = fpar == 1;
   hipError_t e = hipSuccess;
   if (fork) {
     e = hipEventRecord(ctx->fork_ev, st);
     if (e == hipSuccess) e = hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0);
   }
-  for (size_t k = 0; k < groups.size() && e == hipSuccess; ++k) {
-    const int first = groups[k].first, n = groups[k].second;
+  for (size_t k = 0; k < ng && e == hipSuccess; ++k) {
+    const auto& grp = groups[rev ? ng - 1 - k : k];
+    const int first = grp.first, n = grp.second;
     hipStream_t gs = (k & 1) ? ctx->side : st;
     void* mid = ftmp;
     long long shift = 0;

@@ -134,13 +134,13 @@ def three_levels_ratio(r):
     return dims, [l0, l1, l2]
 
 
+@pytest.mark.parametrize("r", [3, 4])
 @pytest.mark.parametrize("mask", ["reference", "fine"])
-def test_three_levels_ratio_3_bitwise(ctx, mask):
-    """Ratio 3 on every level (3 middle steps per level-0 step, 3 finest
-    substeps per middle step, 27-cell block averages into a patched parent
-    and into level 0): all three levels after 2 level-0 steps equal the
-    oracle's at the same ratio, bit for bit."""
-    r = 3
+def test_three_levels_ratio_bitwise(ctx, mask, r):
+    """Ratio r = 3 or 4 on every level (r middle steps per level-0 step, r
+    finest substeps per middle step, r^3-cell block averages into a patched
+    parent and into level 0): all three levels after 2 level-0 steps equal
+    the oracle's at the same ratio, bit for bit."""
     dims, levels = three_levels_ratio(r)
     masks = masks_for(mask, levels)
     rho, u = workloads.random_state(dims, seed=79, rho_sigma=1e-2, u_amp=0.02)
