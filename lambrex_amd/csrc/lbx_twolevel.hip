@@ -208,8 +208,8 @@ __device__ __forceinline__ bool fine_pull(const PatchDesc& P, const S* __restric
 
 // fine_pull for an input whose outer ghost layer is not stored (the first
 // substep's output, k_fine_stream_collide): a population whose upstream cell
-// lies on the outer layer takes that layer's defined value, zero, without a
-// load.
+// lies on the outer layer takes that layer's defined value, zero, without
+// reading that layer.
 template <typename S>
 __device__ __forceinline__ bool fine_pull_inner(const PatchDesc& P, const S* __restrict__ in, long long base,
                                                 int i, int gx, int gy, int gz, double rho0, S* f) {
@@ -219,19 +219,17 @@ __device__ __forceinline__ bool fine_pull_inner(const PatchDesc& P, const S* __r
     return false;
   }
   const int sx = 1, sy = P.g[0], sz = P.g[0] * P.g[1];
-  const bool valid = patch_valid(P, gx, gy, gz);
-  if (valid) {  // every upstream cell is valid or on the first ghost layer
+  // One load per population for every lane (a wave holds valid and ghost
+  // cells alike): a lane whose upstream cell is on the outer layer (never
+  // the case for a valid cell) loads its own cell instead and keeps zero.
 #pragma unroll
-    for (int q = 0; q < NV; ++q)
-      f[q] = in[base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
-  } else {
-#pragma unroll
-    for (int q = 0; q < NV; ++q)
-      f[q] = patch_outer(P, gx - kC[q][0], gy - kC[q][1], gz - kC[q][2])
-                 ? zero_pop<S>(q, rho0)
-                 : in[base + q * P.qs + i - kC[q][0] * sx - kC[q][1] * sy - kC[q][2] * sz];
+  for (int q = 0; q < NV; ++q) {
+    const bool outer = patch_outer(P, gx - kC[q][0], gy - kC[q][1], gz - kC[q][2]);
+    const int off = outer ? 0 : kC[q][0] * sx + kC[q][1] * sy + kC[q][2] * sz;
+    const S v = in[base + q * P.qs + i - off];
+    f[q] = outer ? zero_pop<S>(q, rho0) : v;
   }
-  return valid;
+  return patch_valid(P, gx, gy, gz);
 }
 
 // First fine substep's stream + the second substep's collision: pull, then
