@@ -32,10 +32,11 @@
 // these kernels on patched "middle" levels, which have both a parent and a
 // child: the fill-collide takes its coarse values from patches (PC) and
 // zeroes the cells under the child's mask (MASK); k_fine_stream_zero is the
-// middle level's Stream + ZeroInvalidComponents with the child's block
-// averages added into its valid cells; and k_fine_stream_avg<S, true> is the
-// same for the middle level's last step of its parent's step, with the block
-// averages of its result for the parent.
+// middle level's stream of its valid cells with the child's block averages
+// added into them; and k_fine_stream_avg<S, true> is the same plus
+// ZeroInvalidComponents for the middle level's last step of its parent's
+// step, with the block averages of its result for the parent.  Every pass
+// stores valid cells only.
 //
 // Storage type S is double (FP64, bit-exact with the oracle) or float (the
 // shifted deviation d = f - w rho0 of the FP32 path).  Either way the block
@@ -287,15 +288,18 @@ __device__ __forceinline__ void add_block_sums(S* f, int u, const double* __rest
   for (int q = 0; q < NV; ++q) f[q] = from_f64<S>(acc[q], q, rho0);
 }
 
-// A middle level's Stream + ZeroInvalidComponents (src/AmrSim.cpp:459-463):
-// pull over valid + 1 ghost layer, outer layer zero, then the ghost shell
-// keeps only populations that came from valid cells.  Fused with the second
-// half of SumFromFine into this level (src/AmrSim.cpp:464-466, 592-602): a
-// valid cell whose slot (per grown cell, -1 if none) names a touched cell u
-// adds the child's block averages [start[u], start[u+1]) to its streamed
-// value in FP64, in the same order as k_sum_apply.  `in` holds each patch at
-// element P.base - ishift (a group-sized region when ishift is the group's
-// first base); `out` at P.base.
+// A middle level's Stream + ZeroInvalidComponents (src/AmrSim.cpp:459-463)
+// in a step whose result needs no block averages: the pull of the valid
+// cells.  The ghost shell the reference computes (populations that came
+// from valid cells) is not formed: nothing reads a patched level's ghost
+// cells from its state (its next fill rewrites them, a finer level's fill
+// and every read-back take valid cells only).  Fused with the second half
+// of SumFromFine into this level (src/AmrSim.cpp:464-466, 592-602): a valid
+// cell whose slot (per grown cell, -1 if none) names a touched cell u adds
+// the child's block averages [start[u], start[u+1]) to its streamed value in
+// FP64, in the same order as k_sum_apply.  `in` holds each patch at element
+// P.base - ishift (a group-sized region when ishift is the group's first
+// base); `out` at P.base.
 template <typename S>
 __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ in, S* __restrict__ out,
                                                           const PatchDesc* __restrict__ patches, int first,
@@ -308,13 +312,11 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
+  if (!patch_valid(P, gx, gy, gz)) return;
   S f[NV];
-  if (fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f)) {
-    const int u = slot[P.fill_base + i];
-    if (u >= 0) add_block_sums(f, u, avg, avg_stride, start, contrib, rho0);
-  } else if (!patch_outer(P, gx, gy, gz)) {
-    zero_invalid(P, gx, gy, gz, rho0, f);
-  }
+  fine_pull(P, in, P.base - ishift, i, gx, gy, gz, rho0, f);
+  const int u = slot[P.fill_base + i];
+  if (u >= 0) add_block_sums(f, u, avg, avg_stride, start, contrib, rho0);
 #pragma unroll
   for (int q = 0; q < NV; ++q) out[P.base + q * P.qs + i] = f[q];
 }
@@ -331,11 +333,11 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
 // With SUM (a middle level's
 // last step of its parent's step) the stream is k_fine_stream_zero's, the
 // child's block averages (slot, cavg, start, contrib) added into valid
-// cells, and the block sums are those of the level's resulting state.
-// Without SUM (the finest level's second substep) the input's outer ghost
-// layer is not stored and reads as zero, and only valid cells are stored:
-// the ghost cells of the level's state are rewritten by the next
-// fill-collide before anything reads them.
+// cells, and the block sums are those of the level's resulting state, ghost
+// shell included.  Without SUM (the finest level's second substep) the
+// input's outer ghost layer is not stored and reads as zero.  Either way
+// only valid cells are stored: the ghost cells of a level's state are
+// rewritten by its next fill-collide before anything reads them.
 constexpr int kAvgBlocks = 32;
 
 template <typename S, bool SUM = false>
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     }
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      if (SUM || valid) out[P.base + q * P.qs + i] = f[q];
+      if (valid) out[P.base + q * P.qs + i] = f[q];
       part[q][child][lb] = f[q];
     }
   }
