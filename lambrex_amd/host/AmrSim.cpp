@@ -318,7 +318,6 @@ void AmrSim::MakeNewLevelFromCoarse(int level, double time, const std::vector<la
 void AmrSim::Regrid(int criterion, double threshold, int blocking, int buffer) {
   if (!sim_) throw lambrex::Error("Regrid before InitFromScratch");
   if (max_level_ < 1) throw lambrex::Error("Regrid: max_level must be >= 1");
-  if (ref_ratio_ != kRefRatio) throw lambrex::Error("Regrid: tag clustering refines by ratio 2 only");
   for (auto& t : static_tags_) t = lambrex::IBox{};
   for (int l = 0; l < max_level_ && l <= finest_level_; ++l) {
     const std::vector<lambrex::IBox> fba = RefineFromState(l, criterion, threshold, blocking, buffer);
@@ -363,7 +362,9 @@ std::vector<lambrex::IBox> AmrSim::RefineFromState(int level, int criterion, dou
     CalcHydroVars(level);
     const LevelHost& L = levels_.at(level);
     const lambrex::IBox B = lambrex::minimal_box(L.boxes);
-    const int dom[3] = {NX << level, NY << level, NZ << level};
+    int scale = 1;  // level's index space: the domain refined by every ratio below it
+    for (int l = 0; l < level; ++l) scale *= Ratio(l);
+    const int dom[3] = {NX * scale, NY * scale, NZ * scale};
     bool wrap[3];
     for (int d = 0; d < 3; ++d) {
       wrap[d] = B.lo[d] == 0 && B.hi[d] == dom[d] - 1;
@@ -439,17 +440,40 @@ std::vector<lambrex::IBox> AmrSim::RefineFromState(int level, int criterion, dou
           allowed[c] = inner ? 1 : 0;
         }
   }
+  // The clustering refines by 2 and chops at max_grid_size.  At another
+  // ratio r it runs without the chop (a max_grid_size above twice every
+  // extent), its boxes are taken back to this level's index space, chopped
+  // there at max_grid_size / r and refined by r, so every fine box is
+  // aligned to the ratio (as static refinement does).
+  const int r = Ratio(level);
+  const int mgs = r == kRefRatio ? max_grid_size_ : 2 * std::max({ext[0], ext[1], ext[2]}) + 2;
   int32_t n = 0;
   const uint8_t* al = allowed.empty() ? nullptr : allowed.data();
-  Check(lbx_cluster_tags_allowed(ext, tags.data(), al, blocking, buffer, max_grid_size_, 0, nullptr, &n));
+  Check(lbx_cluster_tags_allowed(ext, tags.data(), al, blocking, buffer, mgs, 0, nullptr, &n));
   std::vector<int32_t> lohi(6 * (size_t)std::max(n, 1));
-  if (n > 0) Check(lbx_cluster_tags_allowed(ext, tags.data(), al, blocking, buffer, max_grid_size_, n, lohi.data(), &n));
-  std::vector<lambrex::IBox> fba((size_t)n);
-  for (int b = 0; b < n; ++b)
+  if (n > 0) Check(lbx_cluster_tags_allowed(ext, tags.data(), al, blocking, buffer, mgs, n, lohi.data(), &n));
+  std::vector<lambrex::IBox> fba;
+  for (int b = 0; b < n; ++b) {
+    lambrex::IBox c;  // refined by 2 relative to org
     for (int d = 0; d < NDIMS; ++d) {
-      fba[b].lo[d] = lohi[6 * b + d] + 2 * org[d];
-      fba[b].hi[d] = lohi[6 * b + 3 + d] + 2 * org[d];
+      c.lo[d] = lohi[6 * b + d];
+      c.hi[d] = lohi[6 * b + 3 + d];
     }
+    if (r == kRefRatio) {
+      for (int d = 0; d < NDIMS; ++d) {
+        c.lo[d] += 2 * org[d];
+        c.hi[d] += 2 * org[d];
+      }
+      fba.push_back(c);
+      continue;
+    }
+    for (int d = 0; d < NDIMS; ++d) {
+      c.lo[d] = c.lo[d] / 2 + org[d];
+      c.hi[d] = (c.hi[d] + 1) / 2 - 1 + org[d];
+    }
+    for (const lambrex::IBox& piece : lambrex::max_size_chop(c, std::max(1, max_grid_size_ / r)))
+      fba.push_back(lambrex::refine(piece, r));
+  }
   return fba;
 }
 

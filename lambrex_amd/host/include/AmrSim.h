@@ -99,9 +99,8 @@ class AmrSim {
   // Refinement ratio between consecutive levels, the same on every level: 2
   // (the reference's, fixed in its constructor, src/AmrSim.cpp:766), 3 or 4.
   // Before InitFromScratch.  A level then takes `ratio` steps per step of
-  // its parent (dt / ratio, tau rescaled by ComputeDt at that ratio) and
-  // Regrid is unavailable (its clustering refines by 2; static refinement
-  // works at any ratio).
+  // its parent (dt / ratio, tau rescaled by ComputeDt at that ratio), and
+  // static refinement and Regrid make fine boxes aligned to the ratio.
   void SetRefRatio(int ratio);
   // Mask the coarse collide with the fine BoxArray instead of the coarse one
   // the reference passes to makeFineMask (src/AmrSim.cpp:419-428).

@@ -277,6 +277,56 @@ TEST_CASE("ThreeLevel Regrid from a density gradient") {
   sim.Iterate(1);
 }
 
+// Regrid at refinement ratio 3: the same pulse plane tags the same coarse
+// tiles, which are refined by 3 and chopped in the coarse index space, so
+// every fine box is aligned to the ratio; level 1's own tags then make a
+// level 2 aligned to 3 and properly nested inside it.
+TEST_CASE("Regrid at ratio 3") {
+  AmrSim sim(NX, NY, NZ, 2, {1, 1, 1}, 0.8, 0.8);
+  sim.SetRefRatio(3);
+  std::vector<double> rho((size_t)NX * NY * NZ, 1.0);
+  for (int j = 0; j < NY; ++j)
+    for (int k = 0; k < NZ; ++k) rho[((size_t)20 * NY + j) * NZ + k] = 1.01;  // plane x = 20, C order
+  sim.SetInitialDensity(rho);
+  sim.SetInitialVelocity(0.0);
+  sim.InitFromScratch(0.0);
+  sim.Regrid(LBX_TAG_DENSITY_GRADIENT, 1e-3);
+  REQUIRE(sim.finestLevel() >= 1);
+  // tags at x = 19 and 21, grown by 1: tiles of 4 cover coarse x in [16, 24)
+  auto ext = sim.GetExtent(1);
+  REQUIRE(ext.first[0] == 3 * 16);
+  REQUIRE(ext.second[0] == 3 * 24 - 1);
+  REQUIRE(ext.first[1] == 0);
+  REQUIRE(ext.second[1] == 3 * NY - 1);
+  REQUIRE(ext.second[2] == 3 * NZ - 1);
+  for (int l = 1; l <= sim.finestLevel(); ++l)
+    for (const lambrex::IBox& b : sim.boxArray(l))
+      for (int d = 0; d < NDIMS; ++d) {
+        REQUIRE(b.lo[d] % 3 == 0);
+        REQUIRE((b.hi[d] + 1) % 3 == 0);
+        REQUIRE(b.length(d) <= 32);
+      }
+  if (sim.finestLevel() == 2) {  // level 2 inside level 1, one level-1 cell from its x faces
+    auto e2 = sim.GetExtent(2);
+    REQUIRE(e2.first[0] >= 3 * (ext.first[0] + 1));
+    REQUIRE(e2.second[0] <= 3 * ext.second[0] - 1);
+  }
+  sim.CalcHydroVars(0);
+  sim.CalcHydroVars(1);
+  bool parent = true;
+  for (int i = ext.first[0]; i <= ext.second[0]; ++i)
+    parent &= sim.GetDensity(i, 7, 4, 1) == sim.GetDensity(i / 3, 7 / 3, 4 / 3, 0);
+  REQUIRE(parent);
+  sim.Iterate(2);
+  REQUIRE(sim.GetTimeStep(1) == 6);
+  sim.CalcHydroVars(1);
+  REQUIRE(std::isfinite(sim.GetDensity(3 * 20, 0, 0, 1)));
+  sim.Regrid(LBX_TAG_DENSITY_GRADIENT, 1e9);
+  REQUIRE(sim.finestLevel() == 0);
+  REQUIRE(sim.boxArray(1).empty());
+  sim.Iterate(1);
+}
+
 // SetPrecision(LBX_FP32_SHIFTED, 1.0) through the facade: a two-level run
 // tracks the same run in FP64 to 1e-6 in density on both levels, and
 // SetPrecision after InitFromScratch is refused.
