@@ -639,7 +639,12 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
                                     const PatchDesc* own, int first, int npatch, int max_cells,
                                     const long long* fill, const unsigned char* mask, const Relax& r,
                                     hipStream_t st, long long oshift) {
-  const dim3 grid((max_cells + 255) / 256, npatch);
+  // Workgroups go to the 8 XCDs round-robin by linear id, so with a
+  // multiple of 8 per patch, chunk j of every patch runs on XCD j % 8: the
+  // rows whose first or last two cells an x-neighbour's ghost fill reads are
+  // then fetched into that XCD's L2 by both patches' workgroups at about the
+  // same time.  The extra workgroups exit at once.
+  const dim3 grid(((max_cells + 255) / 256 + 7) / 8 * 8, npatch);
   if (precision == 0)
     return fill_collide((const double*)fnow, (double*)fnext, (const double*)cnow, clat, cpatches, patches, own,
                         first, fill, mask, r, oshift, grid, st);
