@@ -93,6 +93,16 @@ __device__ __forceinline__ int div_small(int n, int d) {
   return q;
 }
 
+// Chunk of its patch handled by this workgroup.  Workgroups go to the 8
+// XCDs round-robin by linear id; with gridDim.x a multiple of 8, XCD k takes
+// chunks [k G/8, (k+1) G/8) of every patch (G = gridDim.x): a contiguous
+// slab, so neighbouring chunks, and the same rows of neighbouring patches,
+// share that XCD's L2.  The identity for any other grid.
+__device__ __forceinline__ int xcd_chunk() {
+  const int G = gridDim.x, c = blockIdx.x;
+  return (G & 7) ? c : (c & 7) * (G >> 3) + (c >> 3);
+}
+
 // (x, y, z) of index i < n of a box with rows of n0 and planes of n1 rows;
 // n is uniform over the workgroup, so the branch is too.
 __device__ __forceinline__ void split3(int i, int n, int n0, int n1, int& x, int& y, int& z) {
@@ -146,7 +156,7 @@ __global__ __launch_bounds__(256) void k_fine_fill_collide(
   // loads of base / qs from P, patches[s] and cpatches[] into one load through
   // a selected pointer, which kept the copy in 80 B of scratch per lane.
   const PatchDesc& P = own[first + blockIdx.y];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = xcd_chunk() * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
@@ -245,7 +255,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_collide(const S* __restrict
                                                              const PatchDesc* __restrict__ patches, int first,
                                                              Relax r, long long oshift) {
   const PatchDesc P = patches[first + blockIdx.y];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = xcd_chunk() * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
@@ -308,7 +318,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
                                                           const int* __restrict__ start,
                                                           const int* __restrict__ contrib, long long ishift) {
   const PatchDesc P = patches[first + blockIdx.y];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = xcd_chunk() * 256 + threadIdx.x;
   if (i >= P.cells) return;
   int gx, gy, gz;
   patch_coords(P, i, gx, gy, gz);
@@ -354,7 +364,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
   const PatchDesc P = patches[first + blockIdx.y];
   const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
   const int nblk = c0 * c1 * c2;
-  const int b0 = blockIdx.x * kAvgBlocks;  // first block of this workgroup
+  const int b0 = xcd_chunk() * kAvgBlocks;  // first block of this workgroup
   if (b0 >= nblk) return;                  // uniform over the workgroup
   const int child = threadIdx.x / kAvgBlocks, lb = threadIdx.x % kAvgBlocks;
   const int u = b0 + lb;
@@ -639,9 +649,9 @@ hipError_t launch_fine_fill_collide(int precision, const void* fnow, void* fnext
                                     const PatchDesc* own, int first, int npatch, int max_cells,
                                     const long long* fill, const unsigned char* mask, const Relax& r,
                                     hipStream_t st, long long oshift) {
-  // Workgroups go to the 8 XCDs round-robin by linear id, so with a
-  // multiple of 8 per patch, chunk j of every patch runs on XCD j % 8: the
-  // rows whose first or last two cells an x-neighbour's ghost fill reads are
+  // A multiple of 8 workgroups per patch, so that xcd_chunk() gives each XCD
+  // the same slab of every patch: the rows whose first or last two cells an
+  // x-neighbour's ghost fill reads (and the rows a y-neighbour's reads) are
   // then fetched into that XCD's L2 by both patches' workgroups at about the
   // same time.  The extra workgroups exit at once.
   const dim3 grid(((max_cells + 255) / 256 + 7) / 8 * 8, npatch);
@@ -656,7 +666,7 @@ hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, con
                                    int npatch, int max_cells, double rho0, const int* slot, const double* avg,
                                    long long avg_stride, const int* start, const int* contrib, hipStream_t st,
                                    long long ishift) {
-  const dim3 grid((max_cells + 255) / 256, npatch);
+  const dim3 grid(((max_cells + 255) / 256 + 7) / 8 * 8, npatch);  // for xcd_chunk()
   if (precision == 0)
     k_fine_stream_zero<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, rho0, slot,
                                                      avg, avg_stride, start, contrib, ishift);
@@ -670,7 +680,7 @@ hipError_t launch_middle_stream_avg(int precision, const void* in, long long ish
                                    long long avg_stride, const PatchDesc* patches, int first, int npatch,
                                    int max_blocks, double rho0, const int* slot, const double* cavg,
                                    long long cavg_stride, const int* start, const int* contrib, hipStream_t st) {
-  const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
+  const dim3 grid(((max_blocks + kAvgBlocks - 1) / kAvgBlocks + 7) / 8 * 8, npatch);  // for xcd_chunk()
   if (precision == 0)
     k_fine_stream_avg<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride,
                                                                 patches, first, rho0, ishift, slot, cavg, cavg_stride,
@@ -685,7 +695,7 @@ hipError_t launch_middle_stream_avg(int precision, const void* in, long long ish
 hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, long long oshift,
                                       const PatchDesc* patches, int first, int npatch, int max_cells, const Relax& r,
                                       hipStream_t st, bool inner) {
-  const dim3 grid((max_cells + 255) / 256, npatch);
+  const dim3 grid(((max_cells + 255) / 256 + 7) / 8 * 8, npatch);  // for xcd_chunk()
   if (precision == 0 && inner)
     k_fine_stream_collide<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, patches, first, r,
                                                               oshift);
@@ -702,7 +712,7 @@ hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, 
 hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
                                   long long avg_stride, const PatchDesc* patches, int first, int npatch,
                                   int max_blocks, double rho0, hipStream_t st) {
-  const dim3 grid((max_blocks + kAvgBlocks - 1) / kAvgBlocks, npatch);
+  const dim3 grid(((max_blocks + kAvgBlocks - 1) / kAvgBlocks + 7) / 8 * 8, npatch);  // for xcd_chunk()
   if (precision == 0)
     k_fine_stream_avg<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride, patches, first,
                                                     rho0, ishift);
