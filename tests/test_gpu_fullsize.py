@@ -75,10 +75,13 @@ def test_256_cubed_fp32_shifted_three_steps_vs_fp64(ctx):
 
 
 @pytest.mark.parametrize("mask_kind", ["fine", "reference"])
-def test_two_level_workload_one_coarse_step_bitwise(ctx, mask_kind):
+def test_two_level_workload_two_coarse_steps_bitwise(ctx, mask_kind):
     """mask_kind "fine": the benchmark's mask (coarse cells under the fine
     level); "reference": MakeFineMask's own, the coarse BoxArray passed as the
-    fine one (src/AmrSim.cpp:419-428), which masks coarsen(coarse boxes, 2)."""
+    fine one (src/AmrSim.cpp:419-428), which masks coarsen(coarse boxes, 2).
+    Two coarse steps in one call: the fine level's 16 patch groups run in
+    storage order in the first and in reverse order in the second, each
+    step from its own captured graph."""
     sys.path.insert(0, str(ROOT))
     import bench
     dims, cboxes, fboxes, mask, rho, u, _ = bench.two_level_geometry()
@@ -93,7 +96,7 @@ def test_two_level_workload_one_coarse_step_bitwise(ctx, mask_kind):
         sim.set_tau(1, 2 * (0.8 - 0.5) + 0.5)
         sim.fill_from_coarse(1)
         sim.set_fine_mask(mask)
-        sim.iterate(1)
+        sim.iterate(2)
         fg0 = sim.get_f(0)
         fg1, _ = sim.get_f_bbox(1)
     finally:
@@ -101,7 +104,7 @@ def test_two_level_workload_one_coarse_step_bitwise(ctx, mask_kind):
     o = orc.Sim(*dims, 0.8, boxes=lists(cboxes))
     o.init(*c_order(rho, u))
     o.add_fine(lists(fboxes), lists(mask))
-    o.iterate(1)
+    o.iterate(2)
     fo0, fo1 = o.get_f(0), o.get_f(1)
     o.close()
     assert same_bits(fo0, fg0)
