@@ -470,6 +470,44 @@ def test_refinement_ratio_two_level_bitwise(ctx, r, mask_kind):
     assert_bitwise(fo1, fg1, f"fine f, ratio {r}")
 
 
+def test_refinement_ratio_3_patch_groups_bitwise(ctx):
+    """Ratio 3 with a fine level large enough for two launch groups (32
+    boxes of 24^3 over coarse x in [8, 24) of a 32^3 domain, about 84 MB of
+    FP64 state per buffer): the groups alternate between the two streams and
+    run in reverse order on every other coarse step.  Both levels after 3
+    coarse steps equal the oracle's, bit for bit."""
+    r, steps, tau = 3, 3, 0.8
+    dims = (32, 32, 32)
+    cboxes = workloads.chop(dims, 16)
+    fboxes = fine_boxes_in((3 * 8, 0, 0), (3 * 24 - 1, 3 * 32 - 1, 3 * 32 - 1), 24)
+    assert len(fboxes) == 32
+    rho, u = workloads.random_state(dims, seed=113, rho_sigma=1e-2, u_amp=0.02)
+    tau_f = r * (tau - 0.5) + 0.5  # ComputeDt at ratio r, src/AmrSim.cpp:313-314
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, tau)
+        sim.init_from_fields(rho, u)
+        sim.set_ref_ratio(r)
+        sim.define_level(1, fboxes)
+        sim.set_tau(1, tau_f)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(fboxes)
+        sim.iterate(steps)
+        fg0, fg1 = sim.get_f(0), sim.get_f_bbox(1)[0]
+    finally:
+        sim.close()
+    o = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in cboxes])
+    o.init(*c_order(rho, u))
+    o.set_ratio(r)
+    o.add_fine([(list(a), list(b)) for a, b in fboxes], [(list(a), list(b)) for a, b in fboxes])
+    o.iterate(steps)
+    fo0, fo1 = o.get_f(0), o.get_f(1)
+    o.close()
+    assert_bitwise(fo0, fg0, "coarse f, ratio 3")
+    assert_bitwise(fo1, fg1, "fine f, ratio 3")
+
+
 def test_refinement_ratio_3_fp32_tracks_fp64(ctx):
     """FP32-shifted storage at ratio 3 follows the FP64 path from the same
     state: densities of both levels within 1e-6 relative after 3 steps."""
