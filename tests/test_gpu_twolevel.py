@@ -508,6 +508,39 @@ def test_refinement_ratio_3_patch_groups_bitwise(ctx):
     assert_bitwise(fo1, fg1, "fine f, ratio 3")
 
 
+def test_large_patch_bitwise(ctx):
+    """One fine box of 160^3 (a 164^3 grown patch: 4.4 M cells, above the
+    2^22 up to which the kernels split a patch index with the float
+    reciprocal) over a coarse 88^3 domain: both levels after one coarse step
+    equal the oracle's, bit for bit."""
+    dims = (88, 88, 88)
+    cboxes = workloads.chop(dims, 44)
+    fboxes = [((8, 8, 8), (167, 167, 167))]
+    rho, u = workloads.random_state(dims, seed=127, rho_sigma=1e-2, u_amp=0.02)
+    tau = 0.8
+    sim = lbx.Sim(ctx, dims)
+    try:
+        sim.define_level(0, cboxes)
+        sim.set_tau(0, tau)
+        sim.init_from_fields(rho, u)
+        sim.define_level(1, fboxes)
+        sim.set_tau(1, 2 * (tau - 0.5) + 0.5)
+        sim.fill_from_coarse(1)
+        sim.set_fine_mask(fboxes)
+        sim.iterate(1)
+        fg0, fg1 = sim.get_f(0), sim.get_f_bbox(1)[0]
+    finally:
+        sim.close()
+    o = orc.Sim(*dims, tau, boxes=[(list(a), list(b)) for a, b in cboxes])
+    o.init(*c_order(rho, u))
+    o.add_fine([(list(a), list(b)) for a, b in fboxes], [(list(a), list(b)) for a, b in fboxes])
+    o.iterate(1)
+    fo0, fo1 = o.get_f(0), o.get_f(1)
+    o.close()
+    assert_bitwise(fo0, fg0, "coarse f")
+    assert_bitwise(fo1, fg1, "fine f")
+
+
 def test_refinement_ratio_3_fp32_tracks_fp64(ctx):
     """FP32-shifted storage at ratio 3 follows the FP64 path from the same
     state: densities of both levels within 1e-6 relative after 3 steps."""
