@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void k_sum_apply(const double* __restrict__ av
                                                    const long long* __restrict__ cell,
                                                    const int* __restrict__ start,
                                                    const int* __restrict__ contrib, int ncell, double rho0) {
-  const int u = blockIdx.x * 256 + threadIdx.x;
+  const int u = xcd_chunk() * 256 + threadIdx.x;
   if (u >= ncell) return;
   const long long c = cell[u], stride = L.ps;
   double acc[NV];
@@ -737,7 +737,7 @@ hipError_t launch_sum_apply(int precision, const double* avg, long long avg_stri
                             const long long* cell, const int* start, const int* contrib, int ncell, double rho0,
                             hipStream_t st) {
   if (ncell <= 0) return hipSuccess;
-  const int blocks = (ncell + 255) / 256;
+  const int blocks = ((ncell + 255) / 256 + 7) / 8 * 8;  // for xcd_chunk()
   if (precision == 0)
     k_sum_apply<double><<<blocks, 256, 0, st>>>(avg, avg_stride, (double*)cnext, L, cell, start, contrib, ncell,
                                                 rho0);
