@@ -335,11 +335,11 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
 // first half of SumFromFine: the 1/8-weighted sum of each 2x2x2 block of the
 // grown patch (= one coarse cell of the coarsened box grown by 1), children
 // added in (k, j, i) order in FP64, to avg[q * avg_stride + avg_base + block].
-// A workgroup takes 32 consecutive blocks; thread t computes child t / 32 of
-// block t % 32, so each wavefront reads whole x-pairs of its blocks.  The
-// children meet in LDS for the ordered sums, as stored values (S): an FP32
-// workgroup then needs half the LDS of an FP64 one, and each child is
-// converted to FP64 f as it is added (the same bits as converting first).
+// A workgroup takes B = avg_blocks<S>() consecutive blocks; thread t
+// computes child t / B of block t % B, so each wavefront reads whole x-pairs
+// of its blocks.  The children meet in LDS for the ordered sums, as stored
+// values (S), and each child is converted to FP64 f as it is added (the
+// same bits as converting first).
 // With SUM (a middle level's
 // last step of its parent's step) the stream is k_fine_stream_zero's, the
 // child's block averages (slot, cavg, start, contrib) added into valid
@@ -348,7 +348,10 @@ __global__ __launch_bounds__(256) void k_fine_stream_zero(const S* __restrict__ 
 // input's outer ghost layer is not stored and reads as zero.  Either way
 // only valid cells are stored: the ghost cells of a level's state are
 // rewritten by its next fill-collide before anything reads them.
-constexpr int kAvgBlocks = 32;
+// Blocks per workgroup (8 threads each): 32 in FP64; 16 in FP32, whose
+// passes are shorter, so that a barrier spans two waves instead of four.
+template <typename S>
+constexpr int avg_blocks() { return sizeof(S) == 4 ? 16 : 32; }
 
 template <typename S, bool SUM = false>
 __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ in, S* __restrict__ out,
@@ -360,6 +363,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
                                                          long long cavg_stride = 0,
                                                          const int* __restrict__ start = nullptr,
                                                          const int* __restrict__ contrib = nullptr) {
+  constexpr int kAvgBlocks = avg_blocks<S>();
   __shared__ S part[NV][8][kAvgBlocks];
   const PatchDesc P = patches[first + blockIdx.y];
   const int c0 = P.g[0] >> 1, c1 = P.g[1] >> 1, c2 = P.g[2] >> 1;
@@ -392,7 +396,7 @@ __global__ __launch_bounds__(256) void k_fine_stream_avg(const S* __restrict__ i
     }
   }
   __syncthreads();
-  for (int task = threadIdx.x; task < NV * kAvgBlocks; task += 256) {
+  for (int task = threadIdx.x; task < NV * kAvgBlocks; task += 8 * kAvgBlocks) {
     const int q = task / kAvgBlocks, b = task % kAvgBlocks;
     if (b0 + b >= nblk) continue;
     double sum = 0.0;
@@ -676,19 +680,26 @@ hipError_t launch_fine_stream_zero(int precision, const void* in, void* out, con
   return hipGetLastError();
 }
 
+// Grid of k_fine_stream_avg<S>: a multiple of 8 workgroups per patch, for
+// xcd_chunk(); each workgroup has 8 avg_blocks<S>() threads.
+template <typename S>
+dim3 avg_grid(int max_blocks, int npatch) {
+  constexpr int ab = avg_blocks<S>();
+  return dim3(((max_blocks + ab - 1) / ab + 7) / 8 * 8, npatch);
+}
+
 hipError_t launch_middle_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
                                    long long avg_stride, const PatchDesc* patches, int first, int npatch,
                                    int max_blocks, double rho0, const int* slot, const double* cavg,
                                    long long cavg_stride, const int* start, const int* contrib, hipStream_t st) {
-  const dim3 grid(((max_blocks + kAvgBlocks - 1) / kAvgBlocks + 7) / 8 * 8, npatch);  // for xcd_chunk()
   if (precision == 0)
-    k_fine_stream_avg<double, true><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride,
-                                                                patches, first, rho0, ishift, slot, cavg, cavg_stride,
-                                                                start, contrib);
+    k_fine_stream_avg<double, true><<<avg_grid<double>(max_blocks, npatch), 8 * avg_blocks<double>(), 0, st>>>(
+        (const double*)in, (double*)out, avg, avg_stride, patches, first, rho0, ishift, slot, cavg, cavg_stride,
+        start, contrib);
   else
-    k_fine_stream_avg<float, true><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride,
-                                                               patches, first, rho0, ishift, slot, cavg, cavg_stride,
-                                                               start, contrib);
+    k_fine_stream_avg<float, true><<<avg_grid<float>(max_blocks, npatch), 8 * avg_blocks<float>(), 0, st>>>(
+        (const float*)in, (float*)out, avg, avg_stride, patches, first, rho0, ishift, slot, cavg, cavg_stride, start,
+        contrib);
   return hipGetLastError();
 }
 
@@ -712,13 +723,12 @@ hipError_t launch_fine_stream_collide(int precision, const void* in, void* out, 
 hipError_t launch_fine_stream_avg(int precision, const void* in, long long ishift, void* out, double* avg,
                                   long long avg_stride, const PatchDesc* patches, int first, int npatch,
                                   int max_blocks, double rho0, hipStream_t st) {
-  const dim3 grid(((max_blocks + kAvgBlocks - 1) / kAvgBlocks + 7) / 8 * 8, npatch);  // for xcd_chunk()
   if (precision == 0)
-    k_fine_stream_avg<double><<<grid, 256, 0, st>>>((const double*)in, (double*)out, avg, avg_stride, patches, first,
-                                                    rho0, ishift);
+    k_fine_stream_avg<double><<<avg_grid<double>(max_blocks, npatch), 8 * avg_blocks<double>(), 0, st>>>(
+        (const double*)in, (double*)out, avg, avg_stride, patches, first, rho0, ishift);
   else
-    k_fine_stream_avg<float><<<grid, 256, 0, st>>>((const float*)in, (float*)out, avg, avg_stride, patches, first,
-                                                   rho0, ishift);
+    k_fine_stream_avg<float><<<avg_grid<float>(max_blocks, npatch), 8 * avg_blocks<float>(), 0, st>>>(
+        (const float*)in, (float*)out, avg, avg_stride, patches, first, rho0, ishift);
   return hipGetLastError();
 }
 
